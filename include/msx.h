@@ -1,0 +1,197 @@
+/*
+ * msx.h -- C ABI of the msx MI355X multislice engine (libmsx.so).
+ *
+ * This is the drop-in boundary for the reference's multislice hot path:
+ * the projected-potential build (LMultiSlice ModelPotential), the transmission
+ * function, the slice loop (LMultiSlice ModelSimulated / Kirkland mulslice),
+ * STEM probes and annular detectors (Musli), and the objective-lens image.
+ * Every entry point names the reference interface it replaces.
+ *
+ * Conventions (all entry points):
+ *   - Plain C types and pointers only; no C++ or framework types.
+ *   - Return 0 on success, a negative MSX_E* code on failure; msx_last_error()
+ *     returns a per-thread description of the last failure.  Nothing calls
+ *     exit() or waits for input (the reference's CUERR macro did,
+ *     LMultiSlice/stdafx.h:72-76).
+ *   - Images are complex, stored x-major with y fastest, element (ix,iy) at
+ *     offset iy + ix*ny, (re,im) interleaved -- the layout of the reference's
+ *     cfpix (multislice/cfpix.hpp:99-126).  Batches of images are contiguous.
+ *   - prec = MSX_F32 (complex64 / float) or MSX_F64 (complex128 / double) for
+ *     every complex and real buffer of the call.
+ *   - where = MSX_HOST: buffers are host memory, the call copies in and out and
+ *     returns when the result is in host memory (as the reference's synchronous
+ *     calls do).  where = MSX_DEVICE: buffers are device memory on the
+ *     context's GPU; work is queued on the context stream and the call returns
+ *     after that stream has completed it.
+ *   - FFT sign: the "forward" transform (real space -> k-space) has exponent
+ *     +2*pi*i*k.x and is unscaled; the inverse has exponent -2*pi*i*k.x and is
+ *     scaled by 1/(nx*ny) -- cfpix::fft()/ifft() (cfpix.cpp:115-145).
+ *   - Units: Angstrom, keV, radians.  Potentials in volt*Angstrom.
+ *   - Grid sizes per axis: 16, 32, 64, 128, 256, 384, 512, 768, 1024, 2048.
+ */
+#ifndef MSX_H
+#define MSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSX_API __attribute__((visibility("default")))
+
+enum { MSX_F32 = 0, MSX_F64 = 1 };
+enum { MSX_HOST = 0, MSX_DEVICE = 1 };
+enum {
+    MSX_OK = 0,
+    MSX_EINVAL = -1,   /* invalid argument (sizes, Z, pointers) */
+    MSX_ENOMEM = -2,   /* device allocation failed */
+    MSX_EHIP = -3,     /* HIP runtime error */
+    MSX_ESIZE = -4,    /* unsupported grid size */
+    MSX_ESTATE = -5,   /* context misuse */
+};
+/* fparams tables: 0 = LMultiSlice / FPARAMS.DAT fit (ModelPotential.h:80-595),
+ * 1 = Kirkland slicelib fit (multislice/slicelib.cpp:903-2138). */
+enum { MSX_TABLE_LMULTISLICE = 0, MSX_TABLE_KIRKLAND = 1 };
+/* projected-potential formula: 0 = Kirkland vzatom (slicelib.cpp:2666-2699),
+ * 1 = LMultiSlice calculateProjectedPotential (ModelPotential.cu:232-254). */
+enum { MSX_FORMULA_KIRKLAND = 0, MSX_FORMULA_LMULTISLICE = 1 };
+
+typedef struct msx_ctx_s* msx_ctx;
+
+/* ---- context ---------------------------------------------------------- */
+/* Replaces the device selection of Dispatcher::Run (Dispatcher.cu:191-207):
+ * binds one GPU and creates the context's stream.  One context per host
+ * thread; calls on one context are serialised on its stream. */
+MSX_API int msx_create(int device, msx_ctx* out);
+MSX_API int msx_destroy(msx_ctx ctx);
+/* Use an external HIP stream (hipStream_t passed as void*; NULL restores the
+ * context's own stream). */
+MSX_API int msx_set_stream(msx_ctx ctx, void* hip_stream);
+MSX_API const char* msx_last_error(void);
+MSX_API int msx_version(void);
+
+/* ---- scalars ---------------------------------------------------------- */
+/* Microscope::wavelength / Sigma (Microscope.cpp:69-86) = Kirkland
+ * wavelength / sigma (slicelib.cpp:2795-2806, 2387-2401). */
+MSX_API double msx_wavelength(double kev);
+MSX_API double msx_sigma(double kev);
+/* calculateProjectedPotential (ModelPotential.cu:232-254) or vzatom
+ * (slicelib.cpp:2666-2699): projected potential of atom Z at radius r. */
+MSX_API double msx_vzatom(int Z, double r, int table, int formula);
+/* featom (slicelib.cpp:350-374): electron scattering factor f_e(k^2). */
+MSX_API double msx_featom(int Z, double k2, int table);
+
+/* ---- atoms ------------------------------------------------------------ */
+/* One atom: the fields of AModel::Record (amodel/include/BaseStructures.h:
+ * 120-135) the hot path uses, with Cartesian coordinates in Angstrom
+ * (Record holds fractional xsCoordinate; x = xs.x * a etc.). */
+typedef struct msx_atom {
+    int32_t Z;   /* atomic number 1..103 */
+    float occ;   /* occupancy weight (LMultiSlice ignores it: use 1) */
+    double x, y, z;
+} msx_atom;
+
+/* ---- 2D FFT ----------------------------------------------------------- */
+/* cfpix::fft() / cfpix::ifft() (cfpix.cpp:115-145), batched: sign = +1 is
+ * fft() (exponent +i, unscaled), sign = -1 is ifft() (exponent -i, scaled by
+ * 1/(nx*ny)).  In place on nbatch images. */
+MSX_API int msx_fft2d(msx_ctx ctx, void* data, int nbatch, int nx, int ny, int sign, int prec, int where);
+
+/* ---- projected potential ---------------------------------------------- */
+/* ModelPotential::calculatePotentialGrid (ModelPotential.cu:27-99) with the
+ * per-pixel sum of calculatePotentialGridGPU (:125-230):
+ *   atom slice kz: kz*dz <= z < (kz+1)*dz, dz = c/nslices;
+ *   pixel (ix,iy) at (ix*a/nx, iy*b/ny); minimum image per axis;
+ *   atoms with r < radius contribute occ*vz(r) (r clamped >= 1e-10), summed
+ *   in FP64.
+ * V: nslices images of nx*ny reals [kz][ix][iy] (float for MSX_F32, double for
+ * MSX_F64), host or device per `where`. */
+MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c,
+                          int nx, int ny, int nslices, double radius, int table, int formula, void* V,
+                          int prec, int where);
+
+/* ---- transmission ----------------------------------------------------- */
+/* Transmission functions of nlayer potential layers (mulslice.cpp:414-421 and
+ * the anti-aliasing pass :515-522; LMultiSlice phaseObject kernel.cu:4-18):
+ *   t = exp(i*sigma(kev)*V/1000), then if bw > 0: fft, zero every pixel with
+ *   kx2+ky2 >= k2max, ifft; k2max = (bw*min(nx/2ax, ny/2by))^2 in float and
+ *   kx2, ky2 the float tables of freqn (slicelib.cpp:433-450).
+ * V: nlayer real images; t: nlayer complex images. */
+MSX_API int msx_transmission(msx_ctx ctx, const void* V, int nlayer, int nx, int ny, double ax, double by,
+                             double kev, double bw, void* t, int prec, int where);
+
+/* ---- slice stack -------------------------------------------------------- */
+/* The specimen/optics description of one multislice run: the layers, their
+ * thicknesses and the stacking sequence of mulslice.cpp:190-206 (parlay). */
+typedef struct msx_stack {
+    int32_t nx, ny;          /* grid */
+    double ax, by;           /* supercell size (Angstrom) */
+    double kev;              /* beam energy */
+    double ctilt_x, ctilt_y; /* crystal tilt (rad), propagator term of mulslice.cpp:494-513 */
+    double bw;               /* band-limit fraction of Nyquist (Kirkland BW = 2/3) */
+    int32_t nlayer;          /* number of distinct transmission layers */
+    const double* cz;        /* host array [nlayer]: layer thickness (Angstrom) */
+    int32_t nslice;          /* number of slices */
+    const int32_t* layer;    /* host array [nslice]: layer index of each slice */
+} msx_stack;
+
+/* ---- multislice (HRTEM / any incident waves) ------------------------------ */
+/* The coherent slice loop of mulslice.cpp:703-739 (ModelSimulated::
+ * imageCalculation, ModelSimulated.cu:52-119, without its k^4/fftshift/1/nx
+ * quirks): for each slice: wave *= t[layer]; fft; multiply by the Fresnel
+ * propagator exp(-i pi dz (lambda k^2 - k.2tan(tilt))) inside the band limit
+ * (strict kx2+ky2 < k2max), zero outside; ifft.
+ * trans: nlayer complex images (host or device per `where`).
+ * psi: nwave complex images, incident waves in, exit waves out.  If psi0 is
+ * not NULL it is the incident wave for every wave (one image, e.g. a plane
+ * wave) and psi is output only. */
+MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
+                           void* psi, int nwave, int prec, int where);
+
+/* ---- STEM --------------------------------------------------------------- */
+/* Probe of Musli ConvergedBeam/Xi (MSProject.cs:458-514) in Kirkland's
+ * conventions: Psi0(k) = A exp(-i chi(k)) exp(+2 pi i k.r_p) for
+ * kmin2 <= kx2+ky2 <= kmax2 (kmax = alpha_max/lambda), chi = pi lambda k^2
+ * (0.5 Cs lambda^2 k^2 - df) (slicelib.cpp:258-318), normalised to
+ * sum |psi0(x)|^2 = 1.  Cs and df in Angstrom. */
+typedef struct msx_probe {
+    double alpha_max, alpha_min; /* aperture (rad) */
+    double cs, df;               /* spherical aberration, defocus (Angstrom) */
+} msx_probe;
+/* Annular detector (SaveSTEM, MSProject.cs:792-842): k_inner <= |k| <=
+ * k_outer in 1/Angstrom (theta = lambda*k). */
+typedef struct msx_detector {
+    double k_inner, k_outer;
+} msx_detector;
+
+/* STEM scan (MSProjectUseOnce.StartCalculations / MSUseOnceCalculator.
+ * RunCalculation, MSProjectUseOnce.cs:100-291, MSUseOnceCalculator.cs:
+ * 125-277): every probe position runs the slice loop above; after the
+ * propagator of each checkpoint slice the detector sums
+ *   out[ck][d][p] += weight * sum_{annulus d} |Psi(k)|^2 / (nx*ny)
+ * are taken (the fraction of the incident intensity; Musli sums unnormalised
+ * values, MSProject.cs:834-838).  Probes run `batch` at a time (0 = choose).
+ * pos: host [nprobe][2] (x, y) in Angstrom; checkpoints: host [nck] slice
+ * counts 1..nslice, increasing; out: host [nck][ndet][nprobe] doubles,
+ * overwritten if accumulate == 0, else added to.  trans per `where`. */
+MSX_API int msx_stem(msx_ctx ctx, const msx_stack* stack, const void* trans, const msx_probe* probe,
+                     const double* pos, int nprobe, const msx_detector* det, int ndet,
+                     const int32_t* checkpoints, int nck, double weight, int accumulate, double* out,
+                     int batch, int prec, int where);
+
+/* ---- objective-lens image ------------------------------------------------ */
+/* Image of exit waves through the objective lens (mulslice.cpp:612-656 at one
+ * defocus; LMultiSlice objectLens kernel.cu:49-64 with the Kirkland chi):
+ * Psi = fft(psi); Psi *= exp(-i chi(k)) for kx2+ky2 <= (aobj/lambda)^2, else
+ * 0; psi_img = ifft(Psi).  psi_img (complex) and/or intensity (real |psi_img|^2)
+ * may be NULL.  Cs and df in Angstrom, aobj in rad. */
+MSX_API int msx_ctf_image(msx_ctx ctx, const void* psi, int nwave, int nx, int ny, double ax, double by,
+                          double kev, double cs, double df, double aobj, void* psi_img, void* intensity,
+                          int prec, int where);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSX_H */

@@ -1,0 +1,834 @@
+// msx_api.hip -- the C ABI of include/msx.h: context, device tables, and the
+// host-side orchestration of the sweep kernels for each reference entry point.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/msx.h"
+#include "msx_internal.h"
+#include "fparams_tables.inc"
+
+using namespace msx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(e_ == hipErrorOutOfMemory ? MSX_ENOMEM : MSX_EHIP, "%s: %s (%s:%d)", #x,     \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                                  \
+    } while (0)
+
+#define CHK(x)                   \
+    do {                         \
+        int r_ = (x);            \
+        if (r_ != MSX_OK) return r_; \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= n) return MSX_OK;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(MSX_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+        }
+        n = bytes;
+        return MSX_OK;
+    }
+    void release() {
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+// radix-independent twiddle table exp(+2 pi i m / n), m < n, from FP64
+template <typename C> std::vector<C> make_twiddles(int n) {
+    std::vector<C> w(n);
+    for (int m = 0; m < n; m++) {
+        // angle reduced to [0, pi/4] by symmetry before the libm call
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        w[m].x = (decltype(w[m].x))std::cos((double)a);
+        w[m].y = (decltype(w[m].y))std::sin((double)a);
+    }
+    return w;
+}
+
+// freqn (slicelib.cpp:433-450) float k^2 table
+std::vector<float> k2_table(int n, double a) {
+    std::vector<float> k2(n);
+    int imid = (int)(n / 2.0 + 0.5);
+    for (int i = 0; i < n; i++) {
+        float k = (i > imid) ? ((float)(i - n)) / ((float)a) : ((float)i) / ((float)a);
+        k2[i] = k * k;
+    }
+    return k2;
+}
+int fidx(int i, int n) {
+    int imid = (int)(n / 2.0 + 0.5);
+    return i > imid ? i - n : i;
+}
+// mulslice.cpp:481-489 (float arithmetic)
+float k2max_of(int nx, int ny, double ax, double by, double bw) {
+    float k2max = nx / (2.0F * (float)ax);
+    float t = ny / (2.0F * (float)by);
+    if (t < k2max) k2max = t;
+    k2max = (float)bw * k2max;
+    return k2max * k2max;
+}
+
+}  // namespace
+
+struct msx_ctx_s {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::map<int, DevBuf> tw32, tw64;
+    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp;
+
+    template <typename C> int twiddles(int n, const C** out) {
+        auto& m = (sizeof(C) == 8) ? tw32 : tw64;
+        DevBuf& b = m[n];
+        if (!b.p) {
+            std::vector<C> w = make_twiddles<C>(n);
+            CHK(b.ensure(sizeof(C) * n));
+            HIPCHK(hipMemcpy(b.p, w.data(), sizeof(C) * n, hipMemcpyHostToDevice));
+        }
+        *out = static_cast<const C*>(b.p);
+        return MSX_OK;
+    }
+    ~msx_ctx_s() {
+        for (auto& kv : tw32) kv.second.release();
+        for (auto& kv : tw64) kv.second.release();
+        for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp}) b->release();
+        if (own) hipStreamDestroy(own);
+    }
+};
+
+namespace {
+
+int check_grid(int nx, int ny) {
+    if (!supported_n(nx) || !supported_n(ny))
+        return fail(MSX_ESIZE, "grid %d x %d: each axis must be one of 16,32,64,128,256,384,512,768,1024,2048", nx, ny);
+    return MSX_OK;
+}
+int check_prec_where(int prec, int where) {
+    if (prec != MSX_F32 && prec != MSX_F64) return fail(MSX_EINVAL, "prec must be MSX_F32 or MSX_F64");
+    if (where != MSX_HOST && where != MSX_DEVICE) return fail(MSX_EINVAL, "where must be MSX_HOST or MSX_DEVICE");
+    return MSX_OK;
+}
+
+// Per-call device tables for the slice loop: freqn k^2 tables and propagator
+// factors per layer, packed in ctx->tabs.
+template <typename C> struct SliceTables {
+    const float* kx2;
+    const float* ky2;
+    const C* px;  // [nlayer][nx]
+    const C* py;  // [nlayer][ny]
+    float k2max;
+};
+
+template <typename C>
+int build_tables(msx_ctx ctx, const msx_stack* st, bool with_prop, SliceTables<C>* out) {
+    const int nx = st->nx, ny = st->ny, nl = with_prop ? st->nlayer : 0;
+    std::vector<float> kx2 = k2_table(nx, st->ax), ky2 = k2_table(ny, st->by);
+    std::vector<C> px((size_t)nl * nx), py((size_t)nl * ny);
+    const double lam = msx_wavelength(st->kev);
+    const double tx = 2.0 * std::tan(st->ctilt_x), ty = 2.0 * std::tan(st->ctilt_y);
+    for (int l = 0; l < nl; l++) {
+        const double cz = st->cz[l];
+        // mulslice.cpp:494-513: t = pi*cz*(k^2*lambda - k*2tan(tilt)), P = (cos t, -sin t),
+        // evaluated in FP64 with k = index/a
+        for (int i = 0; i < nx; i++) {
+            const double k = (double)fidx(i, nx) / st->ax;
+            const double t = M_PI * cz * (k * k * lam - k * tx);
+            px[(size_t)l * nx + i].x = std::cos(t);
+            px[(size_t)l * nx + i].y = -std::sin(t);
+        }
+        for (int i = 0; i < ny; i++) {
+            const double k = (double)fidx(i, ny) / st->by;
+            const double t = M_PI * cz * (k * k * lam - k * ty);
+            py[(size_t)l * ny + i].x = std::cos(t);
+            py[(size_t)l * ny + i].y = -std::sin(t);
+        }
+    }
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_ky = al(sizeof(float) * nx), o_px = o_ky + al(sizeof(float) * ny),
+                 o_py = o_px + al(sizeof(C) * px.size()), tot = o_py + al(sizeof(C) * py.size() + 1);
+    CHK(ctx->tabs.ensure(tot));
+    char* base = static_cast<char*>(ctx->tabs.p);
+    HIPCHK(hipMemcpyAsync(base, kx2.data(), sizeof(float) * nx, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(base + o_ky, ky2.data(), sizeof(float) * ny, hipMemcpyHostToDevice, ctx->stream));
+    if (nl) {
+        HIPCHK(hipMemcpyAsync(base + o_px, px.data(), sizeof(C) * px.size(), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(base + o_py, py.data(), sizeof(C) * py.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    // the host vectors must outlive the async copies
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    out->kx2 = reinterpret_cast<const float*>(base);
+    out->ky2 = reinterpret_cast<const float*>(base + o_ky);
+    out->px = reinterpret_cast<const C*>(base + o_px);
+    out->py = reinterpret_cast<const C*>(base + o_py);
+    out->k2max = k2max_of(nx, ny, st->ax, st->by, st->bw);
+    return MSX_OK;
+}
+
+template <typename C> RowArgs<C> row_args(C* psi, int nx, int nb, const C* tw) {
+    RowArgs<C> r;
+    std::memset(&r, 0, sizeof r);
+    r.psi = psi;
+    r.nx = nx;
+    r.nb = nb;
+    r.tw = tw;
+    r.scale = 1;
+    return r;
+}
+template <typename C> ColArgs<C> col_args(C* psi, int ny, int nb, const C* tw) {
+    ColArgs<C> c;
+    std::memset(&c, 0, sizeof c);
+    c.psi = psi;
+    c.ny = ny;
+    c.nb = nb;
+    c.tw = tw;
+    c.ops = kColStore;
+    return c;
+}
+
+#define LAUNCH_ROW(n, mode, args) HIPCHK(launch_row<C>((n), (mode), (args), ctx->stream))
+#define LAUNCH_COL(n, mode, args) HIPCHK(launch_col<C>((n), (mode), (args), ctx->stream))
+
+// ---------------------------------------------------------------- fft2d
+template <typename C> int fft2d_dev(msx_ctx ctx, C* d, int nb, int nx, int ny, int sign) {
+    using Re = typename real_of<C>::type;
+    const C *twx, *twy;
+    CHK(ctx->twiddles<C>(nx, &twx));
+    CHK(ctx->twiddles<C>(ny, &twy));
+    RowArgs<C> r = row_args<C>(d, nx, nb, twy);
+    ColArgs<C> c = col_args<C>(d, ny, nb, twx);
+    if (sign > 0) {
+        LAUNCH_ROW(ny, kRowFwd, r);
+        LAUNCH_COL(nx, kColFwd, c);
+    } else {
+        r.scale = (Re)1 / ((Re)nx * (Re)ny);
+        LAUNCH_ROW(ny, kRowInv0, r);
+        LAUNCH_COL(nx, kColInv, c);
+    }
+    return MSX_OK;
+}
+
+// ---------------------------------------------------------------- transmission
+template <typename C>
+int transmission_dev(msx_ctx ctx, const typename real_of<C>::type* V, int nl, int nx, int ny, double ax, double by,
+                     double kev, double bw, C* t) {
+    using Re = typename real_of<C>::type;
+    const C *twx, *twy;
+    CHK(ctx->twiddles<C>(nx, &twx));
+    CHK(ctx->twiddles<C>(ny, &twy));
+    RowArgs<C> r = row_args<C>(t, nx, nl, twy);
+    r.vreal = V;
+    r.vscale = (Re)(msx_sigma(kev) / 1000.0);
+    if (!(bw > 0)) {
+        LAUNCH_ROW(ny, kRowNone, r);
+        return MSX_OK;
+    }
+    msx_stack st{};
+    st.nx = nx;
+    st.ny = ny;
+    st.ax = ax;
+    st.by = by;
+    st.kev = kev;
+    st.bw = bw;
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, &st, false, &tb));
+    LAUNCH_ROW(ny, kRowFwd, r);
+    ColArgs<C> c = col_args<C>(t, ny, nl, twx);
+    c.kx2 = tb.kx2;
+    c.ky2 = tb.ky2;
+    c.k2max = tb.k2max;
+    c.ops = kColMask | kColStore;
+    LAUNCH_COL(nx, kColSlice, c);
+    RowArgs<C> r2 = row_args<C>(t, nx, nl, twy);
+    r2.scale = (Re)1 / ((Re)nx * (Re)ny);
+    LAUNCH_ROW(ny, kRowInv, r2);
+    return MSX_OK;
+}
+
+// ---------------------------------------------------------------- slice loop
+struct DetPlan {
+    int ndet = 0;
+    float dlo[kMaxDet], dhi[kMaxDet];
+    const int32_t* ck = nullptr;
+    int nck = 0;
+    double* out = nullptr;      // device [nck][ndet][nprobe]
+    long long nprobe = 0, probe0 = 0;
+};
+
+// Advances nb waves through the stack.  start: 0 = psi holds real-space waves
+// (or src is a shared incident wave), 1 = psi holds k-space probes already
+// through the column inverse (the first row sweep starts with IFFT_y).
+// With det != nullptr the loop stops after the last checkpoint; otherwise
+// it ends with the final inverse row sweep (exit waves in real space).
+template <typename C>
+int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
+               const C* src, long long src_bstride, int start, DetPlan* det) {
+    using Re = typename real_of<C>::type;
+    const int nx = st->nx, ny = st->ny;
+    const long long img = (long long)nx * ny;
+    const C *twx, *twy;
+    CHK(ctx->twiddles<C>(nx, &twx));
+    CHK(ctx->twiddles<C>(ny, &twy));
+    const Re inv = (Re)1 / ((Re)nx * (Re)ny);
+    const int groups = col_groups<C>(nx, ny);
+    int ick = 0;
+    for (int s = 0; s < st->nslice; s++) {
+        const int l = st->layer[s];
+        RowArgs<C> r = row_args<C>(psi, nx, nb, twy);
+        r.trans = trans + (long long)l * img;
+        if (s == 0 && start == 0) {
+            r.src = src;
+            r.src_bstride = src_bstride;
+            LAUNCH_ROW(ny, kRowFwd, r);
+        } else {
+            r.scale = inv;
+            LAUNCH_ROW(ny, kRowSlice, r);
+        }
+        ColArgs<C> c = col_args<C>(psi, ny, nb, twx);
+        c.kx2 = tb.kx2;
+        c.ky2 = tb.ky2;
+        c.k2max = tb.k2max;
+        c.px = tb.px + (long long)l * nx;
+        c.py = tb.py + (long long)l * ny;
+        c.ops = kColProp | kColStore;
+        bool is_ck = det && ick < det->nck && det->ck[ick] == s + 1;
+        if (is_ck) {
+            c.ops |= kColDetect;
+            c.ndet = det->ndet;
+            for (int d = 0; d < kMaxDet; d++) {
+                c.dlo[d] = det->dlo[d];
+                c.dhi[d] = det->dhi[d];
+            }
+            c.partial = static_cast<double*>(ctx->partial.p);
+        }
+        const bool last = det && is_ck && ick == det->nck - 1;
+        if (last) {
+            c.ops &= ~kColStore;  // nothing reads the wave after the last checkpoint
+            LAUNCH_COL(nx, kColFwd, c);
+        } else {
+            LAUNCH_COL(nx, kColSlice, c);
+        }
+        if (is_ck) {
+            HIPCHK(launch_det_finish(c.partial, det->ndet, nb, groups, 1.0 / ((double)nx * ny),
+                                     det->out + (long long)ick * det->ndet * det->nprobe, det->nprobe,
+                                     det->probe0, ctx->stream));
+            ick++;
+        }
+        if (last) return MSX_OK;
+    }
+    if (!det) {
+        RowArgs<C> r = row_args<C>(psi, nx, nb, twy);
+        r.scale = inv;
+        LAUNCH_ROW(ny, kRowInv, r);
+    }
+    return MSX_OK;
+}
+
+int check_stack(const msx_stack* st) {
+    if (!st) return fail(MSX_EINVAL, "stack is NULL");
+    CHK(check_grid(st->nx, st->ny));
+    if (!(st->ax > 0) || !(st->by > 0) || !(st->kev > 0)) return fail(MSX_EINVAL, "ax, by, kev must be > 0");
+    if (st->nlayer <= 0 || !st->cz) return fail(MSX_EINVAL, "nlayer must be > 0 with cz set");
+    if (st->nslice <= 0 || !st->layer) return fail(MSX_EINVAL, "nslice must be > 0 with layer set");
+    for (int s = 0; s < st->nslice; s++)
+        if (st->layer[s] < 0 || st->layer[s] >= st->nlayer)
+            return fail(MSX_EINVAL, "layer[%d] = %d out of range [0,%d)", s, st->layer[s], st->nlayer);
+    return MSX_OK;
+}
+
+template <typename C>
+int multislice_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const void* psi0, void* psi, int nwave,
+                    int where) {
+    const long long img = (long long)st->nx * st->ny;
+    const size_t tbytes = sizeof(C) * img * st->nlayer, wbytes = sizeof(C) * img * nwave;
+    const C* t_dev = static_cast<const C*>(trans);
+    C* w_dev = static_cast<C*>(psi);
+    const C* s_dev = static_cast<const C*>(psi0);
+    if (where == MSX_HOST) {
+        CHK(ctx->trans.ensure(tbytes));
+        CHK(ctx->wave.ensure(wbytes));
+        HIPCHK(hipMemcpyAsync(ctx->trans.p, trans, tbytes, hipMemcpyHostToDevice, ctx->stream));
+        t_dev = static_cast<const C*>(ctx->trans.p);
+        w_dev = static_cast<C*>(ctx->wave.p);
+        if (psi0) {
+            CHK(ctx->aux.ensure(sizeof(C) * img));
+            HIPCHK(hipMemcpyAsync(ctx->aux.p, psi0, sizeof(C) * img, hipMemcpyHostToDevice, ctx->stream));
+            s_dev = static_cast<const C*>(ctx->aux.p);
+        } else {
+            HIPCHK(hipMemcpyAsync(w_dev, psi, wbytes, hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, st, true, &tb));
+    CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nwave, s_dev, 0, 0, nullptr));
+    if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(psi, w_dev, wbytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+template <typename C>
+int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_probe* pr, const double* pos,
+              int nprobe, const msx_detector* det, int ndet, const int32_t* ck, int nck, double weight,
+              int accumulate, double* out, int batch, int where) {
+    using Re = typename real_of<C>::type;
+    const int nx = st->nx, ny = st->ny;
+    const long long img = (long long)nx * ny;
+    if (batch <= 0) {
+        // default: enough waves to fill the GPU, bounded by ~2 GiB of waves
+        long long b = (2LL << 30) / (long long)(sizeof(C) * img);
+        batch = (int)std::max(1LL, std::min<long long>(b, 256));
+    }
+    batch = std::min(batch, nprobe);
+    const C* t_dev = static_cast<const C*>(trans);
+    if (where == MSX_HOST) {
+        const size_t tbytes = sizeof(C) * img * st->nlayer;
+        CHK(ctx->trans.ensure(tbytes));
+        HIPCHK(hipMemcpyAsync(ctx->trans.p, trans, tbytes, hipMemcpyHostToDevice, ctx->stream));
+        t_dev = static_cast<const C*>(ctx->trans.p);
+    }
+    CHK(ctx->wave.ensure(sizeof(C) * img * batch));
+    C* w_dev = static_cast<C*>(ctx->wave.p);
+    const int groups = col_groups<C>(nx, ny);
+    CHK(ctx->partial.ensure(sizeof(double) * (size_t)ndet * batch * groups));
+    const size_t obytes = sizeof(double) * (size_t)nck * ndet * nprobe;
+    CHK(ctx->outbuf.ensure(obytes + sizeof(double2) * (size_t)batch + 256));
+    double* o_dev = static_cast<double*>(ctx->outbuf.p);
+    double2* pos_dev = reinterpret_cast<double2*>(static_cast<char*>(ctx->outbuf.p) + ((obytes + 255) & ~(size_t)255));
+
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, st, true, &tb));
+    const double lam = msx_wavelength(st->kev);
+    const float kap_hi = (float)((pr->alpha_max / lam) * (pr->alpha_max / lam));
+    const float kap_lo = (float)((pr->alpha_min / lam) * (pr->alpha_min / lam));
+    // aperture pixel count with the same float test as the kernel
+    std::vector<float> kx2 = k2_table(nx, st->ax), ky2 = k2_table(ny, st->by);
+    long long nap = 0;
+    for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+            float k2 = kx2[ix] + ky2[iy];
+            if (k2 <= kap_hi && k2 >= kap_lo) nap++;
+        }
+    if (nap == 0) return fail(MSX_EINVAL, "probe aperture contains no pixel");
+    const C *twx;
+    CHK(ctx->twiddles<C>(nx, &twx));
+
+    DetPlan dp;
+    dp.ndet = ndet;
+    for (int d = 0; d < kMaxDet; d++) {
+        dp.dlo[d] = d < ndet ? (float)(det[d].k_inner * det[d].k_inner) : 1.f;
+        dp.dhi[d] = d < ndet ? (float)(det[d].k_outer * det[d].k_outer) : 0.f;
+    }
+    dp.ck = ck;
+    dp.nck = nck;
+    dp.out = o_dev;
+    dp.nprobe = nprobe;
+    std::vector<double2> hpos(batch);
+    for (int p0 = 0; p0 < nprobe; p0 += batch) {
+        const int nb = std::min(batch, nprobe - p0);
+        for (int b = 0; b < nb; b++) {
+            hpos[b].x = pos[2 * (p0 + b)];
+            hpos[b].y = pos[2 * (p0 + b) + 1];
+        }
+        HIPCHK(hipMemcpyAsync(pos_dev, hpos.data(), sizeof(double2) * nb, hipMemcpyHostToDevice, ctx->stream));
+        ColArgs<C> g = col_args<C>(w_dev, ny, nb, twx);
+        g.ops = kColGen | kColStore;
+        g.kx2 = tb.kx2;
+        g.ky2 = tb.ky2;
+        g.pos = pos_dev;
+        g.ax = st->ax;
+        g.by = st->by;
+        g.chi1 = M_PI * lam;
+        g.chi2 = 0.5 * pr->cs * lam * lam;
+        g.df = pr->df;
+        g.kap_lo = kap_lo;
+        g.kap_hi = kap_hi;
+        g.amp = (Re)std::sqrt((double)img / (double)nap);
+        LAUNCH_COL(nx, kColGenInv, g);
+        dp.probe0 = p0;
+        CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nb, nullptr, 0, 1, &dp));
+        // hpos is reused by the next batch: wait for its copy
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<double> res((size_t)nck * ndet * nprobe);
+    HIPCHK(hipMemcpyAsync(res.data(), o_dev, obytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < res.size(); i++) out[i] = (accumulate ? out[i] : 0.0) + weight * res[i];
+    return MSX_OK;
+}
+
+template <typename C>
+int ctf_impl(msx_ctx ctx, const void* psi, int nwave, int nx, int ny, double ax, double by, double kev, double cs,
+             double df, double aobj, void* psi_img, void* inten, int where) {
+    using Re = typename real_of<C>::type;
+    const long long img = (long long)nx * ny;
+    const size_t wbytes = sizeof(C) * img * nwave, ibytes = sizeof(Re) * img * nwave;
+    CHK(ctx->wave2.ensure(wbytes));
+    C* w = static_cast<C*>(ctx->wave2.p);
+    const C* in = static_cast<const C*>(psi);
+    Re* i_dev = static_cast<Re*>(inten);
+    if (where == MSX_HOST) {
+        HIPCHK(hipMemcpyAsync(w, psi, wbytes, hipMemcpyHostToDevice, ctx->stream));
+        in = w;
+        if (inten) {
+            CHK(ctx->vtmp.ensure(ibytes));
+            i_dev = static_cast<Re*>(ctx->vtmp.p);
+        }
+    }
+    const C *twx, *twy;
+    CHK(ctx->twiddles<C>(nx, &twx));
+    CHK(ctx->twiddles<C>(ny, &twy));
+    msx_stack st{};
+    st.nx = nx;
+    st.ny = ny;
+    st.ax = ax;
+    st.by = by;
+    st.kev = kev;
+    st.bw = 1.0;
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, &st, false, &tb));
+    const double lam = msx_wavelength(kev);
+    RowArgs<C> r = row_args<C>(w, nx, nwave, twy);
+    r.src = in;
+    r.src_bstride = img;
+    LAUNCH_ROW(ny, kRowFwd, r);
+    ColArgs<C> c = col_args<C>(w, ny, nwave, twx);
+    c.ops = kColCtf | kColStore;
+    c.kx2 = tb.kx2;
+    c.ky2 = tb.ky2;
+    c.ax = ax;
+    c.by = by;
+    c.chi1 = M_PI * lam;
+    c.chi2 = 0.5 * cs * lam * lam;
+    c.df = df;
+    c.kap_hi = (float)((aobj / lam) * (aobj / lam));
+    LAUNCH_COL(nx, kColSlice, c);
+    RowArgs<C> r2 = row_args<C>(w, nx, nwave, twy);
+    r2.scale = (Re)1 / ((Re)nx * (Re)ny);
+    r2.inten = inten ? i_dev : nullptr;
+    if (psi_img && where == MSX_DEVICE) {
+        r2.src = w;
+        r2.src_bstride = img;
+        r2.psi = static_cast<C*>(psi_img);
+    }
+    LAUNCH_ROW(ny, kRowInv, r2);
+    if (where == MSX_HOST) {
+        if (psi_img) HIPCHK(hipMemcpyAsync(psi_img, w, wbytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (inten) HIPCHK(hipMemcpyAsync(inten, i_dev, ibytes, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+// host binning of atoms for the splat kernel
+int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c, int nx, int ny,
+                   int nslices, double radius, int table, int formula, void* V, int prec, int where) {
+    const double dz = c / nslices;
+    const int nbx = std::max(1, std::min(1024, (int)std::floor(a / radius)));
+    const int nby = std::max(1, std::min(1024, (int)std::floor(b / radius)));
+    const long long ncell = (long long)nslices * nbx * nby;
+    std::vector<int> cnt(ncell + 1, 0);
+    std::vector<long long> cell_of(natoms, -1);
+    std::vector<AtomRec> rec(natoms);
+    for (int64_t i = 0; i < natoms; i++) {
+        const msx_atom& at = atoms[i];
+        if (at.Z < 1 || at.Z > 103) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..103", (long long)i, at.Z);
+        // ModelPotential.cu:63-66: kz*dz <= z < (kz+1)*dz
+        long long kz = (long long)std::floor(at.z / dz);
+        if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
+        if (kz < 0 || kz >= nslices) continue;
+        double x = at.x - a * std::floor(at.x / a), y = at.y - b * std::floor(at.y / b);
+        if (x >= a) x -= a;
+        if (y >= b) y -= b;
+        int bx = std::min(nbx - 1, (int)(x / (a / nbx))), by = std::min(nby - 1, (int)(y / (b / nby)));
+        const long long cell = (kz * nbx + bx) * nby + by;
+        cell_of[i] = cell;
+        rec[i] = AtomRec{x, y, at.occ, at.Z};
+        cnt[cell + 1]++;
+    }
+    for (long long k = 0; k < ncell; k++) cnt[k + 1] += cnt[k];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    std::vector<AtomRec> sorted(std::max<long long>(1, cnt[ncell]));
+    for (int64_t i = 0; i < natoms; i++)
+        if (cell_of[i] >= 0) sorted[fill[cell_of[i]]++] = rec[i];
+    CHK(ctx->atoms.ensure(sizeof(AtomRec) * sorted.size()));
+    CHK(ctx->cells.ensure(sizeof(int) * cnt.size()));
+    HIPCHK(hipMemcpyAsync(ctx->atoms.p, sorted.data(), sizeof(AtomRec) * sorted.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->cells.p, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, ctx->stream));
+    const size_t vbytes = (prec == MSX_F64 ? sizeof(double) : sizeof(float)) * (size_t)nslices * nx * ny;
+    void* v_dev = V;
+    if (where == MSX_HOST) {
+        CHK(ctx->vtmp.ensure(vbytes));
+        v_dev = ctx->vtmp.p;
+    }
+    SplatArgs sa;
+    sa.cell_start = static_cast<const int*>(ctx->cells.p);
+    sa.atoms = static_cast<const AtomRec*>(ctx->atoms.p);
+    sa.v64 = prec == MSX_F64 ? static_cast<double*>(v_dev) : nullptr;
+    sa.v32 = prec == MSX_F32 ? static_cast<float*>(v_dev) : nullptr;
+    sa.a = a;
+    sa.b = b;
+    sa.radius = radius;
+    sa.nx = nx;
+    sa.ny = ny;
+    sa.nslices = nslices;
+    sa.nbx = nbx;
+    sa.nby = nby;
+    sa.table = table;
+    sa.formula = formula;
+    HIPCHK(launch_splat(sa, ctx->stream));
+    if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(V, v_dev, vbytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+MSX_API int msx_version(void) { return 1; }
+MSX_API const char* msx_last_error(void) { return g_err.c_str(); }
+
+MSX_API int msx_create(int device, msx_ctx* out) {
+    if (!out) return fail(MSX_EINVAL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(MSX_EINVAL, "device %d not present (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    msx_ctx c = new msx_ctx_s();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MSX_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    c->stream = c->own;
+    e = upload_fparams(c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MSX_EHIP, "fparams upload: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return MSX_OK;
+}
+
+MSX_API int msx_destroy(msx_ctx ctx) {
+    if (!ctx) return MSX_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    delete ctx;
+    return MSX_OK;
+}
+
+MSX_API int msx_set_stream(msx_ctx ctx, void* s) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own;
+    return MSX_OK;
+}
+
+MSX_API double msx_wavelength(double kev) {
+    const double emass = 510.99906, hc = 12.3984244;
+    return hc / std::sqrt(kev * (2 * emass + kev));
+}
+MSX_API double msx_sigma(double kev) {
+    const double emass = 510.99906;
+    const double x = (emass + kev) / (2.0 * emass + kev);
+    return 2.0 * (4.0 * std::atan(1.0)) * x / (msx_wavelength(kev) * kev);
+}
+
+static double h_bessi0(double x) {
+    double ax = std::fabs(x);
+    if (ax <= 3.75) {
+        double t = x / 3.75;
+        t = t * t;
+        return 1.0 + t * (3.5156229 + t * (3.0899424 + t * (1.2067492 + t * (0.2659732 + t * (0.0360768 + t * 0.0045813)))));
+    }
+    double t = 3.75 / ax;
+    double s = 0.39894228 + t * (0.01328592 + t * (0.00225319 + t * (-0.00157565 + t * (0.00916281 + t * (-0.02057706 +
+               t * (0.02635537 + t * (-0.01647633 + t * 0.00392377)))))));
+    return std::exp(ax) * s / std::sqrt(ax);
+}
+static double h_bessk0(double x) {
+    double ax = std::fabs(x);
+    if (ax > 0.0 && ax <= 2.0) {
+        double x2 = ax / 2.0;
+        x2 = x2 * x2;
+        double s = -0.57721566 + x2 * (0.42278420 + x2 * (0.23069756 + x2 * (0.03488590 + x2 * (0.00262698 +
+                   x2 * (0.00010750 + x2 * 0.00000740)))));
+        return -std::log(ax / 2.0) * h_bessi0(x) + s;
+    } else if (ax > 2.0) {
+        double x2 = 2.0 / ax;
+        double s = 1.25331414 + x2 * (-0.07832358 + x2 * (0.02189568 + x2 * (-0.01062446 + x2 * (0.00587872 +
+                   x2 * (-0.00251540 + x2 * 0.00053208)))));
+        return std::exp(-ax) * s / std::sqrt(ax);
+    }
+    return 1.0e20;
+}
+
+MSX_API double msx_vzatom(int Z, double r, int table, int formula) {
+    if (Z < 1 || Z > 103 || table < 0 || table > 1) return 0.0;
+    const double* p = MSX_FPARAMS[table][Z - 1];
+    r = std::fabs(r);
+    if (r < 1.0e-10) r = 1.0e-10;
+    double suml = 0.0, sumg = 0.0;
+    if (formula == MSX_FORMULA_KIRKLAND) {
+        const double pi = 3.141592654;
+        for (int i = 0; i < 6; i += 2) suml += p[i] * h_bessk0(2.0 * pi * r * std::sqrt(p[i + 1]));
+        const double x2 = (pi * r) * (pi * r);
+        for (int i = 6; i < 12; i += 2) sumg += p[i] * std::exp(-x2 / p[i + 1]) / p[i + 1];
+        return 300.8242834 * suml + 150.4121417 * sumg;
+    }
+    for (int k = 0; k < 3; k++) suml += p[2 * k] * h_bessk0(6.2831853071796 * r * std::sqrt(p[2 * k + 1]));
+    for (int k = 0; k < 3; k++) sumg += (p[6 + 2 * k] / p[7 + 2 * k]) * std::exp(-(6.2831853071796 * r * r) / p[7 + 2 * k]);
+    return 300.73079394295 * suml + 150.36539697148 * sumg;
+}
+MSX_API double msx_featom(int Z, double k2, int table) {
+    if (Z < 1 || Z > 103 || table < 0 || table > 1) return 0.0;
+    const double* p = MSX_FPARAMS[table][Z - 1];
+    double s = 0.0;
+    for (int i = 0; i < 6; i += 2) s += p[i] / (k2 + p[i + 1]);
+    for (int i = 6; i < 12; i += 2) s += p[i] * std::exp(-k2 * p[i + 1]);
+    return s;
+}
+
+MSX_API int msx_fft2d(msx_ctx ctx, void* data, int nbatch, int nx, int ny, int sign, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_grid(nx, ny));
+    if (!data || nbatch <= 0 || (sign != 1 && sign != -1)) return fail(MSX_EINVAL, "bad data/nbatch/sign");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t es = prec == MSX_F64 ? 16 : 8, bytes = es * (size_t)nx * ny * nbatch;
+    void* d = data;
+    if (where == MSX_HOST) {
+        CHK(ctx->wave.ensure(bytes));
+        d = ctx->wave.p;
+        HIPCHK(hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if (prec == MSX_F64) CHK(fft2d_dev<double2>(ctx, static_cast<double2*>(d), nbatch, nx, ny, sign));
+    else CHK(fft2d_dev<float2>(ctx, static_cast<float2*>(d), nbatch, nx, ny, sign));
+    if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(data, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c, int nx,
+                          int ny, int nslices, double radius, int table, int formula, void* V, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    if (nx <= 0 || ny <= 0 || nx > 16384 || ny > 16384 || nslices <= 0) return fail(MSX_EINVAL, "bad grid %d x %d x %d", nx, ny, nslices);
+    if (!(a > 0) || !(b > 0) || !(c > 0) || !(radius > 0)) return fail(MSX_EINVAL, "a, b, c, radius must be > 0");
+    if (table < 0 || table > 1 || formula < 0 || formula > 1) return fail(MSX_EINVAL, "bad table/formula");
+    if (natoms < 0 || (natoms > 0 && !atoms) || !V) return fail(MSX_EINVAL, "bad atoms/V");
+    HIPCHK(hipSetDevice(ctx->device));
+    return potential_impl(ctx, atoms, natoms, a, b, c, nx, ny, nslices, radius, table, formula, V, prec, where);
+}
+
+MSX_API int msx_transmission(msx_ctx ctx, const void* V, int nlayer, int nx, int ny, double ax, double by, double kev,
+                             double bw, void* t, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_grid(nx, ny));
+    if (!V || !t || nlayer <= 0 || !(ax > 0) || !(by > 0) || !(kev > 0)) return fail(MSX_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)nx * ny * nlayer;
+    const size_t vb = (prec == MSX_F64 ? 8 : 4) * n, tb = (prec == MSX_F64 ? 16 : 8) * n;
+    const void* v_dev = V;
+    void* t_dev = t;
+    if (where == MSX_HOST) {
+        CHK(ctx->vtmp.ensure(vb));
+        CHK(ctx->trans.ensure(tb));
+        HIPCHK(hipMemcpyAsync(ctx->vtmp.p, V, vb, hipMemcpyHostToDevice, ctx->stream));
+        v_dev = ctx->vtmp.p;
+        t_dev = ctx->trans.p;
+    }
+    if (prec == MSX_F64)
+        CHK(transmission_dev<double2>(ctx, static_cast<const double*>(v_dev), nlayer, nx, ny, ax, by, kev, bw,
+                                      static_cast<double2*>(t_dev)));
+    else
+        CHK(transmission_dev<float2>(ctx, static_cast<const float*>(v_dev), nlayer, nx, ny, ax, by, kev, bw,
+                                     static_cast<float2*>(t_dev)));
+    if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(t, t_dev, tb, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0, void* psi,
+                           int nwave, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_stack(stack));
+    if (!trans || !psi || nwave <= 0) return fail(MSX_EINVAL, "bad trans/psi/nwave");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (prec == MSX_F64) return multislice_impl<double2>(ctx, stack, trans, psi0, psi, nwave, where);
+    return multislice_impl<float2>(ctx, stack, trans, psi0, psi, nwave, where);
+}
+
+MSX_API int msx_stem(msx_ctx ctx, const msx_stack* stack, const void* trans, const msx_probe* probe, const double* pos,
+                     int nprobe, const msx_detector* det, int ndet, const int32_t* ck, int nck, double weight,
+                     int accumulate, double* out, int batch, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_stack(stack));
+    if (!trans || !probe || !pos || nprobe <= 0 || !out) return fail(MSX_EINVAL, "bad trans/probe/pos/out");
+    if (ndet <= 0 || ndet > kMaxDet || !det) return fail(MSX_EINVAL, "ndet must be 1..%d", kMaxDet);
+    if (nck <= 0 || !ck) return fail(MSX_EINVAL, "need at least one checkpoint");
+    for (int i = 0; i < nck; i++)
+        if (ck[i] < 1 || ck[i] > stack->nslice || (i && ck[i] <= ck[i - 1]))
+            return fail(MSX_EINVAL, "checkpoints must increase within 1..nslice");
+    if (!(probe->alpha_max > 0)) return fail(MSX_EINVAL, "alpha_max must be > 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (prec == MSX_F64)
+        return stem_impl<double2>(ctx, stack, trans, probe, pos, nprobe, det, ndet, ck, nck, weight, accumulate, out,
+                                  batch, where);
+    return stem_impl<float2>(ctx, stack, trans, probe, pos, nprobe, det, ndet, ck, nck, weight, accumulate, out, batch,
+                             where);
+}
+
+MSX_API int msx_ctf_image(msx_ctx ctx, const void* psi, int nwave, int nx, int ny, double ax, double by, double kev,
+                          double cs, double df, double aobj, void* psi_img, void* intensity, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_grid(nx, ny));
+    if (!psi || nwave <= 0 || !(kev > 0)) return fail(MSX_EINVAL, "bad arguments");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (prec == MSX_F64)
+        return ctf_impl<double2>(ctx, psi, nwave, nx, ny, ax, by, kev, cs, df, aobj, psi_img, intensity, where);
+    return ctf_impl<float2>(ctx, psi, nwave, nx, ny, ax, by, kev, cs, df, aobj, psi_img, intensity, where);
+}
+
+}  // extern "C"

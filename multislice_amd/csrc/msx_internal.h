@@ -1,0 +1,90 @@
+// msx_internal.h -- declarations shared by the engine's translation units
+// (not part of the public C ABI, which is include/msx.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "msx_sweep.hpp"
+
+namespace msx {
+
+// Row-sweep modes: (IN, OUT) FFT directions around the pointwise stage.
+enum RowMode : int {
+    kRowFwd = 0,    // (0, +1)
+    kRowSlice = 1,  // (-1, +1)
+    kRowInv = 2,    // (-1, 0)
+    kRowInv0 = 3,   // (0, -1)
+    kRowNone = 4,   // (0, 0)
+};
+// Column-sweep modes.
+enum ColMode : int {
+    kColSlice = 0,  // (+1, -1)
+    kColGenInv = 1, // (0, -1)
+    kColFwd = 2,    // (+1, 0)
+    kColInv = 3,    // (-1, 0)
+    kColPoint = 4,  // (0, 0)
+};
+
+// Supported line lengths (both axes).
+#define MSX_FOR_EACH_N(X) X(16) X(32) X(64) X(128) X(256) X(384) X(512) X(768) X(1024) X(2048)
+
+#define MSX_DECL_N(NN)                                                                         \
+    template <typename C> hipError_t launch_row_##NN(int mode, const RowArgs<C>& a, hipStream_t s); \
+    template <typename C> hipError_t launch_col_##NN(int mode, const ColArgs<C>& a, hipStream_t s); \
+    template <typename C> int col_groups_##NN(int ny);
+MSX_FOR_EACH_N(MSX_DECL_N)
+#undef MSX_DECL_N
+
+inline bool supported_n(int n) {
+#define MSX_CASE(NN) \
+    if (n == NN) return true;
+    MSX_FOR_EACH_N(MSX_CASE)
+#undef MSX_CASE
+    return false;
+}
+
+template <typename C> hipError_t launch_row(int n, int mode, const RowArgs<C>& a, hipStream_t s) {
+#define MSX_CASE(NN) \
+    if (n == NN) return launch_row_##NN<C>(mode, a, s);
+    MSX_FOR_EACH_N(MSX_CASE)
+#undef MSX_CASE
+    return hipErrorInvalidValue;
+}
+template <typename C> hipError_t launch_col(int n, int mode, const ColArgs<C>& a, hipStream_t s) {
+#define MSX_CASE(NN) \
+    if (n == NN) return launch_col_##NN<C>(mode, a, s);
+    MSX_FOR_EACH_N(MSX_CASE)
+#undef MSX_CASE
+    return hipErrorInvalidValue;
+}
+// number of column groups (work-groups per wave) of the column sweep for an
+// nx-long line and ny columns
+template <typename C> int col_groups(int nx, int ny) {
+#define MSX_CASE(NN) \
+    if (nx == NN) return col_groups_##NN<C>(ny);
+    MSX_FOR_EACH_N(MSX_CASE)
+#undef MSX_CASE
+    return 0;
+}
+
+// potential splat (msx_potential.hip)
+struct AtomRec {
+    double x, y;  // Angstrom, wrapped into [0,a) x [0,b)
+    float occ;
+    int Z;
+};
+struct SplatArgs {
+    const int* cell_start;   // [nslices*nbx*nby + 1] start of each (slice, bin) in atoms
+    const AtomRec* atoms;    // sorted by (slice, bin x, bin y)
+    double* v64;             // FP64 output [nslices][nx][ny], or nullptr
+    float* v32;              // FP32 output [nslices][nx][ny], or nullptr
+    double a, b, radius;
+    int nx, ny, nslices, nbx, nby, table, formula;
+};
+hipError_t launch_splat(const SplatArgs& a, hipStream_t s);
+hipError_t upload_fparams(hipStream_t s);
+
+// detector finish: out[d][p] += w * sum_g partial[d][b][g] (p = probe0 + b)
+hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups, double w, double* out,
+                             long long out_stride_d, long long probe0, hipStream_t s);
+
+}  // namespace msx

@@ -1,0 +1,145 @@
+// msx_potential.hip -- projected-potential splat (the GPU counterpart of
+// LMultiSlice calculatePotentialGridGPU, ModelPotential.cu:125-230) and the
+// detector finishing reduction.
+//
+// Differences from the reference kernel, by design:
+//  * atoms are binned per (slice, lateral bin) on the host, bins at least
+//    `radius` wide, so a 16x16 pixel tile only visits the bins within reach
+//    (the reference loops every pixel over every atom of the slice);
+//  * the whole slab is one launch over (tile, slice) instead of one launch per
+//    4608-atom constant-memory batch per slice;
+//  * distances and the fparams sums are FP64 (the reference computes
+//    distances in float), which removes the small-r rounding of K0's log.
+// The per-pixel contribution rule is the reference's: minimum image per axis,
+// r < radius, r clamped >= 1e-10, formula and table selectable.
+#include "msx_internal.h"
+#include "fparams_tables.inc"
+
+namespace msx {
+
+__constant__ double c_fparams[2][103][12];
+
+hipError_t upload_fparams(hipStream_t s) {
+    return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_fparams), MSX_FPARAMS, sizeof(MSX_FPARAMS), 0,
+                                  hipMemcpyHostToDevice, s);
+}
+
+// Abramowitz & Stegun 9.8.1-9.8.6 polynomials, as slicelib.cpp:168-237 and
+// ModelPotential.cu:256-292.
+__device__ __forceinline__ double d_bessi0(double x) {
+    const double ax = fabs(x);
+    if (ax <= 3.75) {
+        double t = x / 3.75;
+        t = t * t;
+        return 1.0 + t * (3.5156229 + t * (3.0899424 + t * (1.2067492 + t * (0.2659732 +
+                    t * (0.0360768 + t * 0.0045813)))));
+    }
+    const double t = 3.75 / ax;
+    const double sum = 0.39894228 + t * (0.01328592 + t * (0.00225319 + t * (-0.00157565 +
+                       t * (0.00916281 + t * (-0.02057706 + t * (0.02635537 + t * (-0.01647633 +
+                       t * 0.00392377)))))));
+    return exp(ax) * sum / sqrt(ax);
+}
+__device__ __forceinline__ double d_bessk0(double x) {
+    const double ax = fabs(x);
+    if (ax > 0.0 && ax <= 2.0) {
+        double x2 = ax / 2.0;
+        x2 = x2 * x2;
+        const double sum = -0.57721566 + x2 * (0.42278420 + x2 * (0.23069756 + x2 * (0.03488590 +
+                           x2 * (0.00262698 + x2 * (0.00010750 + x2 * 0.00000740)))));
+        return -log(ax / 2.0) * d_bessi0(x) + sum;
+    } else if (ax > 2.0) {
+        const double x2 = 2.0 / ax;
+        const double sum = 1.25331414 + x2 * (-0.07832358 + x2 * (0.02189568 + x2 * (-0.01062446 +
+                           x2 * (0.00587872 + x2 * (-0.00251540 + x2 * 0.00053208)))));
+        return exp(-ax) * sum / sqrt(ax);
+    }
+    return 1.0e20;
+}
+
+// vzatom (slicelib.cpp:2666-2699) / calculateProjectedPotential
+// (ModelPotential.cu:232-254)
+__device__ __forceinline__ double d_vz(int Z, double r, int table, int formula) {
+    const double* p = c_fparams[table][Z - 1];
+    if (r < 1.0e-10) r = 1.0e-10;
+    double suml = 0.0, sumg = 0.0;
+    if (formula == 0) {
+        const double pi = 3.141592654;
+        const double x = 2.0 * pi * r;
+        for (int i = 0; i < 6; i += 2) suml += p[i] * d_bessk0(x * sqrt(p[i + 1]));
+        const double x2 = (pi * r) * (pi * r);
+        for (int i = 6; i < 12; i += 2) sumg += p[i] * exp(-x2 / p[i + 1]) / p[i + 1];
+        return 300.8242834 * suml + 150.4121417 * sumg;
+    }
+    const double dR1 = 6.2831853071796 * r;
+    for (int k = 0; k < 3; k++) suml += p[2 * k] * d_bessk0(dR1 * sqrt(p[2 * k + 1]));
+    for (int k = 0; k < 3; k++) sumg += (p[6 + 2 * k] / p[7 + 2 * k]) * exp(-(6.2831853071796 * r * r) / p[7 + 2 * k]);
+    return 300.73079394295 * suml + 150.36539697148 * sumg;
+}
+
+__global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
+    const int iy = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int ix = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int kz = blockIdx.z;
+    const double dx = p.a / p.nx, dy = p.b / p.ny;
+    const double xp = ix * dx, yp = iy * dy;
+    const double r2max = p.radius * p.radius;
+    // bins reached by this tile
+    const double bwx = p.a / p.nbx, bwy = p.b / p.nby;
+    const double xlo = (double)(blockIdx.y * 16) * dx - p.radius, xhi = (double)(blockIdx.y * 16 + 15) * dx + p.radius;
+    const double ylo = (double)(blockIdx.x * 16) * dy - p.radius, yhi = (double)(blockIdx.x * 16 + 15) * dy + p.radius;
+    int bxa = (int)floor(xlo / bwx), bxb = (int)floor(xhi / bwx);
+    int bya = (int)floor(ylo / bwy), byb = (int)floor(yhi / bwy);
+    if (bxb - bxa + 1 >= p.nbx) { bxa = 0; bxb = p.nbx - 1; }
+    if (byb - bya + 1 >= p.nby) { bya = 0; byb = p.nby - 1; }
+    double v = 0.0;
+    for (int bx = bxa; bx <= bxb; ++bx) {
+        const int bxw = ((bx % p.nbx) + p.nbx) % p.nbx;
+        for (int by = bya; by <= byb; ++by) {
+            const int byw = ((by % p.nby) + p.nby) % p.nby;
+            const int cell = (kz * p.nbx + bxw) * p.nby + byw;
+            const int i0 = p.cell_start[cell], i1 = p.cell_start[cell + 1];
+            for (int i = i0; i < i1; ++i) {
+                const AtomRec at = p.atoms[i];
+                double dX = fabs(at.x - xp);
+                if (dX >= p.a / 2.0) dX -= p.a;
+                double dY = fabs(at.y - yp);
+                if (dY >= p.b / 2.0) dY -= p.b;
+                const double r2 = dX * dX + dY * dY;
+                if (r2 < r2max) v += (double)at.occ * d_vz(at.Z, sqrt(r2), p.table, p.formula);
+            }
+        }
+    }
+    if (ix < p.nx && iy < p.ny) {
+        const long long o = ((long long)kz * p.nx + ix) * p.ny + iy;
+        if (p.v64) p.v64[o] = v;
+        if (p.v32) p.v32[o] = (float)v;
+    }
+}
+
+hipError_t launch_splat(const SplatArgs& a, hipStream_t s) {
+    dim3 grid((a.ny + 15) / 16, (a.nx + 15) / 16, a.nslices);
+    hipLaunchKernelGGL(splat_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+__global__ void det_finish_kernel(const double* partial, int ndet, int nb, int groups, double w, double* out,
+                                  long long out_stride_d, long long probe0) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= ndet * nb) return;
+    const int d = idx / nb, b = idx % nb;
+    double s = 0.0;
+    const double* q = partial + ((long long)d * nb + b) * groups;
+    for (int g = 0; g < groups; ++g) s += q[g];
+    out[d * out_stride_d + probe0 + b] = s * w;
+}
+
+hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups, double w, double* out,
+                             long long out_stride_d, long long probe0, hipStream_t s) {
+    const int n = ndet * nb;
+    hipLaunchKernelGGL(det_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, s, partial, ndet, nb, groups, w,
+                       out, out_stride_d, probe0);
+    return hipGetLastError();
+}
+
+}  // namespace msx

@@ -1,0 +1,332 @@
+"""ctypes binding of libmsx.so (include/msx.h) and the host-side mirror of the
+reference's LMultiSlice call surface.
+
+Functional API (numpy arrays = host buffers, torch CUDA tensors = device
+buffers; complex images have shape (..., nx, ny), y contiguous):
+
+    fft2d, potential, transmission, multislice, stem, ctf_image
+
+Class API mirroring the reference (ImageSimulation/LMultiSlice):
+
+    Microscope(keV, Cs, aperture, defocus)          Microscope.h:3-34
+    ModelPotential(model, nx, ny, nz, dpa, radius)  ModelPotential.h:27-51
+    ModelSimulated(potential, nx, ny, nz, dpa)      ModelSimulated.h:7-28
+    PotentialBuilder                                PotentialBuilder.h:14-22
+
+There is no CPU fallback: every call goes to the HIP library, and importing
+this module fails if libmsx.so has not been built.
+"""
+import ctypes as C
+import importlib.util
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmsx.so")
+
+F32, F64 = 0, 1
+HOST, DEVICE = 0, 1
+TABLE_LMULTISLICE, TABLE_KIRKLAND = 0, 1
+FORMULA_KIRKLAND, FORMULA_LMULTISLICE = 0, 1
+SIZES = (16, 32, 64, 128, 256, 384, 512, 768, 1024, 2048)
+ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESIZE", -5: "ESTATE"}
+
+
+class MsxError(RuntimeError):
+    """A libmsx call returned a negative status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"msx error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Atom(C.Structure):
+    _fields_ = [("Z", C.c_int32), ("occ", C.c_float), ("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+
+ATOM_DTYPE = np.dtype([("Z", np.int32), ("occ", np.float32), ("x", np.float64), ("y", np.float64),
+                       ("z", np.float64)], align=True)
+
+
+class Stack(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("ax", C.c_double), ("by", C.c_double),
+                ("kev", C.c_double), ("ctilt_x", C.c_double), ("ctilt_y", C.c_double), ("bw", C.c_double),
+                ("nlayer", C.c_int32), ("cz", C.POINTER(C.c_double)), ("nslice", C.c_int32),
+                ("layer", C.POINTER(C.c_int32))]
+
+
+class Probe(C.Structure):
+    _fields_ = [("alpha_max", C.c_double), ("alpha_min", C.c_double), ("cs", C.c_double), ("df", C.c_double)]
+
+
+class Detector(C.Structure):
+    _fields_ = [("k_inner", C.c_double), ("k_outer", C.c_double)]
+
+
+EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_last_error", "msx_version",
+           "msx_wavelength", "msx_sigma", "msx_vzatom", "msx_featom", "msx_fft2d", "msx_potential",
+           "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image"]
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _preload_hip_runtime():
+    """Make the process use ONE HIP runtime.  PyTorch-ROCm bundles its own
+    libamdhip64.so (same SONAME as /opt/rocm's); if libmsx.so pulled in the
+    system copy first, a later torch import would bring a second runtime that
+    cannot see the GPU.  Loading torch's copy globally first lets libmsx.so
+    (NEEDED libamdhip64.so.7) and torch share it.  MSX_HIP_RUNTIME=system
+    keeps the system runtime."""
+    if os.environ.get("MSX_HIP_RUNTIME") == "system":
+        return
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.origin:
+        return
+    path = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        C.CDLL(path, mode=C.RTLD_GLOBAL)
+
+
+def load():
+    """Load libmsx.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    _preload_hip_runtime()
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run python -m multislice_amd.build")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "msx_create": (i, [i, C.POINTER(vp)]), "msx_destroy": (i, [vp]), "msx_set_stream": (i, [vp, vp]),
+        "msx_last_error": (C.c_char_p, []), "msx_version": (i, []),
+        "msx_wavelength": (d, [d]), "msx_sigma": (d, [d]), "msx_vzatom": (d, [i, d, i, i]),
+        "msx_featom": (d, [i, d, i]),
+        "msx_fft2d": (i, [vp, vp, i, i, i, i, i, i]),
+        "msx_potential": (i, [vp, vp, C.c_int64, d, d, d, i, i, i, d, i, i, vp, i, i]),
+        "msx_transmission": (i, [vp, vp, i, i, i, d, d, d, d, vp, i, i]),
+        "msx_multislice": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i, i]),
+        "msx_stem": (i, [vp, C.POINTER(Stack), vp, C.POINTER(Probe), vp, i, C.POINTER(Detector), i, vp, i, d, i,
+                         vp, i, i, i]),
+        "msx_ctf_image": (i, [vp, vp, i, i, i, d, d, d, d, d, d, vp, vp, i, i]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise MsxError(rc, load().msx_last_error().decode(errors="replace"))
+
+
+# ------------------------------------------------------------------ buffers
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _buf(x, prec, complex_=True, writable=False):
+    """(pointer, where, keepalive, host_array_or_None) for a numpy array or a
+    torch CUDA tensor."""
+    if _is_torch(x):
+        if not x.is_cuda:
+            raise ValueError("torch tensors must be on the GPU (use numpy for host buffers)")
+        if not x.is_contiguous():
+            raise ValueError("torch tensor must be contiguous")
+        want = ({F32: "complex64", F64: "complex128"} if complex_ else {F32: "float32", F64: "float64"})[prec]
+        if str(x.dtype).split(".")[-1] != want:
+            raise ValueError(f"tensor dtype {x.dtype} != {want}")
+        return C.c_void_p(x.data_ptr()), DEVICE, x, None
+    dt = ({F32: np.complex64, F64: np.complex128} if complex_ else {F32: np.float32, F64: np.float64})[prec]
+    a = np.asarray(x)
+    if a.dtype != dt or not a.flags.c_contiguous:
+        if writable:
+            raise ValueError(f"output array must be C-contiguous {np.dtype(dt)}")
+        a = np.ascontiguousarray(a, dtype=dt)
+    return C.c_void_p(a.ctypes.data), HOST, a, a
+
+
+def _prec_of(x):
+    s = str(getattr(x, "dtype", ""))
+    return F64 if ("complex128" in s or "float64" in s) else F32
+
+
+def _ptr_where(args):
+    wh = {w for (_, w, _, _) in args}
+    if len(wh) > 1:
+        raise ValueError("mix of host and device buffers in one call")
+    return wh.pop()
+
+
+class Context:
+    """An msx context bound to one GPU (msx_create / msx_destroy)."""
+
+    def __init__(self, device=0):
+        L = load()
+        h = C.c_void_p()
+        _check(L.msx_create(int(device), C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            load().msx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream):
+        """Queue work on an external stream (torch.cuda.Stream or raw handle)."""
+        raw = getattr(stream, "cuda_stream", stream)
+        _check(load().msx_set_stream(self.h, C.c_void_p(raw) if raw else None))
+
+    # -------------------------------------------------------------- calls
+    def fft2d(self, data, sign):
+        """In-place cfpix::fft (sign=+1) / cfpix::ifft (sign=-1) on (..., nx, ny)."""
+        prec = _prec_of(data)
+        p, where, keep, arr = _buf(data, prec, writable=True)
+        nx, ny = data.shape[-2], data.shape[-1]
+        nb = int(np.prod(data.shape[:-2])) if len(data.shape) > 2 else 1
+        _check(load().msx_fft2d(self.h, p, nb, nx, ny, int(sign), prec, where))
+        return data
+
+    def potential(self, atoms, cell, nx, ny, nslices, radius, table=TABLE_LMULTISLICE,
+                  formula=FORMULA_KIRKLAND, prec=F32, out=None):
+        """ModelPotential::calculatePotentialGrid -> V[nslices][nx][ny] (V*Angstrom)."""
+        at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        if out is None:
+            out = np.zeros((nslices, nx, ny), np.float64 if prec == F64 else np.float32)
+        p, where, keep, arr = _buf(out, prec, complex_=False, writable=True)
+        _check(load().msx_potential(self.h, C.c_void_p(at.ctypes.data), len(at), float(cell[0]), float(cell[1]),
+                                    float(cell[2]), nx, ny, nslices, float(radius), table, formula, p, prec,
+                                    where))
+        return out
+
+    def transmission(self, V, ax, by, kev, bw=2.0 / 3.0, out=None):
+        """exp(i sigma V / 1000), band-limited; V: (nlayer, nx, ny) real."""
+        prec = _prec_of(V)
+        nl, nx, ny = V.shape
+        pv = _buf(V, prec, complex_=False)
+        if out is None:
+            if _is_torch(V):
+                import torch
+                out = torch.empty((nl, nx, ny), dtype=torch.complex128 if prec == F64 else torch.complex64,
+                                  device=V.device)
+            else:
+                out = np.zeros((nl, nx, ny), np.complex128 if prec == F64 else np.complex64)
+        pt = _buf(out, prec, writable=True)
+        where = _ptr_where([pv, pt])
+        _check(load().msx_transmission(self.h, pv[0], nl, nx, ny, ax, by, kev, bw, pt[0], prec, where))
+        return out
+
+    @staticmethod
+    def _stack(nx, ny, ax, by, kev, cz, layer, tilt=(0.0, 0.0), bw=2.0 / 3.0):
+        cz = np.ascontiguousarray(cz, np.float64)
+        layer = np.ascontiguousarray(layer, np.int32)
+        st = Stack(nx, ny, ax, by, kev, tilt[0], tilt[1], bw, len(cz),
+                   cz.ctypes.data_as(C.POINTER(C.c_double)), len(layer),
+                   layer.ctypes.data_as(C.POINTER(C.c_int32)))
+        return st, (cz, layer)
+
+    def multislice(self, trans, cz, layer, psi, ax, by, kev, tilt=(0.0, 0.0), bw=2.0 / 3.0, psi0=None):
+        """Slice loop (mulslice.cpp:703-739) on nwave waves psi (nwave, nx, ny) in place.
+        With psi0 (one image) every wave starts from psi0 and psi is output only."""
+        prec = _prec_of(trans)
+        nl, nx, ny = trans.shape
+        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
+        pt = _buf(trans, prec)
+        pw = _buf(psi, prec, writable=True)
+        bufs = [pt, pw]
+        p0 = None
+        if psi0 is not None:
+            b0 = _buf(psi0, prec)
+            bufs.append(b0)
+            p0 = b0[0]
+        where = _ptr_where(bufs)
+        nwave = int(np.prod(psi.shape[:-2])) if len(psi.shape) > 2 else 1
+        _check(load().msx_multislice(self.h, C.byref(st), pt[0], p0, pw[0], nwave, prec, where))
+        return psi
+
+    def stem(self, trans, cz, layer, ax, by, kev, probe, positions, detectors, checkpoints,
+             bw=2.0 / 3.0, batch=0, weight=1.0, out=None):
+        """STEM scan; returns out[ck][det][probe] (fraction of incident intensity).
+        probe: dict(alpha_max, alpha_min=0, cs=0, df=0) (rad, Angstrom);
+        detectors: [(k_inner, k_outer)] in 1/Angstrom."""
+        prec = _prec_of(trans)
+        nl, nx, ny = trans.shape
+        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, (0.0, 0.0), bw)
+        pt = _buf(trans, prec)
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 2)
+        det = np.ascontiguousarray(detectors, np.float64).reshape(-1, 2)
+        ck = np.ascontiguousarray(checkpoints, np.int32)
+        pr = Probe(float(probe["alpha_max"]), float(probe.get("alpha_min", 0.0)), float(probe.get("cs", 0.0)),
+                   float(probe.get("df", 0.0)))
+        dets = (Detector * len(det))(*[Detector(float(a), float(b)) for a, b in det])
+        acc = 1
+        if out is None:
+            out = np.zeros((len(ck), len(det), len(pos)), np.float64)
+            acc = 0
+        _check(load().msx_stem(self.h, C.byref(st), pt[0], C.byref(pr), C.c_void_p(pos.ctypes.data), len(pos),
+                               dets, len(det), C.c_void_p(ck.ctypes.data), len(ck), float(weight), acc,
+                               C.c_void_p(out.ctypes.data), int(batch), prec, pt[1]))
+        return out
+
+    def ctf_image(self, psi, ax, by, kev, cs, df, aobj, want_wave=False):
+        """Objective-lens image |ifft(fft(psi) exp(-i chi) A)|^2 of (nwave, nx, ny) waves."""
+        prec = _prec_of(psi)
+        nx, ny = psi.shape[-2], psi.shape[-1]
+        nw = int(np.prod(psi.shape[:-2])) if len(psi.shape) > 2 else 1
+        pw = _buf(psi, prec)
+        if pw[1] == DEVICE:
+            import torch
+            inten = torch.empty(psi.shape, dtype=torch.float64 if prec == F64 else torch.float32, device=psi.device)
+            img = torch.empty_like(psi) if want_wave else None
+        else:
+            inten = np.zeros(psi.shape, np.float64 if prec == F64 else np.float32)
+            img = np.zeros(psi.shape, psi.dtype) if want_wave else None
+        pi = _buf(inten, prec, complex_=False, writable=True)
+        pimg = _buf(img, prec, writable=True)[0] if want_wave else None
+        _check(load().msx_ctf_image(self.h, pw[0], nw, nx, ny, ax, by, kev, cs, df, aobj, pimg, pi[0], prec,
+                                    pw[1]))
+        return (inten, img) if want_wave else inten
+
+
+_default = None
+
+
+def context():
+    """Process-wide default context on device 0."""
+    global _default
+    with _lock:
+        if _default is None:
+            _default = Context(0)
+        return _default
+
+
+def wavelength(kev):
+    return load().msx_wavelength(kev)
+
+
+def sigma(kev):
+    return load().msx_sigma(kev)
+
+
+def vzatom(Z, r, table=TABLE_LMULTISLICE, formula=FORMULA_KIRKLAND):
+    return load().msx_vzatom(int(Z), float(r), int(table), int(formula))
+
+
+def featom(Z, k2, table=TABLE_LMULTISLICE):
+    return load().msx_featom(int(Z), float(k2), int(table))

@@ -1,0 +1,36 @@
+/*
+ * ref_harness.cpp -- C entry points over the reference's own CPU code, built
+ * by oracle/build_ref.sh into oracle/_ref/ (never committed, never shipped).
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle
+ * (oracle/msx_oracle.cpp) against the compiled reference.
+ *
+ * Two reference units are compiled from their sources under the reference
+ * tree, unmodified:
+ *   ImageSimulation/multislice/slicelib.cpp  (Kirkland slicelib: wavelength,
+ *       sigma, bessi0/bessk0, vzatom, featom, freqn, chi, ranflat, rangauss)
+ *   ImageSimulation/fftw-2.1.5/fftw/*.c     (the FFTW 2.1.5 library the
+ *       reference vendors, built in float mode)
+ * This file only forwards C-callable symbols to them.
+ */
+#include "slicelib.hpp"
+
+#define REF_API extern "C" __attribute__((visibility("default")))
+
+REF_API double ref_wavelength(double kev) { return wavelength(kev); }
+REF_API double ref_sigma(double kev) { return sigma(kev); }
+REF_API double ref_bessi0(double x) { return bessi0(x); }
+REF_API double ref_bessk0(double x) { return bessk0(x); }
+REF_API double ref_vzatom(int Z, double r) { return vzatom(Z, r); }
+REF_API double ref_featom(int Z, double k2) { return featom(Z, k2); }
+REF_API void ref_freqn(float *ko, float *ko2, float *xo, int nk, double ak) { freqn(ko, ko2, xo, nk, ak); }
+/* chi with only Cs and defocus set (multiMode 0): slicelib.cpp:258-318 */
+REF_API double ref_chi_csdf(double cs, double df, double alx, double aly) {
+    float p[64];  /* covers every slicelib.hpp parameter index (max 56) */
+    for (int i = 0; i < 64; i++) p[i] = 0.0f;
+    p[pCS] = (float)cs;
+    p[pDEFOCUS] = (float)df;
+    return chi(p, alx, aly, 0);
+}
+REF_API double ref_rangauss(unsigned long *seed) { return rangauss(seed); }
+REF_API double ref_ranflat(unsigned long *seed) { return ranflat(seed); }

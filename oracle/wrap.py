@@ -1,0 +1,256 @@
+"""ctypes wrappers over liboracle.so and the oracle/_ref reference builds.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).  All images use the
+layout of the reference's cfpix (x-major, y fastest): a complex image is a
+numpy array of shape (nx, ny).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+_ref = None
+_fftw2 = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load liboracle.so (built by `make -C oracle`)."""
+    global _lib
+    if _lib is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so not built (run make -C oracle)")
+        L = C.CDLL(path)
+        d, i, f, l = C.c_double, C.c_int, C.c_float, C.c_long
+        for name, res, args in [
+            ("orc_wavelength", d, [d]), ("orc_sigma", d, [d]),
+            ("orc_bessi0", d, [d]), ("orc_bessk0", d, [d]),
+            ("orc_vzatom", d, [i, d, i, i]), ("orc_featom", d, [i, d, i]),
+            ("orc_fparam", d, [i, i, i]),
+            ("orc_freqn", None, [_fp, _fp, _fp, i, d]),
+            ("orc_k2max", f, [i, i, d, d, d]),
+            ("orc_dft2d_f64", None, [_dp, i, i, i]),
+            ("orc_fft2d_f64", None, [_dp, i, i, i]),
+            ("orc_fft2d_f32", None, [_fp, i, i, i]),
+            ("orc_potential", i, [_ip, _dp, C.c_void_p, l, d, d, d, i, i, i, d, i, i, _dp]),
+            ("orc_transmission_f32", None, [_fp, i, i, i, d, d, d, d, _fp]),
+            ("orc_transmission_f64", None, [_dp, i, i, i, d, d, d, d, _dp]),
+            ("orc_propagator_f64", None, [i, i, d, d, d, d, d, d, _dp, _dp, _dp, _dp]),
+            ("orc_multislice_f32", i, [_fp, i, _fp, _ip, i, i, i, d, d, d, d, d, d, _fp, C.c_void_p]),
+            ("orc_multislice_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, _dp, C.c_void_p]),
+            ("orc_probe_f64", i, [i, i, d, d, d, d, d, d, d, d, d, _dp]),
+            ("orc_stem_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
+            ("orc_stem_f32", i, [_fp, i, _fp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
+            ("orc_ctf_image_f64", None, [_dp, i, i, d, d, d, d, d, d, C.c_void_p, C.c_void_p]),
+        ]:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref():
+    """Load oracle/_ref/libkirkland_ref.so (reference slicelib.cpp), or None."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(_HERE, "_ref", "libkirkland_ref.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        d, i = C.c_double, C.c_int
+        for name, res, args in [
+            ("ref_wavelength", d, [d]), ("ref_sigma", d, [d]),
+            ("ref_bessi0", d, [d]), ("ref_bessk0", d, [d]),
+            ("ref_vzatom", d, [i, d]), ("ref_featom", d, [i, d]),
+            ("ref_freqn", None, [_fp, _fp, _fp, i, d]),
+            ("ref_chi_csdf", d, [d, d, d, d]),
+            ("ref_rangauss", d, [C.POINTER(C.c_ulong)]),
+            ("ref_ranflat", d, [C.POINTER(C.c_ulong)]),
+        ]:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _ref = L
+    return _ref
+
+
+def ref_fftw2():
+    """Load oracle/_ref/libfftw2f_ref.so (reference-vendored FFTW 2.1.5), or None."""
+    global _fftw2
+    if _fftw2 is None:
+        path = os.path.join(_HERE, "_ref", "libfftw2f_ref.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        L.ref_fftw2_2d.restype = C.c_int
+        L.ref_fftw2_2d.argtypes = [_fp, C.c_int, C.c_int, C.c_int]
+        _fftw2 = L
+    return _fftw2
+
+
+def _c128(a):
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.complex64)
+
+
+class Oracle:
+    """numpy-level interface to the oracle (all static)."""
+
+    @staticmethod
+    def wavelength(kev):
+        return lib().orc_wavelength(kev)
+
+    @staticmethod
+    def sigma(kev):
+        return lib().orc_sigma(kev)
+
+    @staticmethod
+    def vzatom(Z, r, table=1, formula=0):
+        return lib().orc_vzatom(int(Z), float(r), int(table), int(formula))
+
+    @staticmethod
+    def featom(Z, k2, table=1):
+        return lib().orc_featom(int(Z), float(k2), int(table))
+
+    @staticmethod
+    def freqn(n, a):
+        ko = np.zeros(n, np.float32)
+        ko2 = np.zeros(n, np.float32)
+        xo = np.zeros(n, np.float32)
+        lib().orc_freqn(ko, ko2, xo, n, a)
+        return ko, ko2, xo
+
+    @staticmethod
+    def k2max(nx, ny, ax, by, bw=2.0 / 3.0):
+        return lib().orc_k2max(nx, ny, ax, by, bw)
+
+    @staticmethod
+    def dft2(x, sign):
+        a = _c128(x).copy()
+        lib().orc_dft2d_f64(a.view(np.float64).reshape(-1), a.shape[0], a.shape[1], sign)
+        return a
+
+    @staticmethod
+    def fft2(x, sign, f32=False):
+        if f32:
+            a = _c64(x).copy()
+            lib().orc_fft2d_f32(a.view(np.float32).reshape(-1), a.shape[0], a.shape[1], sign)
+        else:
+            a = _c128(x).copy()
+            lib().orc_fft2d_f64(a.view(np.float64).reshape(-1), a.shape[0], a.shape[1], sign)
+        return a
+
+    @staticmethod
+    def potential(Z, xyz, cell, nx, ny, nslices, radius, table=0, formula=0, occ=None):
+        Z = np.ascontiguousarray(Z, np.int32)
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        V = np.zeros((nslices, nx, ny), np.float64)
+        occ_p = None
+        if occ is not None:
+            occ = np.ascontiguousarray(occ, np.float64)
+            occ_p = occ.ctypes.data
+        rc = lib().orc_potential(Z, xyz, occ_p, len(Z), cell[0], cell[1], cell[2], nx, ny,
+                                 nslices, radius, table, formula, V.reshape(-1))
+        if rc != 0:
+            raise ValueError("orc_potential failed %d" % rc)
+        return V
+
+    @staticmethod
+    def transmission(V, ax, by, kev, bw=2.0 / 3.0, f32=False):
+        V = np.asarray(V)
+        nl, nx, ny = V.shape
+        if f32:
+            t = np.zeros((nl, nx, ny), np.complex64)
+            lib().orc_transmission_f32(np.ascontiguousarray(V, np.float32).reshape(-1), nl, nx, ny,
+                                       ax, by, kev, bw, t.view(np.float32).reshape(-1))
+        else:
+            t = np.zeros((nl, nx, ny), np.complex128)
+            lib().orc_transmission_f64(np.ascontiguousarray(V, np.float64).reshape(-1), nl, nx, ny,
+                                       ax, by, kev, bw, t.view(np.float64).reshape(-1))
+        return t
+
+    @staticmethod
+    def propagator(nx, ny, ax, by, kev, cz, tilt=(0.0, 0.0)):
+        pxr, pxi = np.zeros(nx), np.zeros(nx)
+        pyr, pyi = np.zeros(ny), np.zeros(ny)
+        lib().orc_propagator_f64(nx, ny, ax, by, kev, cz, tilt[0], tilt[1], pxr, pxi, pyr, pyi)
+        return pxr + 1j * pxi, pyr + 1j * pyi
+
+    @staticmethod
+    def multislice(trans, cz, layer, wave, ax, by, kev, tilt=(0.0, 0.0), bw=2.0 / 3.0,
+                   f32=False, want_intensity=False):
+        trans = np.asarray(trans)
+        nl, nx, ny = trans.shape
+        layer = np.ascontiguousarray(layer, np.int32)
+        inten = np.zeros(len(layer), np.float64)
+        ip = inten.ctypes.data if want_intensity else None
+        if f32:
+            w = _c64(wave).copy()
+            rc = lib().orc_multislice_f32(_c64(trans).view(np.float32).reshape(-1), nl,
+                                          np.ascontiguousarray(cz, np.float32), layer, len(layer),
+                                          nx, ny, ax, by, kev, tilt[0], tilt[1], bw,
+                                          w.view(np.float32).reshape(-1), ip)
+        else:
+            w = _c128(wave).copy()
+            rc = lib().orc_multislice_f64(_c128(trans).view(np.float64).reshape(-1), nl,
+                                          np.ascontiguousarray(cz, np.float64), layer, len(layer),
+                                          nx, ny, ax, by, kev, tilt[0], tilt[1], bw,
+                                          w.view(np.float64).reshape(-1), ip)
+        if rc != 0:
+            raise ValueError("orc_multislice failed %d" % rc)
+        return (w, inten) if want_intensity else w
+
+    @staticmethod
+    def probe(nx, ny, ax, by, kev, alpha_max, alpha_min=0.0, cs=0.0, df=0.0, pos=(0.0, 0.0)):
+        w = np.zeros((nx, ny), np.complex128)
+        rc = lib().orc_probe_f64(nx, ny, ax, by, kev, alpha_max, alpha_min, cs, df, pos[0], pos[1],
+                                 w.view(np.float64).reshape(-1))
+        if rc != 0:
+            raise ValueError("orc_probe failed %d" % rc)
+        return w
+
+    @staticmethod
+    def stem(trans, cz, layer, ax, by, kev, alpha_max, positions, detectors, checkpoints,
+             alpha_min=0.0, cs=0.0, df=0.0, bw=2.0 / 3.0, nthreads=1, f32=False):
+        trans = np.asarray(trans)
+        nl, nx, ny = trans.shape
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        det = np.ascontiguousarray(detectors, np.float64).reshape(-1)
+        ck = np.ascontiguousarray(checkpoints, np.int32)
+        npr, nd, nck = len(pos) // 2, len(det) // 2, len(ck)
+        out = np.zeros((nck, nd, npr), np.float64)
+        layer = np.ascontiguousarray(layer, np.int32)
+        if f32:
+            rc = lib().orc_stem_f32(_c64(trans).view(np.float32).reshape(-1), nl,
+                                    np.ascontiguousarray(cz, np.float32), layer, len(layer), nx, ny,
+                                    ax, by, kev, bw, alpha_max, alpha_min, cs, df, pos, npr, det, nd,
+                                    ck, nck, out.reshape(-1), nthreads)
+        else:
+            rc = lib().orc_stem_f64(_c128(trans).view(np.float64).reshape(-1), nl,
+                                    np.ascontiguousarray(cz, np.float64), layer, len(layer), nx, ny,
+                                    ax, by, kev, bw, alpha_max, alpha_min, cs, df, pos, npr, det, nd,
+                                    ck, nck, out.reshape(-1), nthreads)
+        if rc != 0:
+            raise ValueError("orc_stem failed %d" % rc)
+        return out
+
+    @staticmethod
+    def ctf_image(wave, ax, by, kev, cs, df, aobj):
+        w = _c128(wave)
+        nx, ny = w.shape
+        img_w = np.zeros((nx, ny), np.complex128)
+        img = np.zeros((nx, ny), np.float64)
+        lib().orc_ctf_image_f64(w.view(np.float64).reshape(-1), nx, ny, ax, by, kev, cs, df, aobj,
+                                img_w.ctypes.data, img.ctypes.data)
+        return img_w, img
