@@ -71,6 +71,16 @@ MSX_API int msx_set_stream(msx_ctx ctx, void* hip_stream);
 MSX_API const char* msx_last_error(void);
 MSX_API int msx_version(void);
 
+/* Kernel timing (the reference timed its loops with cudaEvents and printed
+ * them, ModelPotential.cu:46-50,92-96 and ModelSimulated.cu:43-47,178-182).
+ * With profiling on, every kernel launch of the context is bracketed by HIP
+ * events on the context stream and waited for, and its time and algorithmic
+ * HBM bytes are added to per-kind totals (enabling resets them).  This
+ * serialises launches: use it for measurement passes, not production runs. */
+enum { MSX_PROF_ROW = 0, MSX_PROF_COL = 1, MSX_PROF_SPLAT = 2, MSX_PROF_DET = 3, MSX_PROF_KINDS = 4 };
+MSX_API int msx_profile(msx_ctx ctx, int enable);
+MSX_API int msx_profile_read(msx_ctx ctx, int kind, int64_t* launches, double* total_ms, double* total_bytes);
+
 /* ---- scalars ---------------------------------------------------------- */
 /* Microscope::wavelength / Sigma (Microscope.cpp:69-86) = Kirkland
  * wavelength / sigma (slicelib.cpp:2795-2806, 2387-2401). */

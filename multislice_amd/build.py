@@ -43,8 +43,11 @@ def _run(cmd):
     return err
 
 
-def build(jobs=None, verbose=True):
-    bdir = os.path.join(HERE, "_build")
+def build(jobs=None, verbose=True, defines=(), out=None, bdir=None):
+    """Compile every unit and link libmsx.so.  `defines` (e.g. MSX_COL_TW=8)
+    and `out` build a shape variant elsewhere (used by the tuner)."""
+    out = out or OUT
+    bdir = bdir or os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
     cc = hipcc()
     units = []
@@ -55,15 +58,17 @@ def build(jobs=None, verbose=True):
         units.append((os.path.join(CSRC, name + ".hip"), os.path.join(bdir, name + ".o"), []))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:
-        futs = [ex.submit(_run, [cc] + flags() + extra + ["-c", src, "-o", obj]) for src, obj, extra in units]
+        dflags = ["-D" + d for d in defines]
+        futs = [ex.submit(_run, [cc] + flags() + dflags + extra + ["-c", src, "-o", obj])
+                for src, obj, extra in units]
         for f in futs:
             f.result()
-    tmp = OUT + ".tmp"
+    tmp = out + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [u[1] for u in units])
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print("built", OUT)
-    return OUT
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":

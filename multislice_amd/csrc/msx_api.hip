@@ -92,6 +92,12 @@ int fidx(int i, int n) {
     int imid = (int)(n / 2.0 + 0.5);
     return i > imid ? i - n : i;
 }
+// fraction of y columns inside the band (host copy of the kernels' test)
+double band_fraction(const float* ky2_host, int ny, float k2max) {
+    int k = 0;
+    for (int i = 0; i < ny; i++) k += ky2_host[i] < k2max;
+    return (double)k / ny;
+}
 // mulslice.cpp:481-489 (float arithmetic)
 float k2max_of(int nx, int ny, double ax, double by, double bw) {
     float k2max = nx / (2.0F * (float)ax);
@@ -107,6 +113,32 @@ struct msx_ctx_s {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
+    // kernel profiling (msx_profile): per kernel kind launch count and time
+    bool prof = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    long long prof_n[MSX_PROF_KINDS] = {0};
+    double prof_ms[MSX_PROF_KINDS] = {0};
+    double prof_bytes[MSX_PROF_KINDS] = {0};
+    double band = 1.0;  // in-band fraction of y columns of the current slice loop
+
+    // Launch through `go`; with profiling on, bracket it with events on the
+    // launch stream and wait for it (serialises the stream: diagnostics only).
+    template <typename F> int timed(int kind, double bytes, F&& go) {
+        if (!prof) {
+            HIPCHK(go());
+            return MSX_OK;
+        }
+        HIPCHK(hipEventRecord(ev0, stream));
+        HIPCHK(go());
+        HIPCHK(hipEventRecord(ev1, stream));
+        HIPCHK(hipEventSynchronize(ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+        prof_n[kind]++;
+        prof_ms[kind] += ms;
+        prof_bytes[kind] += bytes;
+        return MSX_OK;
+    }
     std::map<int, DevBuf> tw32, tw64;
     DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp;
 
@@ -125,6 +157,8 @@ struct msx_ctx_s {
         for (auto& kv : tw32) kv.second.release();
         for (auto& kv : tw64) kv.second.release();
         for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp}) b->release();
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
         if (own) hipStreamDestroy(own);
     }
 };
@@ -194,6 +228,7 @@ int build_tables(msx_ctx ctx, const msx_stack* st, bool with_prop, SliceTables<C
     out->px = reinterpret_cast<const C*>(base + o_px);
     out->py = reinterpret_cast<const C*>(base + o_py);
     out->k2max = k2max_of(nx, ny, st->ax, st->by, st->bw);
+    ctx->band = band_fraction(ky2.data(), ny, out->k2max);
     return MSX_OK;
 }
 
@@ -218,8 +253,24 @@ template <typename C> ColArgs<C> col_args(C* psi, int ny, int nb, const C* tw) {
     return c;
 }
 
-#define LAUNCH_ROW(n, mode, args) HIPCHK(launch_row<C>((n), (mode), (args), ctx->stream))
-#define LAUNCH_COL(n, mode, args) HIPCHK(launch_col<C>((n), (mode), (args), ctx->stream))
+// Algorithmic HBM bytes of one sweep launch: every wave pixel read and
+// written once (unless not stored), plus one shared transmission image.
+template <typename C> double row_bytes(const RowArgs<C>& a, int n, double band) {
+    const double px = (double)a.nx * n;
+    const double fin = (a.prune & 1) ? band : 1.0, fout = (a.prune & 2) ? band : 1.0;
+    return px * a.nb * sizeof(C) * (fin + fout) + (a.trans ? px * sizeof(C) : 0.0) +
+           (a.vreal ? px * a.nb * (0.5 * (double)sizeof(C) - (double)sizeof(C)) : 0.0);
+}
+template <typename C> double col_bytes(const ColArgs<C>& a, int n, double band) {
+    const double px = (double)a.ny * n * a.nb * (a.prune ? band : 1.0);
+    return px * (((a.ops & kColGen) ? 0.0 : sizeof(C)) + ((a.ops & kColStore) ? sizeof(C) : 0.0));
+}
+#define LAUNCH_ROW(n, mode, args) \
+    CHK(ctx->timed(MSX_PROF_ROW, ctx->prof ? row_bytes<C>((args), (n), ctx->band) : 0.0, \
+                   [&] { return launch_row<C>((n), (mode), (args), ctx->stream); }))
+#define LAUNCH_COL(n, mode, args) \
+    CHK(ctx->timed(MSX_PROF_COL, ctx->prof ? col_bytes<C>((args), (n), ctx->band) : 0.0, \
+                   [&] { return launch_col<C>((n), (mode), (args), ctx->stream); }))
 
 // ---------------------------------------------------------------- fft2d
 template <typename C> int fft2d_dev(msx_ctx ctx, C* d, int nb, int nx, int ny, int sign) {
@@ -306,8 +357,15 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
     int ick = 0;
     for (int s = 0; s < st->nslice; s++) {
         const int l = st->layer[s];
+        // Band pruning (exact): after a column sweep every y column with
+        // ky2 >= k2max is identically zero, so row sweeps neither load those
+        // inputs (prune & 1, from the second slice on) nor store those outputs
+        // (prune & 2: the next column sweep skips them).
         RowArgs<C> r = row_args<C>(psi, nx, nb, twy);
         r.trans = trans + (long long)l * img;
+        r.ky2 = tb.ky2;
+        r.k2max = tb.k2max;
+        r.prune = (s == 0) ? 2 : 3;
         if (s == 0 && start == 0) {
             r.src = src;
             r.src_bstride = src_bstride;
@@ -317,6 +375,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
             LAUNCH_ROW(ny, kRowSlice, r);
         }
         ColArgs<C> c = col_args<C>(psi, ny, nb, twx);
+        c.prune = 1;
         c.kx2 = tb.kx2;
         c.ky2 = tb.ky2;
         c.k2max = tb.k2max;
@@ -341,9 +400,11 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
             LAUNCH_COL(nx, kColSlice, c);
         }
         if (is_ck) {
-            HIPCHK(launch_det_finish(c.partial, det->ndet, nb, groups, 1.0 / ((double)nx * ny),
-                                     det->out + (long long)ick * det->ndet * det->nprobe, det->nprobe,
-                                     det->probe0, ctx->stream));
+            CHK(ctx->timed(MSX_PROF_DET, 8.0 * det->ndet * nb * (groups + 1), [&] {
+                return launch_det_finish(c.partial, det->ndet, nb, groups, 1.0 / ((double)nx * ny),
+                                         det->out + (long long)ick * det->ndet * det->nprobe, det->nprobe,
+                                         det->probe0, ctx->stream);
+            }));
             ick++;
         }
         if (last) return MSX_OK;
@@ -351,6 +412,9 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
     if (!det) {
         RowArgs<C> r = row_args<C>(psi, nx, nb, twy);
         r.scale = inv;
+        r.ky2 = tb.ky2;
+        r.k2max = tb.k2max;
+        r.prune = 1;
         LAUNCH_ROW(ny, kRowInv, r);
     }
     return MSX_OK;
@@ -608,7 +672,7 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     sa.nby = nby;
     sa.table = table;
     sa.formula = formula;
-    HIPCHK(launch_splat(sa, ctx->stream));
+    CHK(ctx->timed(MSX_PROF_SPLAT, (double)vbytes, [&] { return launch_splat(sa, ctx->stream); }));
     if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(V, v_dev, vbytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MSX_OK;
@@ -652,6 +716,31 @@ MSX_API int msx_destroy(msx_ctx ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     delete ctx;
+    return MSX_OK;
+}
+
+MSX_API int msx_profile(msx_ctx ctx, int enable) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (enable && !ctx->ev0) {
+        HIPCHK(hipEventCreate(&ctx->ev0));
+        HIPCHK(hipEventCreate(&ctx->ev1));
+    }
+    ctx->prof = enable != 0;
+    for (int k = 0; k < MSX_PROF_KINDS; k++) {
+        ctx->prof_n[k] = 0;
+        ctx->prof_ms[k] = 0;
+        ctx->prof_bytes[k] = 0;
+    }
+    return MSX_OK;
+}
+
+MSX_API int msx_profile_read(msx_ctx ctx, int kind, int64_t* launches, double* total_ms, double* total_bytes) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    if (kind < 0 || kind >= MSX_PROF_KINDS) return fail(MSX_EINVAL, "kind %d out of range", kind);
+    if (launches) *launches = ctx->prof_n[kind];
+    if (total_ms) *total_ms = ctx->prof_ms[kind];
+    if (total_bytes) *total_bytes = ctx->prof_bytes[kind];
     return MSX_OK;
 }
 

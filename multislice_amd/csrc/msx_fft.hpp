@@ -43,6 +43,18 @@ template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
 template <typename C> __device__ __forceinline__ C cscale(C a, typename real_of<C>::type s) {
     return mk<C>(a.x * s, a.y * s);
 }
+// x if keep else 0, as a bitwise AND on the loaded bits: the load stays
+// unconditional (a select lets the compiler turn it into a branch-guarded
+// load, whose control-flow join then forces a wait on every outstanding load)
+__device__ __forceinline__ float2 keep_or_zero(float2 x, unsigned keep) {
+    return make_float2(__uint_as_float(__float_as_uint(x.x) & keep), __uint_as_float(__float_as_uint(x.y) & keep));
+}
+__device__ __forceinline__ double2 keep_or_zero(double2 x, unsigned keep) {
+    const unsigned long long k = keep ? ~0ull : 0ull;
+    return make_double2(__longlong_as_double(__double_as_longlong(x.x) & k),
+                        __longlong_as_double(__double_as_longlong(x.y) & k));
+}
+
 // a * (S*i)
 template <int S, typename C> __device__ __forceinline__ C mul_si(C a) {
     return (S > 0) ? mk<C>(-a.y, a.x) : mk<C>(a.y, -a.x);
@@ -75,7 +87,26 @@ template <int R, int S, typename C> __device__ __forceinline__ C root(int m) {
     const int j = ((m % R) * (48 / R)) % 48;
     return mk<C>((Re)Roots48::c[j], (Re)(S * Roots48::s(j)));
 }
-template <typename C> __device__ __forceinline__ bool is_one(int j) { return j == 0; }
+// x * omega_R^m with the trivial roots (+-1, +-i) done as moves/negations and
+// the odd multiples of pi/4 as two adds and two multiplies (IEEE rules forbid
+// the compiler from folding multiplications by 0 or 1 itself).
+template <int R, int S, typename C> __device__ __forceinline__ C mul_root(C x, int m) {
+    using Re = typename real_of<C>::type;
+    const int j = ((m % R) * (48 / R)) % 48;  // angle in units of 2*pi/48
+    const int jj = S > 0 ? j : (48 - j) % 48;
+    const Re h = (Re)0.7071067811865475244008444;
+    switch (jj) {
+        case 0: return x;
+        case 12: return mk<C>(-x.y, x.x);    // * i
+        case 24: return mk<C>(-x.x, -x.y);   // * -1
+        case 36: return mk<C>(x.y, -x.x);    // * -i
+        case 6: return mk<C>(h * (x.x - x.y), h * (x.x + x.y));    // * (1+i)/sqrt2
+        case 18: return mk<C>(-h * (x.x + x.y), h * (x.x - x.y));  // * (-1+i)/sqrt2
+        case 30: return mk<C>(h * (x.y - x.x), -h * (x.x + x.y));  // * (-1-i)/sqrt2
+        case 42: return mk<C>(h * (x.x + x.y), h * (x.y - x.x));   // * (1-i)/sqrt2
+        default: return cmul(x, root<R, S, C>(m));
+    }
+}
 
 // ---------------------------------------------------------------- small DFTs
 // In-place DFT of R values in registers, natural order in and out.
@@ -127,10 +158,7 @@ template <int A, int B, int S, typename C> struct DftCT {
             for (int n1 = 0; n1 < A; ++n1) u[n1] = x[B * n1 + n2];
             Dft<A, S, C>::run(u);
 #pragma unroll
-            for (int k1 = 0; k1 < A; ++k1) {
-                const int m = n2 * k1;
-                y[n2 * A + k1] = (m == 0) ? u[k1] : cmul(u[k1], root<R, S, C>(m));
-            }
+            for (int k1 = 0; k1 < A; ++k1) y[n2 * A + k1] = mul_root<R, S>(u[k1], n2 * k1);
         }
 #pragma unroll
         for (int k1 = 0; k1 < A; ++k1) {
@@ -149,25 +177,32 @@ template <int S, typename C> struct Dft<12, S, C> : DftCT<3, 4, S, C> {};
 template <int S, typename C> struct Dft<16, S, C> : DftCT<4, 4, S, C> {};
 
 // ---------------------------------------------------------------- plans
-// Radix sequence and threads per line for each supported length.
+// A plan is a radix sequence and the number of threads T per line.
+template <int NN, int TT, int... RR> struct PlanT {
+    static constexpr int N = NN, T = TT;
+    static constexpr int R[] = {RR...};
+    static constexpr int P = sizeof...(RR);
+};
+// Row-sweep plans (contiguous lines; T divides 64 where possible so a line
+// stays inside one wavefront).
 template <int N> struct Plan;
-#define MSX_PLAN(NN, TT, ...)                                  \
-    template <> struct Plan<NN> {                              \
-        static constexpr int N = NN, T = TT;                   \
-        static constexpr int R[] = {__VA_ARGS__};              \
-        static constexpr int P = sizeof(R) / sizeof(R[0]);     \
-    };
-MSX_PLAN(16, 4, 4, 4)
-MSX_PLAN(32, 4, 4, 8)
-MSX_PLAN(64, 8, 8, 8)
-MSX_PLAN(128, 8, 16, 8)
-MSX_PLAN(256, 16, 16, 16)
-MSX_PLAN(384, 16, 6, 8, 8)
-MSX_PLAN(512, 64, 8, 8, 8)
-MSX_PLAN(768, 32, 12, 8, 8)
-MSX_PLAN(1024, 64, 16, 8, 8)
-MSX_PLAN(2048, 128, 16, 16, 8)
-#undef MSX_PLAN
+template <> struct Plan<16> : PlanT<16, 4, 4, 4> {};
+template <> struct Plan<32> : PlanT<32, 4, 4, 8> {};
+template <> struct Plan<64> : PlanT<64, 8, 8, 8> {};
+template <> struct Plan<128> : PlanT<128, 8, 16, 8> {};
+template <> struct Plan<256> : PlanT<256, 16, 16, 16> {};
+template <> struct Plan<384> : PlanT<384, 16, 6, 8, 8> {};
+template <> struct Plan<512> : PlanT<512, 64, 8, 8, 8> {};
+template <> struct Plan<768> : PlanT<768, 32, 12, 8, 8> {};
+template <> struct Plan<1024> : PlanT<1024, 64, 16, 8, 8> {};
+template <> struct Plan<2048> : PlanT<2048, 128, 16, 16, 8> {};
+// Column-sweep plans: fewer threads per line (more elements per thread) so a
+// work-group of TW columns stays within 512 threads and its register budget.
+#ifndef MSX_COL512_T
+#define MSX_COL512_T 32
+#endif
+template <int N> struct ColPlan : Plan<N> {};
+template <> struct ColPlan<512> : PlanT<512, MSX_COL512_T, 8, 8, 8> {};
 
 // ---------------------------------------------------------------- LDS views
 // Element m of the line owned by this thread, in LDS.
@@ -180,65 +215,130 @@ template <typename C> struct LdsView {
     __device__ __forceinline__ C* at(int m) const { return base + (padded ? (m + (m >> 4)) : m * stride); }
 };
 
-// ---------------------------------------------------------------- passes
-// Twiddle table tw[m] = exp(+2*pi*i*m/N) (m < N), generated in FP64 on the
-// host; the S = -1 transform uses the conjugate.
-template <int N, int S, typename C> __device__ __forceinline__ C twiddle(const C* __restrict__ tw, int m) {
-    C w = tw[m];
-    if (S < 0) w.y = -w.y;
-    return w;
+// ---------------------------------------------------------------- twiddles
+// Twiddle table tw[m] = exp(+2*pi*i*m/N) (m < N) is generated in FP64 on the
+// host.  Each thread needs, per pass with Ns > 1, the factors
+// W^(r*k*N/(Ns*R)) of its Q butterflies (k = (t + q*T) mod Ns): they are
+// loaded once into registers at kernel start and reused by every transform
+// the thread performs (forward ones directly, inverse ones conjugated).
+template <class PL> constexpr int tw_offset(int upto) {
+    int c = 0, ns = 1;
+    for (int i = 0; i < upto; ++i) {
+        const int R = PL::R[i], Q = PL::N / (R * PL::T);
+        if (ns > 1) c += Q * (R - 1);
+        ns *= R;
+    }
+    return c;
+}
+template <class PL> constexpr int tw_count() { return tw_offset<PL>(PL::P); }
+
+template <class PL, typename C> struct TwReg {
+    static constexpr int K = tw_count<PL>() > 0 ? tw_count<PL>() : 1;
+    C w[K];
+};
+
+template <class PL, int I, int Ns, typename C>
+__device__ __forceinline__ void load_tw_pass(C* w, int t, const C* __restrict__ table) {
+    if constexpr (I < PL::P) {
+        constexpr int R = PL::R[I], Q = PL::N / (R * PL::T);
+        if constexpr (Ns > 1) {
+            constexpr int off = tw_offset<PL>(I), step = PL::N / (Ns * R);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int k = (t + q * PL::T) % Ns;
+#pragma unroll
+                for (int r = 1; r < R; ++r) w[off + q * (R - 1) + r - 1] = table[r * k * step];
+            }
+        }
+        load_tw_pass<PL, I + 1, Ns * R>(w, t, table);
+    }
+}
+template <class PL, typename C>
+__device__ __forceinline__ void load_twiddles(TwReg<PL, C>& tw, int t, const C* __restrict__ table) {
+    load_tw_pass<PL, 0, 1>(tw.w, t, table);
 }
 
-template <int N, int T, int R, int Ns, int S, bool LAST, typename C>
-__device__ __forceinline__ void stockham_pass(C (&a)[N / T], int t, const LdsView<C>& lds,
-                                              const C* __restrict__ tw) {
+// Work-group barrier that orders LDS only: the fences are restricted to the
+// "local" address space, so outstanding global loads (the prefetch of the
+// next tile) and stores stay in flight across it.  __syncthreads() would
+// wait for every outstanding global access (vmcnt(0)) at each exchange.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// LDS synchronisation between the write and read halves of an exchange.
+// WAVE = true when every line lies inside one wavefront (T divides 64):
+// then a wavefront barrier suffices (the __syncwarp pattern, LDS only).
+template <bool WAVE> __device__ __forceinline__ void lds_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+    } else {
+        lds_barrier();
+    }
+}
+
+// ---------------------------------------------------------------- passes
+// DB (double buffering): exchange x of a line uses LDS buffer x & 1, so the
+// write of exchange x+1 cannot overtake reads of exchange x and one barrier
+// per exchange suffices; X is the running exchange index.
+template <int N, int T, int R, int Ns, int S, bool LAST, bool WAVE, bool DB, int X, int TWOFF, typename C>
+__device__ __forceinline__ void stockham_pass(C (&a)[N / T], int t, const LdsView<C> (&lds)[2], const C* w) {
     constexpr int Q = N / (R * T);
     static_assert(Q * R * T == N, "radix/thread split must tile N");
+    const LdsView<C>& L = lds[DB ? (X & 1) : 0];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         C b[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) b[r] = a[q + r * Q];
         const int j = t + q * T;
-        const int k = j % Ns;
-        if (Ns > 1) {
-            constexpr int step = N / (Ns * R);
+        if constexpr (Ns > 1) {
 #pragma unroll
-            for (int r = 1; r < R; ++r) b[r] = cmul(b[r], twiddle<N, S>(tw, r * k * step));
+            for (int r = 1; r < R; ++r) {
+                C f = w[TWOFF + q * (R - 1) + r - 1];
+                if (S < 0) f.y = -f.y;
+                b[r] = cmul(b[r], f);
+            }
         }
         Dft<R, S, C>::run(b);
-        if (LAST) {
+        if constexpr (LAST) {
 #pragma unroll
             for (int r = 0; r < R; ++r) a[q + r * Q] = b[r];
         } else {
-            const int base = (j / Ns) * Ns * R + k;
+            const int base = (j / Ns) * Ns * R + (j % Ns);
 #pragma unroll
-            for (int r = 0; r < R; ++r) *lds.at(base + r * Ns) = b[r];
+            for (int r = 0; r < R; ++r) *L.at(base + r * Ns) = b[r];
         }
     }
-    if (!LAST) {
-        __syncthreads();
+    if constexpr (!LAST) {
+        lds_sync<WAVE>();
 #pragma unroll
-        for (int e = 0; e < N / T; ++e) a[e] = *lds.at(t + e * T);
-        __syncthreads();
+        for (int e = 0; e < N / T; ++e) a[e] = *L.at(t + e * T);
+        if constexpr (!DB) lds_sync<WAVE>();
     }
 }
 
-template <class PL, int I, int Ns, int S, typename C>
-__device__ __forceinline__ void run_passes(C (&a)[PL::N / PL::T], int t, const LdsView<C>& lds,
-                                           const C* __restrict__ tw) {
+template <class PL, int I, int Ns, int S, bool WAVE, bool DB, int X, typename C>
+__device__ __forceinline__ void run_passes(C (&a)[PL::N / PL::T], int t, const LdsView<C> (&lds)[2], const C* w) {
     if constexpr (I < PL::P) {
         constexpr int R = PL::R[I];
-        stockham_pass<PL::N, PL::T, R, Ns, S, I == PL::P - 1>(a, t, lds, tw);
-        run_passes<PL, I + 1, Ns * R, S>(a, t, lds, tw);
+        constexpr bool LAST = I == PL::P - 1;
+        stockham_pass<PL::N, PL::T, R, Ns, S, LAST, WAVE, DB, X, tw_offset<PL>(I)>(a, t, lds, w);
+        run_passes<PL, I + 1, Ns * R, S, WAVE, DB, LAST ? X : X + 1>(a, t, lds, w);
     }
 }
 
-// Full length-N transform of the registers a[] (unscaled), sign S.
-template <int N, int S, typename C>
-__device__ __forceinline__ void fft_line(C (&a)[N / Plan<N>::T], int t, const LdsView<C>& lds,
-                                         const C* __restrict__ tw) {
-    run_passes<Plan<N>, 0, 1, S>(a, t, lds, tw);
+// Full length-N transform of the registers a[] (unscaled), sign S.  With DB,
+// X0 is the index of the transform's first exchange; a transform performs
+// PL::P - 1 exchanges.
+template <class PL, int S, bool WAVE, bool DB = false, int X0 = 0, typename C>
+__device__ __forceinline__ void fft_line(C (&a)[PL::N / PL::T], int t, const LdsView<C> (&lds)[2],
+                                         const TwReg<PL, C>& tw) {
+    run_passes<PL, 0, 1, S, WAVE, DB, X0>(a, t, lds, tw.w);
 }
 
 }  // namespace msx

@@ -15,16 +15,26 @@ namespace msx {
 
 namespace {
 constexpr int kN = MSX_N;
-constexpr int kT = Plan<kN>::T;
+constexpr int kT = Plan<kN>::T;      // threads per row line
+constexpr int kTc = ColPlan<kN>::T;  // threads per column line
 
 // lines per work-group of the row sweep: 256 threads (4 wavefronts)
 constexpr int row_lpw() { return kT >= 256 ? 1 : 256 / kT; }
 
 // columns per work-group of the column sweep (coalescing width), bounded by
 // 1024 threads and by the LDS tile N*(TW+1) complex values
+#ifndef MSX_COL_TW
+#define MSX_COL_TW 16
+#endif
+#ifndef MSX_COL_DB
+#define MSX_COL_DB 1
+#endif
+#ifndef MSX_COL_THREADS
+#define MSX_COL_THREADS 512
+#endif
 template <typename C> constexpr int col_tw() {
-    int tw = 16;
-    while (tw > 1 && (kT * tw > 1024 || (long)kN * (tw + 1) * (long)sizeof(C) > 80 * 1024)) tw /= 2;
+    int tw = MSX_COL_TW;
+    while (tw > 1 && (kTc * tw > MSX_COL_THREADS || (long)kN * (tw + 1) * (long)sizeof(C) > 80 * 1024)) tw /= 2;
     return tw;
 }
 
@@ -35,33 +45,58 @@ template <typename K> hipError_t set_smem(K kernel, size_t bytes) {
     return hipSuccess;
 }
 
-template <typename C, int IN, int OUT> hipError_t row_go(const RowArgs<C>& a, hipStream_t s) {
+// waves per work-group: enough work-groups to fill 256 CUs several times over,
+// up to 8 waves per work-group to amortise the per-thread constants
+inline int pick_wpb(int requested, long long units, long long target_blocks) {
+    if (requested > 0) return requested;
+    long long w = units / target_blocks;
+    return (int)(w < 1 ? 1 : (w > 8 ? 8 : w));
+}
+
+template <typename C, int IN, int OUT, bool VREAL = false> hipError_t row_go(const RowArgs<C>& a0, hipStream_t s) {
     constexpr int LPW = row_lpw();
-    const long long nlines = (long long)a.nx * a.nb;
-    const unsigned grid = (unsigned)((nlines + LPW - 1) / LPW);
+    RowArgs<C> a = a0;
+    const int nxg = (a.nx + LPW - 1) / LPW;
+    a.wpb = pick_wpb(a.wpb, (long long)nxg * a.nb, 4096);
+    const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
+    const unsigned grid = (unsigned)(nxg * nchunk);
     const size_t smem = (size_t)LPW * (kN + kN / 16) * sizeof(C);
-    auto k = row_sweep<C, kN, IN, OUT, LPW>;
+    auto k = row_sweep<C, kN, IN, OUT, LPW, VREAL>;
     hipError_t e = set_smem(k, smem);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kT * LPW), smem, s, a);
     return hipGetLastError();
 }
 
-template <typename C, int IN, int OUT> hipError_t col_go(const ColArgs<C>& a, hipStream_t s) {
+template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<C>& a0, hipStream_t s) {
     constexpr int TW = col_tw<C>();
+    ColArgs<C> a = a0;
     const int groups = (a.ny + TW - 1) / TW;
-    const unsigned grid = (unsigned)groups * (unsigned)a.nb;
-    size_t smem = (size_t)kN * (TW + 1) * sizeof(C);
+    a.wpb = pick_wpb(a.wpb, (long long)groups * a.nb, 2048);
+    const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
+    const unsigned grid = (unsigned)(groups * nchunk);
+    constexpr size_t tile = (size_t)kN * (TW + 1) * sizeof(C);
+    // double-buffer the exchanges when two tiles fit in the work-group LDS
+    // limit (a launch asking for the full 160 KiB is refused)
+    constexpr bool DB = MSX_COL_DB && 2 * tile <= 150 * 1024;
+    size_t smem = DB ? 2 * tile : tile;
     if (smem < 2048) smem = 2048;  // detector reduction scratch
-    auto k = col_sweep<C, kN, IN, OUT, TW>;
+    auto k = col_sweep<C, kN, IN, OUT, TW, OPS, DB>;
     hipError_t e = set_smem(k, smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kT * TW), smem, s, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kTc * TW), smem, s, a);
     return hipGetLastError();
 }
 }  // namespace
 
 template <typename C> hipError_t MSX_CAT(launch_row_, MSX_N)(int mode, const RowArgs<C>& a, hipStream_t s) {
+    if (a.vreal) {
+        switch (mode) {
+            case kRowFwd: return row_go<C, 0, 1, true>(a, s);
+            case kRowNone: return row_go<C, 0, 0, true>(a, s);
+            default: return hipErrorInvalidValue;
+        }
+    }
     switch (mode) {
         case kRowFwd: return row_go<C, 0, 1>(a, s);
         case kRowSlice: return row_go<C, -1, 1>(a, s);
@@ -73,14 +108,30 @@ template <typename C> hipError_t MSX_CAT(launch_row_, MSX_N)(int mode, const Row
 }
 
 template <typename C> hipError_t MSX_CAT(launch_col_, MSX_N)(int mode, const ColArgs<C>& a, hipStream_t s) {
+    const int ops = a.ops & ~kColStore;
+    constexpr int kPD = kColProp | kColDetect;
     switch (mode) {
-        case kColSlice: return col_go<C, 1, -1>(a, s);
-        case kColGenInv: return col_go<C, 0, -1>(a, s);
-        case kColFwd: return col_go<C, 1, 0>(a, s);
-        case kColInv: return col_go<C, -1, 0>(a, s);
-        case kColPoint: return col_go<C, 0, 0>(a, s);
-        default: return hipErrorInvalidValue;
+        case kColSlice:
+            if (ops == kColProp) return col_go<C, 1, -1, kColProp>(a, s);
+            if (ops == kPD) return col_go<C, 1, -1, kPD>(a, s);
+            if (ops == kColMask) return col_go<C, 1, -1, kColMask>(a, s);
+            if (ops == kColCtf) return col_go<C, 1, -1, kColCtf>(a, s);
+            if (ops == 0) return col_go<C, 1, -1, 0>(a, s);
+            break;
+        case kColFwd:
+            if (ops == kPD) return col_go<C, 1, 0, kPD>(a, s);
+            if (ops == 0) return col_go<C, 1, 0, 0>(a, s);
+            break;
+        case kColGenInv:
+            if (ops == kColGen) return col_go<C, 0, -1, kColGen>(a, s);
+            break;
+        case kColInv:
+            if (ops == 0) return col_go<C, -1, 0, 0>(a, s);
+            break;
+        default:
+            break;
     }
+    return hipErrorInvalidValue;
 }
 
 template <typename C> int MSX_CAT(col_groups_, MSX_N)(int ny) {

@@ -23,6 +23,21 @@
 #pragma once
 #include "msx_fft.hpp"
 
+// Build-time shape knobs (defaults are the tuned values; the build can
+// override them to produce variants for the tuner):
+//   MSX_ROW_PREFETCH  prefetch the next wave's row during the transforms
+//   MSX_COL_PREFETCH  same for the column tile
+//   MSX_COL_CONST     keep per-element k^2 / propagator factors in registers
+#ifndef MSX_ROW_PREFETCH
+#define MSX_ROW_PREFETCH 1
+#endif
+#ifndef MSX_COL_PREFETCH
+#define MSX_COL_PREFETCH 1
+#endif
+#ifndef MSX_COL_CONST
+#define MSX_COL_CONST 1
+#endif
+
 namespace msx {
 
 constexpr int kMaxDet = 8;
@@ -42,62 +57,127 @@ template <typename C> struct RowArgs {
     Re scale;               // multiplies the data after the IN transform
     Re* inten;              // optional |psi|^2 output [nb][nx][N] (after the pointwise stage)
     int nx, nb;
+    int wpb;                // waves handled by one work-group (loop with prefetch)
+    // band pruning along y: element iy is "out of band" when ky2[iy] >= k2max.
+    // prune & 1: out-of-band inputs are known to be exactly zero (not loaded);
+    // prune & 2: out-of-band outputs are never read (not stored).
+    const float* ky2;
+    float k2max;
+    int prune;
 };
 
 // IN/OUT: 0 none, +1 forward (exp +i), -1 inverse (exp -i, unscaled)
-template <typename C, int N, int IN, int OUT, int LPW>
+// Work-group = LPW adjacent rows ix of `wpb` consecutive waves: the
+// transmission row and the twiddles are loaded once into registers and the
+// next wave's row is prefetched while the current one is transformed.
+// VREAL: the input is a real potential turned into exp(i*vscale*V) (the
+// transmission build); a compile-time switch so the slice kernels carry none
+// of its sincos code.
+template <typename C, int N, int IN, int OUT, int LPW, bool VREAL>
 __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
     using Re = typename real_of<C>::type;
-    constexpr int T = Plan<N>::T, E = N / T;
+    using PL = Plan<N>;
+    constexpr int T = PL::T, E = N / T;
+    constexpr bool WAVE = (64 % T) == 0;  // a line never straddles two wavefronts
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
     const int lw = threadIdx.x / T, t = threadIdx.x % T;
-    const long long nlines = (long long)p.nx * p.nb;
-    long long L = (long long)blockIdx.x * LPW + lw;
-    const bool live = L < nlines;
-    if (!live) L = nlines - 1;  // keep every thread in the barriers, store nothing
-    // line order: waves fastest so consecutive work-groups reuse a t row
-    const int ix = (int)(L / p.nb), b = (int)(L % p.nb);
-    C* row = p.psi + ((long long)b * p.nx + ix) * N;
-    LdsView<C> lds{smem + lw * (N + N / 16), 1, true};
+    const int nxg = (p.nx + LPW - 1) / LPW;
+    const int gx = blockIdx.x % nxg, chunk = blockIdx.x / nxg;
+    const int ix_raw = gx * LPW + lw;
+    const bool live = ix_raw < p.nx;
+    const int ix = live ? ix_raw : p.nx - 1;  // dead rows compute, never store
+    const int b0 = chunk * p.wpb, b1 = min(p.nb, b0 + p.wpb);
+    const LdsView<C> lds[2] = {{smem + lw * (N + N / 16), 1, true}, {smem + lw * (N + N / 16), 1, true}};
 
-    C a[E];
-    if (p.vreal) {
-        const Re* v = p.vreal + ((long long)b * p.nx + ix) * N;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            Re ph = p.vscale * v[t + e * T];
-            Re s, c;
-            if constexpr (sizeof(Re) == 4) sincosf(ph, &s, &c); else sincos(ph, &s, &c);
-            a[e] = mk<C>(c, s);
-        }
-    } else {
-        const C* in = p.src ? p.src + (long long)b * p.src_bstride + (long long)ix * N : row;
-#pragma unroll
-        for (int e = 0; e < E; ++e) a[e] = in[t + e * T];
-    }
-    if constexpr (IN != 0) fft_line<N, IN>(a, t, lds, p.tw);
-    if (p.scale != (Re)1) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) a[e] = cscale(a[e], p.scale);
-    }
+    TwReg<PL, C> tw;
+    if constexpr (IN != 0 || OUT != 0) load_twiddles<PL>(tw, t, p.tw);
+    C tr[E];
     if (p.trans) {
-        const C* tr = p.trans + (long long)ix * N;
+        const C* trow = p.trans + (long long)ix * N;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            // cfpix::operator*= (cfpix.cpp:296-318): (wr*tr - wi*ti, wr*ti + wi*tr)
-            a[e] = cmul(a[e], tr[t + e * T]);
+        for (int e = 0; e < E; ++e) tr[e] = trow[t + e * T];
+    }
+
+    unsigned inb = ~0u;  // bit e: element t + e*T lies inside the y band
+    if (p.prune) {
+        inb = 0u;
+#pragma unroll
+        for (int e = 0; e < E; ++e) inb |= (p.ky2[t + e * T] < p.k2max ? 1u : 0u) << e;
+    }
+    const unsigned ld_mask = (p.prune & 1) ? inb : ~0u;
+    const unsigned st_mask = (p.prune & 2) ? inb : ~0u;
+
+    auto load = [&](int b, C (&v)[E]) {
+        if constexpr (VREAL) {
+            const Re* vr = p.vreal + ((long long)b * p.nx + ix) * N;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                Re s, c;
+                if constexpr (sizeof(Re) == 4) sincosf(p.vscale * vr[t + e * T], &s, &c);
+                else sincos(p.vscale * vr[t + e * T], &s, &c);
+                v[e] = mk<C>(c, s);
+            }
+        } else {
+            const C* in = p.src ? p.src + (long long)b * p.src_bstride + (long long)ix * N
+                                : p.psi + ((long long)b * p.nx + ix) * N;
+            // masked elements read element t (in band, fetched anyway) and are
+            // zeroed after the load: unconditional loads keep the prefetch
+            // free of branch-induced waits
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = in[((ld_mask >> e) & 1u) ? t + e * T : t];
         }
-    }
-    if (p.inten && live) {
-        Re* o = p.inten + ((long long)b * p.nx + ix) * N;
+    };
+
+    // Out-of-band elements are loaded from a redirected in-band address and
+    // zeroed only when the registers are consumed: zeroing right after the
+    // load would make the prefetch wait for its data immediately.
+    auto mask_in = [&](C (&v)[E]) {
+        if (ld_mask != ~0u) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) o[t + e * T] = a[e].x * a[e].x + a[e].y * a[e].y;
+            for (int e = 0; e < E; ++e)
+                if (!((ld_mask >> e) & 1u)) v[e] = mk<C>((Re)0, (Re)0);
+        }
+    };
+    C a[E], nxt[E];
+    if (MSX_ROW_PREFETCH && b0 < b1) {
+        load(b0, a);
+        mask_in(a);
     }
-    if constexpr (OUT != 0) fft_line<N, OUT>(a, t, lds, p.tw);
-    if (live) {
+    for (int b = b0; b < b1; ++b) {
+        if (MSX_ROW_PREFETCH) {
+            if (b + 1 < b1) load(b + 1, nxt);
+        } else {
+            load(b, a);
+            mask_in(a);
+        }
+        if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
+        if (p.scale != (Re)1) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) row[t + e * T] = a[e];
+            for (int e = 0; e < E; ++e) a[e] = cscale(a[e], p.scale);
+        }
+        if (p.trans) {
+            // cfpix::operator*= (cfpix.cpp:296-318): (wr*tr - wi*ti, wr*ti + wi*tr)
+#pragma unroll
+            for (int e = 0; e < E; ++e) a[e] = cmul(a[e], tr[e]);
+        }
+        if (p.inten && live) {
+            Re* o = p.inten + ((long long)b * p.nx + ix) * N;
+#pragma unroll
+            for (int e = 0; e < E; ++e) o[t + e * T] = a[e].x * a[e].x + a[e].y * a[e].y;
+        }
+        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(a, t, lds, tw);
+        if (live) {
+            C* row = p.psi + ((long long)b * p.nx + ix) * N;
+#pragma unroll
+            for (int e = 0; e < E; ++e)
+                if ((st_mask >> e) & 1u) row[t + e * T] = a[e];
+        }
+        if (MSX_ROW_PREFETCH) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) a[e] = nxt[e];
+            mask_in(a);
+        }
     }
 }
 
@@ -136,6 +216,10 @@ template <typename C> struct ColArgs {
     float kap_lo, kap_hi;  // aperture: kap_lo <= kx2+ky2 <= kap_hi
     Re amp;                // probe amplitude
     int ny, nb, ops;
+    int wpb;               // waves handled by one work-group (loop with prefetch)
+    // prune != 0: columns with ky2[iy] >= k2max are skipped entirely (their
+    // k-space values are all masked to zero, so nothing downstream reads them)
+    int prune;
 };
 
 __device__ __forceinline__ int freq_index(int i, int n) {
@@ -155,116 +239,206 @@ template <typename C> __device__ __forceinline__ C expi_reduced(double ph) {
     return mk<C>(c, s);
 }
 
-template <typename C, int N, int IN, int OUT, int TW>
-__global__ __launch_bounds__(Plan<N>::T* TW) void col_sweep(ColArgs<C> p) {
+// Work-group = TW adjacent columns iy of `wpb` consecutive waves; per-column
+// and per-thread constants (k^2, propagator factors, twiddles) are loaded
+// once, the next wave's tile is prefetched while the current one is
+// transformed.
+// OPS: compile-time subset of kColProp | kColMask | kColDetect | kColGen |
+// kColCtf (only kColStore is read from p.ops at run time), so the per-slice
+// kernel carries no registers for the probe, detector or lens paths.
+// DB: double-buffered LDS exchanges (one barrier per exchange); needs twice
+// the LDS tile.
+#ifndef MSX_COL_MINW
+#define MSX_COL_MINW 1
+#endif
+#ifndef MSX_XCD_REMAP
+#define MSX_XCD_REMAP 1
+#endif
+// Bijection hardware block id -> logical id such that the blocks one XCD
+// receives (ids congruent mod 8) cover a contiguous logical range.
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+    constexpr int kXcd = 8;
+    const int per = nblk / kXcd, rem = nblk % kXcd;
+    const int xcd = bid % kXcd, idx = bid / kXcd;
+    return xcd * per + min(xcd, rem) + idx;
+}
+template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB>
+__global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(ColArgs<C> p) {
     using Re = typename real_of<C>::type;
-    constexpr int T = Plan<N>::T, E = N / T;
+    using PL = ColPlan<N>;
+    constexpr int T = PL::T, E = N / T;
+    constexpr bool WAVE = false;  // a column is spread over several wavefronts
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
     const int lw = threadIdx.x % TW, t = threadIdx.x / TW;
     const int groups = (p.ny + TW - 1) / TW;
-    const int b = blockIdx.x / groups, g = blockIdx.x % groups;
-    int iy = g * TW + lw;
-    const bool live = iy < p.ny;
-    if (!live) iy = p.ny - 1;
-    C* col = p.psi + (long long)b * N * p.ny + iy;
-    LdsView<C> lds{smem + lw, TW + 1, false};
+    // XCD-aware order: work-groups are dispatched round-robin over the 8 XCDs;
+    // renumber them so that consecutive logical groups (neighbouring columns,
+    // which share L2 lines when TW*sizeof(C) < 128 B) run on the same XCD.
+    const int bid = MSX_XCD_REMAP ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int g = bid % groups, chunk = bid / groups;
+    const int b0 = chunk * p.wpb, b1 = min(p.nb, b0 + p.wpb);
+    const int iy_raw = g * TW + lw;
+    const int iy = iy_raw < p.ny ? iy_raw : p.ny - 1;
+    const bool live = iy_raw < p.ny && (!p.prune || p.ky2[iy] < p.k2max);
+    // dead lanes of a partly live group load from the group's first live
+    // column (same cache lines as a live lane); their column is transformed
+    // on its own, never stored and left out of the detector sums, so the
+    // values need no zeroing
+    int first_live = -1;
+    for (int c = TW - 1; c >= 0; --c) {
+        const int yc = g * TW + c;
+        if (yc < p.ny && (!p.prune || p.ky2[yc] < p.k2max)) first_live = c;
+    }
+    const int ld_col = live ? lw : (first_live >= 0 ? first_live : 0);
+    if (!__syncthreads_or(live)) {
+        // whole group outside the band: its detector partials are zero
+        if constexpr ((OPS & kColDetect) != 0) {
+            if ((int)threadIdx.x < p.ndet)
+                for (int b = b0; b < b1; ++b) p.partial[((long long)threadIdx.x * p.nb + b) * groups + g] = 0.0;
+        }
+        return;
+    }
+    const LdsView<C> lds[2] = {{smem + lw, TW + 1, false}, {smem + N * (TW + 1) + lw, TW + 1, false}};
+    constexpr bool gen = (OPS & kColGen) != 0;
+    constexpr bool point = (OPS & (kColProp | kColMask | kColDetect | kColCtf)) != 0;
 
-    C a[E];
-    if (p.ops & kColGen) {
-        const double2 pos = p.pos[b];
-        const double ky = (double)freq_index(iy, p.ny) / p.by;
-        const float ky2 = p.ky2[iy];
+    TwReg<PL, C> tw;
+    if constexpr (IN != 0 || OUT != 0) load_twiddles<PL>(tw, t, p.tw);
+    // per-element constants along x (registers), per-column constants along y
+    float kx2c[MSX_COL_CONST ? E : 1];
+    C pxc[MSX_COL_CONST ? E : 1];
+    float ky2 = 0.f;
+    C py = mk<C>((Re)1, (Re)0);
+    if constexpr (point || gen) ky2 = p.ky2[iy];
+    if constexpr ((OPS & kColProp) != 0) py = p.py[iy];
+    if constexpr (MSX_COL_CONST) {
+        if constexpr (point || gen) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int ix = t + e * T;
-            const float k2f = p.kx2[ix] + ky2;
-            if (k2f <= p.kap_hi && k2f >= p.kap_lo) {
-                const double kx = (double)freq_index(ix, N) / p.ax;
-                const double k2 = kx * kx + ky * ky;
-                const double ph = -p.chi1 * k2 * (p.chi2 * k2 - p.df) +
-                                  6.283185307179586476925287 * (kx * pos.x + ky * pos.y);
-                a[e] = cscale(expi_reduced<C>(ph), p.amp);
-            } else {
-                a[e] = mk<C>((Re)0, (Re)0);
+            for (int e = 0; e < E; ++e) kx2c[e] = p.kx2[t + e * T];
+        }
+        if constexpr ((OPS & kColProp) != 0) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) pxc[e] = p.px[t + e * T];
+        }
+    }
+    auto KX2 = [&](int e) -> float {
+        if constexpr (MSX_COL_CONST) return kx2c[e]; else return p.kx2[t + e * T];
+    };
+    auto PX = [&](int e) -> C {
+        if constexpr (MSX_COL_CONST) return pxc[e]; else return p.px[t + e * T];
+    };
+    const double ky = (gen || (OPS & kColCtf)) ? (double)freq_index(iy, p.ny) / p.by : 0.0;
+
+    // strided accesses: uniform per-(work-group, wave) base + 32-bit lane
+    // offset (one image is far below 4 GiB)
+    const unsigned off0 = (unsigned)t * (unsigned)p.ny + (unsigned)ld_col;
+    const unsigned ostep = (unsigned)T * (unsigned)p.ny;
+    auto load = [&](int b, C (&v)[E]) {
+        const C* col = p.psi + (long long)b * N * p.ny + g * TW;
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = col[off0 + e * ostep];
+    };
+
+    C a[E], nxt[MSX_COL_PREFETCH ? E : 1];
+    if (MSX_COL_PREFETCH && !gen && b0 < b1) load(b0, a);
+    for (int b = b0; b < b1; ++b) {
+        if constexpr (MSX_COL_PREFETCH) {
+            if (!gen && b + 1 < b1) load(b + 1, nxt);
+        } else {
+            if (!gen) load(b, a);
+        }
+        if constexpr (gen) {
+            const double2 pos = p.pos[b];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int ix = t + e * T;
+                const float k2f = KX2(e) + ky2;
+                if (k2f <= p.kap_hi && k2f >= p.kap_lo) {
+                    const double kx = (double)freq_index(ix, N) / p.ax;
+                    const double k2 = kx * kx + ky * ky;
+                    const double ph = -p.chi1 * k2 * (p.chi2 * k2 - p.df) +
+                                      6.283185307179586476925287 * (kx * pos.x + ky * pos.y);
+                    a[e] = cscale(expi_reduced<C>(ph), p.amp);
+                } else {
+                    a[e] = mk<C>((Re)0, (Re)0);
+                }
             }
         }
-    } else {
-#pragma unroll
-        for (int e = 0; e < E; ++e) a[e] = col[(long long)(t + e * T) * p.ny];
-    }
-    if constexpr (IN != 0) fft_line<N, IN>(a, t, lds, p.tw);
+        if constexpr (IN != 0) fft_line<PL, IN, WAVE, DB, 0>(a, t, lds, tw);
 
-    if (p.ops & (kColProp | kColMask | kColDetect | kColCtf)) {
-        const float ky2 = p.ky2[iy];
-        C py = mk<C>((Re)1, (Re)0);
-        if (p.ops & kColProp) py = p.py[iy];
-        float acc[kMaxDet], cmp[kMaxDet];
+        if constexpr (point) {
+            Re acc[kMaxDet], cmp[kMaxDet];
 #pragma unroll
-        for (int d = 0; d < kMaxDet; ++d) acc[d] = cmp[d] = 0.f;
-        const double ky = (double)freq_index(iy, p.ny) / p.by;
+            for (int d = 0; d < kMaxDet; ++d) acc[d] = cmp[d] = (Re)0;
 #pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const int ix = t + e * T;
-            const float k2 = p.kx2[ix] + ky2;
-            if (p.ops & (kColProp | kColMask)) {
-                if (k2 < p.k2max) {
-                    // slicelib.cpp:765-770: (w * py) * px
-                    if (p.ops & kColProp) a[e] = cmul(cmul(a[e], py), p.px[ix]);
-                } else {
-                    a[e] = mk<C>((Re)0, (Re)0);
+            for (int e = 0; e < E; ++e) {
+                const float k2 = KX2(e) + ky2;
+                if constexpr ((OPS & (kColProp | kColMask)) != 0) {
+                    if (k2 < p.k2max) {
+                        // slicelib.cpp:765-770: (w * py) * px
+                        if constexpr ((OPS & kColProp) != 0) a[e] = cmul(cmul(a[e], py), PX(e));
+                    } else {
+                        a[e] = mk<C>((Re)0, (Re)0);
+                    }
                 }
-            }
-            if (p.ops & kColCtf) {
-                if (k2 <= p.kap_hi) {
-                    const double kx = (double)freq_index(ix, N) / p.ax;
-                    const double kk = kx * kx + ky * ky;
-                    a[e] = cmul(a[e], expi_reduced<C>(-p.chi1 * kk * (p.chi2 * kk - p.df)));
-                } else {
-                    a[e] = mk<C>((Re)0, (Re)0);
+                if constexpr ((OPS & kColCtf) != 0) {
+                    if (k2 <= p.kap_hi) {
+                        const double kx = (double)freq_index(t + e * T, N) / p.ax;
+                        const double kk = kx * kx + ky * ky;
+                        a[e] = cmul(a[e], expi_reduced<C>(-p.chi1 * kk * (p.chi2 * kk - p.df)));
+                    } else {
+                        a[e] = mk<C>((Re)0, (Re)0);
+                    }
                 }
-            }
-            if (p.ops & kColDetect) {
-                const float v = (float)(a[e].x * a[e].x + a[e].y * a[e].y);
+                if constexpr ((OPS & kColDetect) != 0) {
+                    const Re v = a[e].x * a[e].x + a[e].y * a[e].y;
 #pragma unroll
-                for (int d = 0; d < kMaxDet; ++d) {
-                    if (d < p.ndet && k2 >= p.dlo[d] && k2 <= p.dhi[d]) {
-                        // Kahan-compensated per-thread accumulation
-                        const float y = v - cmp[d];
-                        const float s = acc[d] + y;
-                        cmp[d] = (s - acc[d]) - y;
-                        acc[d] = s;
+                    for (int d = 0; d < kMaxDet; ++d) {
+                        if (d < p.ndet && k2 >= p.dlo[d] && k2 <= p.dhi[d]) {
+                            // Kahan-compensated per-thread accumulation
+                            const Re y = v - cmp[d];
+                            const Re s = acc[d] + y;
+                            cmp[d] = (s - acc[d]) - y;
+                            acc[d] = s;
+                        }
                     }
                 }
             }
-        }
-        if (p.ops & kColDetect) {
-            // deterministic work-group reduction in FP64: wavefront butterfly,
-            // then the per-wave results in fixed order through LDS.
-            __syncthreads();  // LDS may still be read by the last FFT pass
-            double* red = reinterpret_cast<double*>(smem_raw);
-            const int nwaves = (T * TW + 63) / 64;
-            const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-            for (int d = 0; d < p.ndet; ++d) {
-                double v = live ? (double)acc[d] - (double)cmp[d] : 0.0;
+            if constexpr ((OPS & kColDetect) != 0) {
+                // deterministic work-group reduction in FP64: wavefront
+                // butterfly, then the per-wave results in fixed order via LDS.
+                lds_barrier();  // LDS may still be read by the last FFT pass
+                double* red = reinterpret_cast<double*>(smem_raw);
+                const int nwaves = (T * TW + 63) / 64;
+                const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+                for (int d = 0; d < p.ndet; ++d) {
+                    double v = live ? (double)acc[d] - (double)cmp[d] : 0.0;
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                if (lane == 0) red[d * 32 + wv] = v;
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                    if (lane == 0) red[d * 32 + wv] = v;
+                }
+                lds_barrier();
+                if (threadIdx.x < p.ndet) {
+                    const int d = threadIdx.x;
+                    double s = 0.0;
+                    for (int w = 0; w < nwaves; ++w) s += red[d * 32 + w];
+                    p.partial[((long long)d * p.nb + b) * groups + g] = s;
+                }
+                lds_barrier();
             }
-            __syncthreads();
-            if (threadIdx.x < p.ndet) {
-                const int d = threadIdx.x;
-                double s = 0.0;
-                for (int w = 0; w < nwaves; ++w) s += red[d * 32 + w];
-                p.partial[((long long)d * p.nb + b) * groups + g] = s;
-            }
-            __syncthreads();
         }
-    }
-    if constexpr (OUT != 0) fft_line<N, OUT>(a, t, lds, p.tw);
-    if ((p.ops & kColStore) && live) {
+        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE, DB, PL::P - 1>(a, t, lds, tw);
+        if ((p.ops & kColStore) && live) {
+            C* col = p.psi + (long long)b * N * p.ny + g * TW + lw;
+            const unsigned so = (unsigned)t * (unsigned)p.ny;
 #pragma unroll
-        for (int e = 0; e < E; ++e) col[(long long)(t + e * T) * p.ny] = a[e];
+            for (int e = 0; e < E; ++e) col[so + e * ostep] = a[e];
+        }
+        if constexpr (MSX_COL_PREFETCH) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) a[e] = nxt[e];
+        }
     }
 }
 

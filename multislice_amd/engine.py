@@ -24,7 +24,8 @@ import threading
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmsx.so")
+# MSX_LIB selects an alternative build of the same library (tuning variants)
+LIB_PATH = os.environ.get("MSX_LIB") or os.path.join(_HERE, "libmsx.so")
 
 F32, F64 = 0, 1
 HOST, DEVICE = 0, 1
@@ -65,7 +66,10 @@ class Detector(C.Structure):
     _fields_ = [("k_inner", C.c_double), ("k_outer", C.c_double)]
 
 
+PROF_ROW, PROF_COL, PROF_SPLAT, PROF_DET = 0, 1, 2, 3
+
 EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_last_error", "msx_version",
+           "msx_profile", "msx_profile_read",
            "msx_wavelength", "msx_sigma", "msx_vzatom", "msx_featom", "msx_fft2d", "msx_potential",
            "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image"]
 
@@ -106,6 +110,8 @@ def load():
     sig = {
         "msx_create": (i, [i, C.POINTER(vp)]), "msx_destroy": (i, [vp]), "msx_set_stream": (i, [vp, vp]),
         "msx_last_error": (C.c_char_p, []), "msx_version": (i, []),
+        "msx_profile": (i, [vp, i]),
+        "msx_profile_read": (i, [vp, i, C.POINTER(C.c_int64), C.POINTER(d), C.POINTER(d)]),
         "msx_wavelength": (d, [d]), "msx_sigma": (d, [d]), "msx_vzatom": (d, [i, d, i, i]),
         "msx_featom": (d, [i, d, i]),
         "msx_fft2d": (i, [vp, vp, i, i, i, i, i, i]),
@@ -192,6 +198,16 @@ class Context:
         """Queue work on an external stream (torch.cuda.Stream or raw handle)."""
         raw = getattr(stream, "cuda_stream", stream)
         _check(load().msx_set_stream(self.h, C.c_void_p(raw) if raw else None))
+
+    def profile(self, enable=True):
+        """Turn per-kernel event timing on/off (resets the totals)."""
+        _check(load().msx_profile(self.h, int(bool(enable))))
+
+    def profile_read(self, kind):
+        """(launches, total_ms, total_algorithmic_bytes) for a kernel kind."""
+        n, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        _check(load().msx_profile_read(self.h, int(kind), C.byref(n), C.byref(ms), C.byref(by)))
+        return n.value, ms.value, by.value
 
     # -------------------------------------------------------------- calls
     def fft2d(self, data, sign):
