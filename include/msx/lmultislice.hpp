@@ -1,0 +1,359 @@
+// msx/lmultislice.hpp -- header-only C++ mirror of the reference's LMultiSlice
+// call surface over the msx C ABI (include/msx.h):
+//
+//   Microscope        ImageSimulation/LMultiSlice/Microscope.h:3-34
+//   ModelPotential    ImageSimulation/LMultiSlice/ModelPotential.h:27-51
+//   ModelSimulated    ImageSimulation/LMultiSlice/ModelSimulated.h:7-28
+//   CPotentialBuilder ImageSimulation/PotentialBuilder/PotentialBuilder.h:14-22
+//   AModel::Model-compatible atom container (amodel/include/Model.h:20-174,
+//   BaseStructures.h:50-135) and an ImageSpace::Image-compatible result
+//   (ImageSimulation/Image/Image.h:29-88) for callers without those DLLs.
+//
+// The classes live in namespace msx::lms.  Defining MSX_LMS_GLOBAL_NAMES before
+// including this header also exports ModelPotential, ModelSimulated,
+// Microscope and CPotentialBuilder into the global namespace, as the reference
+// declares them.  Member names, argument meanings and the 0 / -1 return
+// convention follow the reference; all computation goes to libmsx.so on the
+// GPU (one process-wide context on device 0 unless set_context() is called).
+//
+// Physics differences from LMultiSlice, by design (its k^4 propagator,
+// per-slice fftshift and 1/nx normalisation are not reproduced): the slice
+// loop is Kirkland's (transmission with the 2/3 band limit, Fresnel propagator
+// exp(-i pi lambda dz k^2), forward FFT unscaled / inverse scaled by
+// 1/(nx ny)), and the lens is the Kirkland CTF exp(-i chi) with a hard
+// objective aperture.
+#ifndef MSX_LMULTISLICE_HPP
+#define MSX_LMULTISLICE_HPP
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../msx.h"
+
+namespace msx {
+namespace lms {
+
+// ----------------------------------------------------------------- context
+inline msx_ctx& context_slot() {
+    static msx_ctx c = nullptr;
+    return c;
+}
+// Process-wide context (device 0) created on first use.
+inline msx_ctx context() {
+    msx_ctx& c = context_slot();
+    if (!c && msx_create(0, &c) != MSX_OK) c = nullptr;
+    return c;
+}
+inline void set_context(msx_ctx c) { context_slot() = c; }
+
+// ----------------------------------------------------------------- atoms
+namespace amodel {
+struct XYZ {
+    float x = 0, y = 0, z = 0;
+};
+struct Record {  // BaseStructures.h:120-128
+    char Atom[4] = {0, 0, 0, 0};
+    float oxygenation = 0;
+    XYZ xsCoordinate;  // fractional coordinates
+    float R_e = 0, occupation = 1, absorbtion = 0;
+};
+struct Cortege {  // BaseStructures.h:129-135
+    bool isInclude = true;
+    Record element;
+};
+
+// Z from the element symbol (the table of Model::getNumberByName, Model.h:62-81).
+inline int number_by_name(const char* name) {
+    static const char* table[] = {
+        "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si", "P",  "S",  "Cl", "Ar",
+        "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr",
+        "Rb", "Sr", "Y",  "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb", "Te", "I",  "Xe",
+        "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb", "Lu", "Hf",
+        "Ta", "W",  "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At", "Rn", "Fr", "Ra", "Ac", "Th",
+        "Pa", "U",  "Np", "Pu", "Am", "Cm", "Bk", "Cf", "Es", "Fm", "Md", "No", "Lr"};
+    for (int i = 0; i < 103; i++)
+        if (std::strcmp(name, table[i]) == 0) return i + 1;
+    return -1;
+}
+
+// Minimal container with the accessors ModelPotential uses on AModel::Model.
+class Model {
+public:
+    Model() = default;
+    Model(float a, float b, float c) : a_(a), b_(b), c_(c) {}
+    float getA() const { return a_; }
+    float getB() const { return b_; }
+    float getC() const { return c_; }
+    size_t getNumberAtoms() const { return cells_.size(); }
+    Cortege* getTableCell() { return cells_.data(); }
+    int getNumberByName(const char* name) const { return number_by_name(name); }
+    void setA(float v) { a_ = v; }
+    void setB(float v) { b_ = v; }
+    void setC(float v) { c_ = v; }
+    void add(const char* symbol, float fx, float fy, float fz, float occ = 1.f) {
+        Cortege c;
+        std::strncpy(c.element.Atom, symbol, 3);
+        c.element.xsCoordinate = XYZ{fx, fy, fz};
+        c.element.occupation = occ;
+        cells_.push_back(c);
+    }
+
+private:
+    float a_ = 0, b_ = 0, c_ = 0;
+    std::vector<Cortege> cells_;
+};
+}  // namespace amodel
+
+// ----------------------------------------------------------------- image
+// Interleaved image with the fields of ImageSpace::Image that callers read
+// (nChannels, depth in bytes per value, width, height, thickness, imageData).
+struct Image {
+    int nChannels = 1, depth = 8, width = 0, height = 0, thickness = 1;
+    std::vector<double> store;
+    char* imageData = nullptr;
+    Image() = default;
+    Image(size_t nx, size_t ny, size_t nz, size_t depth_, size_t nCh)
+        : nChannels((int)nCh), depth((int)depth_), width((int)nx), height((int)ny), thickness((int)nz),
+          store(nx * ny * nz * nCh, 0.0) {
+        imageData = reinterpret_cast<char*>(store.data());
+    }
+    double* data() { return store.data(); }
+    // |psi| of a 2-channel complex image (Image::getModule, Image.cu:138-163)
+    Image getModule() const {
+        Image m((size_t)width, (size_t)height, (size_t)thickness, 8, 1);
+        for (size_t i = 0; i < m.store.size(); i++)
+            m.store[i] = std::hypot(store[2 * i], nChannels > 1 ? store[2 * i + 1] : 0.0);
+        return m;
+    }
+};
+
+// MRC2000 writer (mode 2, float data); the reference writes _pt.mrc and .mrc
+// through ImageSpace::MRC (MRC.cpp:206-345).  Returns 0 / -1.
+inline int write_mrc(const char* path, const double* v, int nx, int ny, int nz, float xlen, float ylen, float zlen) {
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return -1;
+    unsigned char h[1024];
+    std::memset(h, 0, sizeof h);
+    auto put_i = [&](int word, int32_t x) { std::memcpy(h + 4 * word, &x, 4); };
+    auto put_f = [&](int word, float x) { std::memcpy(h + 4 * word, &x, 4); };
+    const size_t n = (size_t)nx * ny * nz;
+    double mn = n ? v[0] : 0, mx = mn, sum = 0;
+    for (size_t i = 0; i < n; i++) {
+        mn = std::min(mn, v[i]);
+        mx = std::max(mx, v[i]);
+        sum += v[i];
+    }
+    const double mean = n ? sum / n : 0;
+    double var = 0;
+    for (size_t i = 0; i < n; i++) var += (v[i] - mean) * (v[i] - mean);
+    put_i(0, nx); put_i(1, ny); put_i(2, nz); put_i(3, 2);
+    put_i(7, nx); put_i(8, ny); put_i(9, nz);
+    put_f(10, xlen); put_f(11, ylen); put_f(12, zlen);
+    put_f(13, 90.f); put_f(14, 90.f); put_f(15, 90.f);
+    put_i(16, 1); put_i(17, 2); put_i(18, 3);
+    put_f(19, (float)mn); put_f(20, (float)mx); put_f(21, (float)mean);
+    std::memcpy(h + 208, "MAP ", 4);
+    h[212] = 0x44; h[213] = 0x44;  // little-endian machine stamp
+    put_f(54, (float)std::sqrt(n ? var / n : 0));
+    bool ok = std::fwrite(h, 1, 1024, f) == 1024;
+    std::vector<float> buf(n);
+    for (size_t i = 0; i < n; i++) buf[i] = (float)v[i];
+    ok = ok && std::fwrite(buf.data(), sizeof(float), n, f) == n;
+    return (std::fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+// ----------------------------------------------------------------- Microscope
+// Microscope.cpp:8-86.  Units as there: keV, Cs [mm], aperture [mrad]
+// (objective aperture semi-angle), defocus [nm].
+class Microscope {
+public:
+    Microscope() = default;
+    Microscope(double keV, double Cs, double aperture, double defocus)
+        : keV_(keV), Cs_(Cs), aperture_(aperture), defocus_(defocus) {}
+    double getKeV() const { return keV_; }
+    double getCs() const { return Cs_; }
+    double getAperture() const { return aperture_; }
+    double getDefocus() const { return defocus_; }
+    double getLambda() const { return msx_wavelength(keV_); }
+    double getSigma() const { return msx_sigma(keV_); }
+    void setKeV(double v) { keV_ = v; }
+    void setCs(double v) { Cs_ = v; }
+    void setAperture(double v) { aperture_ = v; }
+    void setDefocus(double v) { defocus_ = v; }
+    // Kirkland aberration phase chi(k) = pi lambda k^2 (0.5 Cs lambda^2 k^2 - df)
+    // for |k|^2 = k2 (1/A^2); replaces Microscope::alpha (Microscope.cpp:21-25).
+    double alpha(double k2) const {
+        const double l = getLambda(), cs = Cs_ * 1e7, df = defocus_ * 10.0;
+        return M_PI * l * k2 * (0.5 * cs * l * l * k2 - df);
+    }
+
+private:
+    double keV_ = 200, Cs_ = 0, aperture_ = 0, defocus_ = 0;
+};
+
+// ----------------------------------------------------------------- ModelPotential
+// ModelPotential.h:27-51.  `potential` is a host double[nz][ny][nx] (x
+// fastest, the reference's layout) filled by calculatePotentialGrid().
+// The default reproduces the LMultiSlice kernel's formula and table
+// (ModelPotential.cu:232-254); set_formula(MSX_FORMULA_KIRKLAND) selects
+// Kirkland's vzatom.
+template <class ModelT = amodel::Model> class ModelPotentialT {
+public:
+    ModelPotentialT(ModelT* model, size_t nx, size_t ny, size_t nz, double dpa, double radius)
+        : potential(nullptr), model_(model), nx_(nx), ny_(ny), nz_(nz), dpa_(dpa), radius_(radius) {
+        store_.assign(nx * ny * nz, 0.0);
+        potential = store_.data();
+    }
+    int calculatePotentialGrid() {
+        msx_ctx c = context();
+        if (!c || !model_) return -1;
+        const size_t n = model_->getNumberAtoms();
+        auto* cells = model_->getTableCell();
+        const double a = model_->getA(), b = model_->getB(), cc = model_->getC();
+        std::vector<msx_atom> atoms;
+        atoms.reserve(n);
+        for (size_t i = 0; i < n; i++) {
+            const auto& e = cells[i].element;
+            msx_atom at;
+            at.Z = model_->getNumberByName(e.Atom);
+            if (at.Z < 1) return -1;
+            at.occ = use_occupancy_ ? e.occupation : 1.0f;
+            at.x = e.xsCoordinate.x * a;
+            at.y = e.xsCoordinate.y * b;
+            at.z = (double)e.xsCoordinate.z * cc;
+            atoms.push_back(at);
+        }
+        std::vector<double> v(nx_ * ny_ * nz_);  // [kz][ix][iy] from the engine
+        if (msx_potential(c, atoms.data(), (int64_t)atoms.size(), a, b, cc, (int)nx_, (int)ny_, (int)nz_, radius_,
+                          table_, formula_, v.data(), MSX_F64, MSX_HOST) != MSX_OK)
+            return -1;
+        for (size_t kz = 0; kz < nz_; kz++)
+            for (size_t ix = 0; ix < nx_; ix++)
+                for (size_t iy = 0; iy < ny_; iy++)
+                    potential[(kz * ny_ + iy) * nx_ + ix] = v[(kz * nx_ + ix) * ny_ + iy];
+        return 0;
+    }
+    // <filename>_pt.mrc, as ModelPotential::savePotential (ModelPotential.cu:294-304)
+    int savePotential(const char* filename) {
+        std::string p = std::string(filename) + "_pt.mrc";
+        return write_mrc(p.c_str(), potential, (int)nx_, (int)ny_, (int)nz_, model_ ? model_->getA() : 0,
+                         model_ ? model_->getB() : 0, model_ ? model_->getC() : 0);
+    }
+    ModelT* getModel() { return model_; }
+    size_t getNx() const { return nx_; }
+    size_t getNy() const { return ny_; }
+    size_t getNz() const { return nz_; }
+    double getDpa() const { return dpa_; }
+    void set_formula(int f) { formula_ = f; }
+    void set_table(int t) { table_ = t; }
+    void set_use_occupancy(bool u) { use_occupancy_ = u; }
+
+    double* potential;
+
+private:
+    std::vector<double> store_;
+    ModelT* model_;
+    size_t nx_, ny_, nz_;
+    double dpa_, radius_;
+    int table_ = MSX_TABLE_LMULTISLICE, formula_ = MSX_FORMULA_LMULTISLICE;
+    bool use_occupancy_ = false;
+};
+using ModelPotential = ModelPotentialT<amodel::Model>;
+
+// ----------------------------------------------------------------- ModelSimulated
+// ModelSimulated.h:7-28.  imageCalculation(result, microscope): plane wave
+// through the nz slices of the potential (thickness c/nz each), then the
+// objective lens; result receives the image-plane wave as interleaved complex
+// doubles (nChannels 2), like Image::copyFFTtoImage (Image.h:62-73).
+template <class MP = ModelPotential> class ModelSimulatedT {
+public:
+    ModelSimulatedT(MP* mp, size_t nx, size_t ny, size_t nz, double dpa)
+        : mp_(mp), nx_(nx), ny_(ny), nz_(nz), dpa_(dpa) {}
+    template <class ImageT> int imageCalculation(ImageT* result, const Microscope* mic) {
+        msx_ctx c = context();
+        if (!c || !mp_ || !result || !mic) return -1;
+        auto* model = mp_->getModel();
+        const double a = model->getA(), b = model->getB(), cc = model->getC();
+        const size_t n = nx_ * ny_;
+        // potential [kz][iy][ix] -> engine layout [kz][ix][iy]
+        std::vector<float> V(n * nz_);
+        for (size_t kz = 0; kz < nz_; kz++)
+            for (size_t iy = 0; iy < ny_; iy++)
+                for (size_t ix = 0; ix < nx_; ix++)
+                    V[(kz * nx_ + ix) * ny_ + iy] = (float)mp_->potential[(kz * ny_ + iy) * nx_ + ix];
+        std::vector<float> t(2 * n * nz_);
+        const double kev = mic->getKeV();
+        if (msx_transmission(c, V.data(), (int)nz_, (int)nx_, (int)ny_, a, b, kev, 2.0 / 3.0, t.data(), MSX_F32,
+                             MSX_HOST) != MSX_OK)
+            return -1;
+        std::vector<double> cz(nz_, cc / nz_);
+        std::vector<int32_t> layer(nz_);
+        for (size_t k = 0; k < nz_; k++) layer[k] = (int32_t)k;
+        msx_stack st{};
+        st.nx = (int32_t)nx_;
+        st.ny = (int32_t)ny_;
+        st.ax = a;
+        st.by = b;
+        st.kev = kev;
+        st.bw = 2.0 / 3.0;
+        st.nlayer = (int32_t)nz_;
+        st.cz = cz.data();
+        st.nslice = (int32_t)nz_;
+        st.layer = layer.data();
+        std::vector<float> one(2 * n, 0.f), psi(2 * n);
+        for (size_t i = 0; i < n; i++) one[2 * i] = 1.f;
+        if (msx_multislice(c, &st, t.data(), one.data(), psi.data(), 1, MSX_F32, MSX_HOST) != MSX_OK) return -1;
+        exit_wave_.assign(psi.begin(), psi.end());
+        std::vector<float> img(2 * n);
+        if (msx_ctf_image(c, psi.data(), 1, (int)nx_, (int)ny_, a, b, kev, mic->getCs() * 1e7,
+                          mic->getDefocus() * 10.0, mic->getAperture() * 1e-3, img.data(), nullptr, MSX_F32,
+                          MSX_HOST) != MSX_OK)
+            return -1;
+        // engine (ix,iy) -> image row iy, column ix, interleaved (re, im)
+        double* out = reinterpret_cast<double*>(result->imageData);
+        const int nch = result->nChannels;
+        for (size_t iy = 0; iy < ny_; iy++)
+            for (size_t ix = 0; ix < nx_; ix++) {
+                out[nch * (iy * nx_ + ix) + 0] = img[2 * (ix * ny_ + iy)];
+                if (nch > 1) out[nch * (iy * nx_ + ix) + 1] = img[2 * (ix * ny_ + iy) + 1];
+            }
+        return 0;
+    }
+    // exit wave of the last imageCalculation (engine layout, complex float)
+    const std::vector<float>& exitWave() const { return exit_wave_; }
+
+private:
+    MP* mp_;
+    size_t nx_, ny_, nz_;
+    double dpa_;
+    std::vector<float> exit_wave_;
+};
+using ModelSimulated = ModelSimulatedT<ModelPotential>;
+
+// ----------------------------------------------------------------- PotentialBuilder
+// PotentialBuilder.h:14-22: the reference DLL is an empty skeleton whose only
+// function returns 42; kept for link compatibility.
+class CPotentialBuilder {
+public:
+    CPotentialBuilder() = default;
+    int fnPotentialBuilder() { return 42; }
+};
+inline int nPotentialBuilder = 0;
+
+}  // namespace lms
+}  // namespace msx
+
+#ifdef MSX_LMS_GLOBAL_NAMES
+using msx::lms::CPotentialBuilder;
+using msx::lms::Microscope;
+using msx::lms::ModelPotential;
+using msx::lms::ModelSimulated;
+#endif
+
+#endif  // MSX_LMULTISLICE_HPP

@@ -384,7 +384,12 @@ ORC_API int orc_potential(const int *Z, const double *xyz, const double *occ, lo
             if (z < kz * dz) kz--; else kz++;
             if (kz < 0 || kz >= nslices) continue;
         }
-        double xa = xyz[3 * i + 0], ya = xyz[3 * i + 1];
+        // periodic supercell: positions wrapped into [0,a) x [0,b) (the
+        // reference's fractional coordinates already lie there)
+        double xa = xyz[3 * i + 0] - a * std::floor(xyz[3 * i + 0] / a);
+        double ya = xyz[3 * i + 1] - b * std::floor(xyz[3 * i + 1] / b);
+        if (xa >= a) xa -= a;
+        if (ya >= b) ya -= b;
         double w = occ ? occ[i] : 1.0;
         double *Vs = V + (size_t)kz * nx * ny;
         for (int ix = 0; ix < nx; ix++) {

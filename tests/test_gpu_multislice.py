@@ -147,3 +147,18 @@ def test_vacuum_is_identity(ctx):
     ctx.multislice(t, [3.0], [0], psi, 20.0, 20.0, 200.0)
     ctx.multislice(t, [-3.0], [0], psi, 20.0, 20.0, 200.0)
     assert rel_l2(psi, w) < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_ctf_image_matches_oracle(ctx, si_layers, prec):
+    """Objective-lens image (mulslice.cpp:612-656 at one defocus; Kirkland chi)."""
+    at, cell, V = si_layers
+    cz, layer = _stack()
+    o64t = O.transmission(V, cell[0], cell[1], KEV)
+    exitw = O.multislice(o64t, cz, layer, np.ones((256, 256)), cell[0], cell[1], KEV)
+    ref_w, ref_i = O.ctf_image(exitw, cell[0], cell[1], KEV, cs=1.2e7, df=-600.0, aobj=0.02)
+    src = exitw[None] if prec == "f64" else exitw[None].astype(np.complex64)
+    inten, w = ctx.ctf_image(src, cell[0], cell[1], KEV, 1.2e7, -600.0, 0.02, want_wave=True)
+    tol = 1e-12 if prec == "f64" else 2e-6
+    assert rel_l2(w[0], ref_w) < tol
+    assert rel_l2(inten[0], ref_i) < 2 * tol
