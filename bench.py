@@ -58,7 +58,7 @@ def build_specimen(ctx, torch):
     trans = []
     t0 = time.perf_counter()
     for c in range(NCONF):
-        at = specimen.frozen_phonon(at0, specimen.SRTIO3_U, seed=1234, config=c)
+        at = msx.displace(at0, specimen.SRTIO3_U, seed=1234, config=c)  # Philox, per (config, atom)
         V = torch.empty((NSLICE, N, N), dtype=torch.float32, device="cuda")
         ctx.potential(at, cell, N, N, NSLICE, 3.0, table=msx.TABLE_LMULTISLICE,
                       formula=msx.FORMULA_KIRKLAND, prec=msx.F32, out=V)

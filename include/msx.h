@@ -102,6 +102,22 @@ typedef struct msx_atom {
     double x, y, z;
 } msx_atom;
 
+/* ---- frozen phonons ------------------------------------------------------ */
+/* Isotropic Gaussian displacement of every atom for one frozen-phonon
+ * configuration: x += u*gx, y += u*gy, z += u*gz with u = u_rms[Z] (Angstrom,
+ * per axis) and g ~ N(0,1) from Philox-4x32-10 (counter = atom index and
+ * configuration, key = seed) through Box-Muller in FP64 -- the same numbers on
+ * the host (where = MSX_HOST; ctx may be NULL) and on the GPU (MSX_DEVICE;
+ * in/out device arrays).  Replaces the reference's sequential generators
+ * (Kirkland ReadXYZcoord wobble, slicelib.cpp:2197-2320 with rangauss,
+ * :792-830; Musli Vobble, MusliRandom.cs:43-55), whose draws depend on the
+ * visiting order.  u_rms: host array of 104 doubles indexed by Z. */
+MSX_API int msx_displace(msx_ctx ctx, const msx_atom* in, int64_t natoms, const double* u_rms, uint64_t seed,
+                         int32_t config, msx_atom* out, int where);
+/* The Philox-4x32-10 block function itself (counter, key -> 4 words), exposed
+ * for known-answer tests of the generator. */
+MSX_API void msx_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
 /* ---- 2D FFT ----------------------------------------------------------- */
 /* cfpix::fft() / cfpix::ifft() (cfpix.cpp:115-145), batched: sign = +1 is
  * fft() (exponent +i, unscaled), sign = -1 is ifft() (exponent -i, scaled by

@@ -11,6 +11,7 @@
 
 #include "../../include/msx.h"
 #include "msx_internal.h"
+#include "msx_philox.h"
 #include "fparams_tables.inc"
 
 using namespace msx;
@@ -813,6 +814,46 @@ MSX_API double msx_featom(int Z, double k2, int table) {
     for (int i = 0; i < 6; i += 2) s += p[i] / (k2 + p[i + 1]);
     for (int i = 6; i < 12; i += 2) s += p[i] * std::exp(-k2 * p[i + 1]);
     return s;
+}
+
+MSX_API int msx_displace(msx_ctx ctx, const msx_atom* in, int64_t natoms, const double* u_rms, uint64_t seed,
+                         int32_t config, msx_atom* out, int where) {
+    static_assert(sizeof(msx_atom) == sizeof(AtomIO), "atom layout");
+    if (natoms < 0 || (natoms > 0 && (!in || !out)) || !u_rms) return fail(MSX_EINVAL, "bad atoms/u_rms");
+    if (where == MSX_HOST) {
+        const double twopi = 6.283185307179586476925287;
+        const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+        for (int64_t i = 0; i < natoms; i++) {
+            msx_atom a = in[i];
+            const double u = (a.Z >= 0 && a.Z < 104) ? u_rms[a.Z] : 0.0;
+            const uint32_t c0 = (uint32_t)i, c1 = (uint32_t)((uint64_t)i >> 32);
+            const Philox4 r0 = philox4x32_10(c0, c1, (uint32_t)config, 0u, k0, k1);
+            const Philox4 r1 = philox4x32_10(c0, c1, (uint32_t)config, 1u, k0, k1);
+            const double u1 = uniform53(r0.v[0], r0.v[1]), u2 = uniform53(r0.v[2], r0.v[3]);
+            const double u3 = uniform53(r1.v[0], r1.v[1]), u4 = uniform53(r1.v[2], r1.v[3]);
+            const double ra = std::sqrt(-2.0 * std::log(u1)), rb = std::sqrt(-2.0 * std::log(u3));
+            a.x += u * ra * std::cos(twopi * u2);
+            a.y += u * ra * std::sin(twopi * u2);
+            a.z += u * rb * std::cos(twopi * u4);
+            out[i] = a;
+        }
+        return MSX_OK;
+    }
+    if (where != MSX_DEVICE) return fail(MSX_EINVAL, "where must be MSX_HOST or MSX_DEVICE");
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    CHK(ctx->aux.ensure(104 * sizeof(double)));
+    HIPCHK(hipMemcpyAsync(ctx->aux.p, u_rms, 104 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(launch_displace(reinterpret_cast<const AtomIO*>(in), natoms, static_cast<const double*>(ctx->aux.p), seed,
+                           config, reinterpret_cast<AtomIO*>(out), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+// Philox-4x32-10 block for the known-answer tests
+MSX_API void msx_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    const Philox4 r = philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+    for (int i = 0; i < 4; i++) out[i] = r.v[i];
 }
 
 MSX_API int msx_fft2d(msx_ctx ctx, void* data, int nbatch, int nx, int ny, int sign, int prec, int where) {

@@ -83,6 +83,15 @@ struct SplatArgs {
 hipError_t launch_splat(const SplatArgs& a, hipStream_t s);
 hipError_t upload_fparams(hipStream_t s);
 
+// frozen-phonon displacement (msx_potential.hip); u_rms in device memory [104]
+struct AtomIO {
+    int32_t Z;
+    float occ;
+    double x, y, z;
+};
+hipError_t launch_displace(const AtomIO* in, long long n, const double* u_rms, unsigned long long seed, int config,
+                           AtomIO* out, hipStream_t s);
+
 // detector finish: out[d][p] += w * sum_g partial[d][b][g] (p = probe0 + b)
 hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups, double w, double* out,
                              long long out_stride_d, long long probe0, hipStream_t s);

@@ -13,6 +13,7 @@
 // The per-pixel contribution rule is the reference's: minimum image per axis,
 // r < radius, r clamped >= 1e-10, formula and table selectable.
 #include "msx_internal.h"
+#include "msx_philox.h"
 #include "fparams_tables.inc"
 
 namespace msx {
@@ -120,6 +121,35 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
 hipError_t launch_splat(const SplatArgs& a, hipStream_t s) {
     dim3 grid((a.ny + 15) / 16, (a.nx + 15) / 16, a.nslices);
     hipLaunchKernelGGL(splat_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// Frozen-phonon displacement: one thread per atom, Philox counter (atom, config)
+__global__ __launch_bounds__(256) void displace_kernel(const AtomIO* in, long long n, const double* u_rms,
+                                                       unsigned long long seed, int config, AtomIO* out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    AtomIO a = in[i];
+    const double u = (a.Z >= 0 && a.Z < 104) ? u_rms[a.Z] : 0.0;
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    const uint32_t c0 = (uint32_t)i, c1 = (uint32_t)((unsigned long long)i >> 32);
+    const Philox4 r0 = philox4x32_10(c0, c1, (uint32_t)config, 0u, k0, k1);
+    const Philox4 r1 = philox4x32_10(c0, c1, (uint32_t)config, 1u, k0, k1);
+    const double twopi = 6.283185307179586476925287;
+    const double u1 = uniform53(r0.v[0], r0.v[1]), u2 = uniform53(r0.v[2], r0.v[3]);
+    const double u3 = uniform53(r1.v[0], r1.v[1]), u4 = uniform53(r1.v[2], r1.v[3]);
+    const double ra = sqrt(-2.0 * log(u1)), rb = sqrt(-2.0 * log(u3));
+    a.x += u * ra * cos(twopi * u2);
+    a.y += u * ra * sin(twopi * u2);
+    a.z += u * rb * cos(twopi * u4);
+    out[i] = a;
+}
+
+hipError_t launch_displace(const AtomIO* in, long long n, const double* u_rms, unsigned long long seed, int config,
+                           AtomIO* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(displace_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, n, u_rms, seed,
+                       config, out);
     return hipGetLastError();
 }
 

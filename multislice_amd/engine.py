@@ -69,7 +69,7 @@ class Detector(C.Structure):
 PROF_ROW, PROF_COL, PROF_SPLAT, PROF_DET = 0, 1, 2, 3
 
 EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_last_error", "msx_version",
-           "msx_profile", "msx_profile_read",
+           "msx_profile", "msx_profile_read", "msx_displace", "msx_philox4x32_10",
            "msx_wavelength", "msx_sigma", "msx_vzatom", "msx_featom", "msx_fft2d", "msx_potential",
            "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image"]
 
@@ -114,6 +114,8 @@ def load():
         "msx_profile_read": (i, [vp, i, C.POINTER(C.c_int64), C.POINTER(d), C.POINTER(d)]),
         "msx_wavelength": (d, [d]), "msx_sigma": (d, [d]), "msx_vzatom": (d, [i, d, i, i]),
         "msx_featom": (d, [i, d, i]),
+        "msx_displace": (i, [vp, vp, C.c_int64, vp, C.c_uint64, C.c_int32, vp, i]),
+        "msx_philox4x32_10": (None, [vp, vp, vp]),
         "msx_fft2d": (i, [vp, vp, i, i, i, i, i, i]),
         "msx_potential": (i, [vp, vp, C.c_int64, d, d, d, i, i, i, d, i, i, vp, i, i]),
         "msx_transmission": (i, [vp, vp, i, i, i, d, d, d, d, vp, i, i]),
@@ -330,6 +332,41 @@ def context():
         if _default is None:
             _default = Context(0)
         return _default
+
+
+def displace(atoms, u_rms, seed, config, ctx=None, out=None):
+    """Frozen-phonon displacement (msx_displace).  atoms: structured ATOM_DTYPE
+    numpy array (host, ctx optional) or a torch CUDA uint8 view of one (device,
+    ctx required); u_rms: {Z: u} or a 104-array (Angstrom per axis)."""
+    u = np.zeros(104)
+    if isinstance(u_rms, dict):
+        for z, v in u_rms.items():
+            u[int(z)] = v
+    else:
+        u[:] = np.asarray(u_rms, np.float64)
+    if _is_torch(atoms):
+        if ctx is None:
+            raise ValueError("device displacement needs a Context")
+        n = atoms.numel() // ATOM_DTYPE.itemsize
+        if out is None:
+            out = atoms.clone()
+        _check(load().msx_displace(ctx.h, C.c_void_p(atoms.data_ptr()), n, C.c_void_p(u.ctypes.data), seed, config,
+                                   C.c_void_p(out.data_ptr()), DEVICE))
+        return out
+    at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+    if out is None:
+        out = np.empty_like(at)
+    _check(load().msx_displace(ctx.h if ctx else None, C.c_void_p(at.ctypes.data), len(at),
+                               C.c_void_p(u.ctypes.data), seed, config, C.c_void_p(out.ctypes.data), HOST))
+    return out
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    o = np.zeros(4, np.uint32)
+    load().msx_philox4x32_10(C.c_void_p(c.ctypes.data), C.c_void_p(k.ctypes.data), C.c_void_p(o.ctypes.data))
+    return o
 
 
 def wavelength(kev):

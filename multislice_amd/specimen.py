@@ -53,15 +53,10 @@ def srtio3_001(ncell=(12, 12, 50), a=3.905):
 
 def frozen_phonon(atoms, u_rms, seed, config):
     """Isotropic Gaussian displacements (x, y, z) with per-element RMS u_rms
-    {Z: u} (Angstrom, per axis), reproducible per (seed, config)."""
-    rng = np.random.default_rng([int(seed), int(config)])
-    out = atoms.copy()
-    u = np.array([u_rms.get(int(z), 0.0) for z in atoms["Z"]])
-    d = rng.standard_normal((len(atoms), 3)) * u[:, None]
-    out["x"] += d[:, 0]
-    out["y"] += d[:, 1]
-    out["z"] += d[:, 2]
-    return out
+    {Z: u} (Angstrom, per axis), reproducible per (seed, config, atom):
+    msx_displace (Philox-4x32-10, identical on host and GPU)."""
+    from .engine import displace
+    return displace(atoms, u_rms, seed, config)
 
 
 SRTIO3_U = {38: 0.088, 22: 0.075, 8: 0.095}  # Angstrom, per axis

@@ -113,3 +113,14 @@ def test_repeat_calls_are_independent(ctx, small):
     ctx.stem(other, args["cz"], args["layer"], cell[0], cell[1], KEV, dict(alpha_max=0.03), pos[::-1], det, [nsl])
     a2 = ctx.stem(t64, args["cz"], args["layer"], cell[0], cell[1], KEV, dict(alpha_max=0.02), pos, det, [1, nsl])
     assert np.array_equal(a1, a2)
+
+
+def test_displacement_host_equals_device(ctx):
+    torch = pytest.importorskip("torch")
+    at, _ = specimen.srtio3_001((6, 6, 10))
+    host = msx.displace(at, specimen.SRTIO3_U, seed=99, config=5)
+    dev_in = torch.from_numpy(at.view(np.uint8).copy()).cuda()
+    dev = msx.displace(dev_in, specimen.SRTIO3_U, seed=99, config=5, ctx=ctx).cpu().numpy().view(msx.ATOM_DTYPE)
+    for f in ("x", "y", "z"):
+        assert np.abs(dev[f] - host[f]).max() < 1e-13
+    assert np.array_equal(dev["Z"], host["Z"])
