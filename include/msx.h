@@ -216,6 +216,41 @@ MSX_API int msx_ctf_image(msx_ctx ctx, const void* psi, int nwave, int nx, int n
                           double kev, double cs, double df, double aobj, void* psi_img, void* intensity,
                           int prec, int where);
 
+/* ---- frozen-phonon surrogate (no reference counterpart) --------------------
+ * A U-Net that maps the mean ADF image of a few frozen-phonon configurations
+ * to the converged many-configuration mean; trained and run on the GPU by the
+ * library's own BF16 matrix-core convolution kernels (FP32 accumulation).
+ * Architecture: `levels` resolutions with base_channels << l channels, two
+ * 3x3 conv+ReLU per level, 2x2 average pooling, nearest upsampling with skip
+ * concatenation, a 1x1 head and a residual connection (output = input + head).
+ * Images are FP32, n x h x w (one channel), h and w multiples of
+ * 2^(levels-1).  Parameters are one packed FP32 vector: conv by conv (encoder
+ * levels 0..L-1 two each, decoder levels L-2..0 two each, head), each as
+ * W[Cout][Cin][k*k] followed by bias[Cout].  Loss: mean squared error. */
+typedef struct msx_unet_s* msx_unet;
+MSX_API int msx_unet_create(msx_ctx ctx, int levels, int base_channels, uint64_t seed, msx_unet* out);
+MSX_API int msx_unet_destroy(msx_unet net);
+MSX_API int64_t msx_unet_param_count(msx_unet net);
+MSX_API int msx_unet_params(msx_unet net, float* dst, int where);
+/* replaces the parameters and resets the optimiser state */
+MSX_API int msx_unet_set_params(msx_unet net, const float* src, int where);
+MSX_API int msx_unet_forward(msx_unet net, const float* x, float* y, int n, int h, int w, int where);
+/* loss (may be NULL) and its gradient w.r.t. the packed parameters (grad may be NULL) */
+MSX_API int msx_unet_gradient(msx_unet net, const float* x, const float* target, int n, int h, int w,
+                              double* loss, float* grad, int where);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch; loss may be NULL */
+MSX_API int msx_unet_train_step(msx_unet net, const float* x, const float* target, int n, int h, int w, float lr,
+                                double* loss, int where);
+/* One convolution of the network's kernels on device buffers (stride 1, zero
+ * padding ks/2, ks = 1 or 3, NCHW FP32, BF16 operands, FP32 accumulation):
+ *   mode 0: out[n,co] = relu?(sum W[co,ci,tap] x[n,ci,p+off(tap)] + bias[co])
+ *   mode 1: out[n,ci] = sum W[co,ci,tap] G[n,co,p-off(tap)]           (input gradient)
+ *   mode 2: out[co,ci,tap] += sum_{n,p} G[n,co,p] x[n,ci,p+off(tap)]  (weight gradient)
+ * with G = g masked by (y > 0) when relu is set (y = the forward output). */
+MSX_API int msx_unet_conv(msx_ctx ctx, int mode, int ks, const float* x, const float* w, const float* bias,
+                          const float* g, const float* y, float* out, int n, int cin, int cout, int h, int wd,
+                          int relu);
+
 #ifdef __cplusplus
 }
 #endif

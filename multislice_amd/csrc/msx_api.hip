@@ -16,7 +16,7 @@
 
 using namespace msx;
 
-namespace {
+namespace msx {
 
 thread_local std::string g_err;
 
@@ -29,6 +29,10 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
+
+}  // namespace msx
+
+namespace {
 
 #define HIPCHK(x)                                                                                    \
     do {                                                                                             \
@@ -680,6 +684,9 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
 }
 
 }  // namespace
+
+int msx_ctx_device(const msx_ctx_s* ctx) { return ctx->device; }
+hipStream_t msx_ctx_stream(const msx_ctx_s* ctx) { return ctx->stream; }
 
 // ============================================================== C ABI
 extern "C" {

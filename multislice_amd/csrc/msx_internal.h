@@ -96,4 +96,13 @@ hipError_t launch_displace(const AtomIO* in, long long n, const double* u_rms, u
 hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups, double w, double* out,
                              long long out_stride_d, long long probe0, hipStream_t s);
 
+// error reporting shared by the C ABI units: records the message for
+// msx_last_error() and returns `code`
+int fail(int code, const char* fmt, ...);
+
 }  // namespace msx
+
+// context accessors for C ABI units other than msx_api.hip
+struct msx_ctx_s;
+int msx_ctx_device(const msx_ctx_s* ctx);
+hipStream_t msx_ctx_stream(const msx_ctx_s* ctx);

@@ -1,0 +1,322 @@
+// msx_unet_kernels.hip -- GPU kernels of the frozen-phonon surrogate (a small
+// U-Net that maps a 4-configuration mean ADF image to the converged
+// 64-configuration one).
+//
+// Every convolution (3x3 or 1x1, stride 1, zero padding) is an implicit GEMM on
+// the gfx950 matrix cores: v_mfma_f32_32x32x16_bf16, BF16 operands converted
+// from the FP32 tensors while they are staged in LDS, FP32 accumulation.
+// The same kernel runs the three products of training:
+//   FWD : Y[n,co,p]   = sum_{ci,tap} W[co,ci,tap] X[n,ci,p+off(tap)]    (+bias, ReLU)
+//   BWDD: dX[n,ci,p]  = sum_{co,tap} W[co,ci,flip(tap)] G[n,co,p+off(tap)]
+//   BWDW: dW[co,ci,tap] += sum_{n,p} G[n,co,p] X[n,ci,p+off(tap)]       (split over p)
+// where G = dY masked by the ReLU of the layer's output.  Tensors are NCHW FP32.
+#include <hip/hip_runtime.h>
+
+#include "msx_unet.h"
+
+namespace msx {
+namespace unet {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 64, BN = 64, BK = 16;  // block tile; 4 wavefronts of 32x32
+
+// activation gather X[n, c, y+dy, x+dx] with zero padding
+__device__ __forceinline__ float gather(const float* X, int C, int H, int W, int n, int c, int y, int x) {
+    if ((unsigned)y >= (unsigned)H || (unsigned)x >= (unsigned)W) return 0.f;
+    return X[(((long long)n * C + c) * H + y) * W + x];
+}
+
+template <int MODE, int KS>
+__global__ __launch_bounds__(256) void conv_gemm(ConvArgs p) {
+    __shared__ __bf16 As[BM][BK + 8];  // [m][k]  (+8 keeps 16-byte rows and spreads banks)
+    __shared__ __bf16 Bs[BN][BK + 8];  // [n][k]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int HW = p.H * p.W;
+    constexpr int TAPS = KS * KS, R = KS / 2;
+    // GEMM sizes per mode
+    long long M, N, K;
+    if (MODE == kFwd) { M = p.Cout; N = (long long)p.N * HW; K = (long long)p.Cin * TAPS; }
+    else if (MODE == kBwdData) { M = p.Cin; N = (long long)p.N * HW; K = (long long)p.Cout * TAPS; }
+    else { M = p.Cout; N = (long long)p.Cin * TAPS; K = (long long)p.N * HW; }
+    const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
+    // split-K range (weight gradient only)
+    long long k_beg = 0, k_end = K;
+    if (MODE == kBwdWeight) {
+        const long long chunk = ((K + gridDim.z - 1) / gridDim.z + BK - 1) / BK * BK;
+        k_beg = (long long)blockIdx.z * chunk;
+        k_end = k_beg + chunk < K ? k_beg + chunk : K;
+    }
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+
+    // element (row r, col c) of the staging pattern: 256 threads x 4 values
+    for (long long k0 = k_beg; k0 < k_end; k0 += BK) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + q * 256;      // 0..1023
+            const int mm = e / BK, kk = e % BK;  // A: [64][16]
+            const long long m = m0 + mm, k = k0 + kk;
+            float va = 0.f;
+            if (m < M && k < k_end) {
+                if (MODE == kFwd) {
+                    va = p.w[m * K + k];  // W[co][ci][tap] with k = ci*TAPS + tap
+                } else if (MODE == kBwdData) {
+                    const int co = (int)(k / TAPS), tap = (int)(k % TAPS);
+                    va = p.w[((long long)co * p.Cin + m) * TAPS + (TAPS - 1 - tap)];
+                } else {
+                    const long long n = k / HW, pix = k % HW;
+                    const long long o = ((n * p.Cout + m) * HW + pix);
+                    va = p.g[o];
+                    if (p.relu && p.y[o] <= 0.f) va = 0.f;
+                }
+            }
+            As[mm][kk] = (__bf16)va;
+            const int nn = e / BK, kb = e % BK;  // B: [64 n][16 k]
+            const long long n = n0 + nn, kB = k0 + kb;
+            float vb = 0.f;
+            if (n < N && kB < k_end) {
+                if (MODE == kFwd || MODE == kBwdData) {
+                    const int img = (int)(n / HW), pix = (int)(n % HW), y = pix / p.W, x = pix % p.W;
+                    const int c = (int)(kB / TAPS), tap = (int)(kB % TAPS);
+                    const int dy = tap / KS - R, dx = tap % KS - R;
+                    if (MODE == kFwd) {
+                        vb = gather(p.x, p.Cin, p.H, p.W, img, c, y + dy, x + dx);
+                    } else {
+                        const int yy = y + dy, xx = x + dx;
+                        if ((unsigned)yy < (unsigned)p.H && (unsigned)xx < (unsigned)p.W) {
+                            const long long o = (((long long)img * p.Cout + c) * p.H + yy) * p.W + xx;
+                            vb = p.g[o];
+                            if (p.relu && p.y[o] <= 0.f) vb = 0.f;
+                        }
+                    }
+                } else {
+                    const int ci = (int)(n / TAPS), tap = (int)(n % TAPS);
+                    const int dy = tap / KS - R, dx = tap % KS - R;
+                    const int img = (int)(kB / HW), pix = (int)(kB % HW), y = pix / p.W, x = pix % p.W;
+                    vb = gather(p.x, p.Cin, p.H, p.W, img, ci, y + dy, x + dx);
+                }
+            }
+            Bs[nn][kb] = (__bf16)vb;
+        }
+        __syncthreads();
+        // fragments: lane l holds row/col (l % 32), k = 8*(l/32) .. +7
+        const int r = lane & 31, kh = (lane >> 5) * 8;
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + r][kh]);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + r][kh]);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        __syncthreads();
+    }
+    // epilogue: acc[v] is C[row][col], col = lane % 32,
+    // row = 8*(v/4) + 4*(lane/32) + v%4 (32x32 MFMA output layout)
+    const int col = lane & 31;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const long long m = m0 + wm * 32 + row, n = n0 + wn * 32 + col;
+        if (m >= M || n >= N) continue;
+        float c = acc[v];
+        if (MODE == kFwd) {
+            if (p.bias) c += p.bias[m];
+            if (p.relu && c < 0.f) c = 0.f;
+            const int img = (int)(n / HW), pix = (int)(n % HW);
+            float* dst = p.out + ((long long)img * p.out_c + p.out_c0 + m) * HW + pix;
+            if (p.accumulate) *dst += c; else *dst = c;
+        } else if (MODE == kBwdData) {
+            const int img = (int)(n / HW), pix = (int)(n % HW);
+            float* dst = p.out + ((long long)img * p.out_c + p.out_c0 + m) * HW + pix;
+            if (p.accumulate) *dst += c; else *dst = c;
+        } else {
+            atomicAdd(p.out + m * N + n, c);  // dW[co][ci][tap], vector atomics
+        }
+    }
+}
+
+hipError_t launch_conv(int mode, int ks, const ConvArgs& a, hipStream_t s) {
+    const int HW = a.H * a.W;
+    long long M, N, K;
+    if (mode == kFwd) { M = a.Cout; N = (long long)a.N * HW; K = (long long)a.Cin * ks * ks; }
+    else if (mode == kBwdData) { M = a.Cin; N = (long long)a.N * HW; K = (long long)a.Cout * ks * ks; }
+    else { M = a.Cout; N = (long long)a.Cin * ks * ks; K = (long long)a.N * HW; }
+    int splits = 1;
+    if (mode == kBwdWeight) {
+        const long long tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+        long long want = 2048 / (tiles > 0 ? tiles : 1);
+        long long maxs = (K + 4 * BK - 1) / (4 * BK);
+        splits = (int)(want < 1 ? 1 : (want > maxs ? maxs : want));
+        if (splits < 1) splits = 1;
+    }
+    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM), (unsigned)splits);
+#define MSX_CONV(MODE_, KS_) hipLaunchKernelGGL((conv_gemm<MODE_, KS_>), grid, dim3(256), 0, s, a)
+    if (ks == 3) {
+        if (mode == kFwd) MSX_CONV(kFwd, 3);
+        else if (mode == kBwdData) MSX_CONV(kBwdData, 3);
+        else MSX_CONV(kBwdWeight, 3);
+    } else if (ks == 1) {
+        if (mode == kFwd) MSX_CONV(kFwd, 1);
+        else if (mode == kBwdData) MSX_CONV(kBwdData, 1);
+        else MSX_CONV(kBwdWeight, 1);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef MSX_CONV
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ small ops
+__global__ void avgpool2_fwd(const float* x, float* y, long long nc, int H, int W) {
+    const int Ho = H / 2, Wo = W / 2;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc * Ho * Wo) return;
+    const long long c = i / (Ho * Wo);
+    const int pix = (int)(i % (Ho * Wo)), yo = pix / Wo, xo = pix % Wo;
+    const float* s = x + c * H * W + (2 * yo) * W + 2 * xo;
+    y[i] = 0.25f * (s[0] + s[1] + s[W] + s[W + 1]);
+}
+__global__ void avgpool2_bwd(const float* gy, float* gx, long long nc, int H, int W) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc * H * W) return;
+    const long long c = i / (H * W);
+    const int pix = (int)(i % (H * W)), y = pix / W, x = pix % W;
+    gx[i] += 0.25f * gy[(c * (H / 2) + y / 2) * (W / 2) + x / 2];
+}
+// nearest-neighbour x2 upsampling into channels [c0, c0+C) of a tensor with Ct channels
+__global__ void upsample2_fwd(const float* x, float* y, int N, int C, int H, int W, int Ct, int c0) {
+    const long long tot = (long long)N * C * 4 * H * W;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tot) return;
+    const int Wo = 2 * W, Ho = 2 * H;
+    const int xo = (int)(i % Wo), yo = (int)((i / Wo) % Ho);
+    const long long nc = i / ((long long)Wo * Ho);
+    const int n = (int)(nc / C), c = (int)(nc % C);
+    y[(((long long)n * Ct + c0 + c) * Ho + yo) * Wo + xo] = x[((long long)nc * H + yo / 2) * W + xo / 2];
+}
+__global__ void upsample2_bwd(const float* gy, float* gx, int N, int C, int H, int W, int Ct, int c0) {
+    const long long tot = (long long)N * C * H * W;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tot) return;
+    const int x = (int)(i % W), y = (int)((i / W) % H);
+    const long long nc = i / ((long long)W * H);
+    const int n = (int)(nc / C), c = (int)(nc % C);
+    const int Wo = 2 * W;
+    const float* s = gy + (((long long)n * Ct + c0 + c) * 2 * H + 2 * y) * Wo + 2 * x;
+    gx[i] += s[0] + s[1] + s[Wo] + s[Wo + 1];
+}
+// copy channels: dst[n, d0 + c] (+)= src[n, s0 + c], c < C
+__global__ void copy_channels(const float* src, int Cs, int s0, float* dst, int Cd, int d0, int N, int C, int HW,
+                              int accumulate) {
+    const long long tot = (long long)N * C * HW;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= tot) return;
+    const int pix = (int)(i % HW);
+    const long long nc = i / HW;
+    const int n = (int)(nc / C), c = (int)(nc % C);
+    const float v = src[((long long)n * Cs + s0 + c) * HW + pix];
+    float* d = dst + ((long long)n * Cd + d0 + c) * HW + pix;
+    if (accumulate) *d += v; else *d = v;
+}
+// bias gradient: one block per channel, deterministic tree reduction
+__global__ __launch_bounds__(256) void bias_grad(const float* g, const float* y, int relu, int N, int C, int HW,
+                                                 float* db) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (long long i = threadIdx.x; i < (long long)N * HW; i += 256) {
+        const int n = (int)(i / HW), pix = (int)(i % HW);
+        const long long o = ((long long)n * C + c) * HW + pix;
+        const float v = g[o];
+        s += (relu && y[o] <= 0.f) ? 0.f : v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
+}
+// residual output out = in + z; MSE loss gradient g = 2 (out - t) / count
+__global__ void residual_mse(const float* in, const float* z, const float* t, float* out, float* g, long long n,
+                             float scale) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float o = in[i] + z[i];
+    out[i] = o;
+    if (g) g[i] = scale * (o - t[i]);
+}
+// per-block partial sums of (a - b)^2 in FP64 (fixed order: deterministic loss)
+__global__ __launch_bounds__(256) void sq_diff(const float* a, const float* b, long long n, double* partial) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double d = (double)a[i] - (double)b[i];
+        s += d * d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+// Adam (Kingma & Ba 2015), bias-corrected
+__global__ void adam_step(float* w, const float* gw, float* m, float* v, long long n, float lr, float b1, float b2,
+                          float eps, float c1, float c2) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = gw[i];
+    const float mi = b1 * m[i] + (1.f - b1) * g;
+    const float vi = b2 * v[i] + (1.f - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+#define MSX_GRID(n) dim3((unsigned)(((n) + 255) / 256)), dim3(256), 0, s
+hipError_t launch_avgpool_fwd(const float* x, float* y, long long nc, int H, int W, hipStream_t s) {
+    hipLaunchKernelGGL(avgpool2_fwd, MSX_GRID(nc * (H / 2) * (W / 2)), x, y, nc, H, W);
+    return hipGetLastError();
+}
+hipError_t launch_avgpool_bwd(const float* gy, float* gx, long long nc, int H, int W, hipStream_t s) {
+    hipLaunchKernelGGL(avgpool2_bwd, MSX_GRID(nc * H * W), gy, gx, nc, H, W);
+    return hipGetLastError();
+}
+hipError_t launch_upsample_fwd(const float* x, float* y, int N, int C, int H, int W, int Ct, int c0, hipStream_t s) {
+    hipLaunchKernelGGL(upsample2_fwd, MSX_GRID((long long)N * C * 4 * H * W), x, y, N, C, H, W, Ct, c0);
+    return hipGetLastError();
+}
+hipError_t launch_upsample_bwd(const float* gy, float* gx, int N, int C, int H, int W, int Ct, int c0, hipStream_t s) {
+    hipLaunchKernelGGL(upsample2_bwd, MSX_GRID((long long)N * C * H * W), gy, gx, N, C, H, W, Ct, c0);
+    return hipGetLastError();
+}
+hipError_t launch_copy_channels(const float* src, int Cs, int s0, float* dst, int Cd, int d0, int N, int C, int HW,
+                                int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL(copy_channels, MSX_GRID((long long)N * C * HW), src, Cs, s0, dst, Cd, d0, N, C, HW, accumulate);
+    return hipGetLastError();
+}
+hipError_t launch_bias_grad(const float* g, const float* y, int relu, int N, int C, int HW, float* db, hipStream_t s) {
+    hipLaunchKernelGGL(bias_grad, dim3(C), dim3(256), 0, s, g, y, relu, N, C, HW, db);
+    return hipGetLastError();
+}
+hipError_t launch_residual_mse(const float* in, const float* z, const float* t, float* out, float* g, long long n,
+                               float scale, hipStream_t s) {
+    hipLaunchKernelGGL(residual_mse, MSX_GRID(n), in, z, t, out, g, n, scale);
+    return hipGetLastError();
+}
+hipError_t launch_sq_diff(const float* a, const float* b, long long n, double* partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(sq_diff, dim3(nblocks), dim3(256), 0, s, a, b, n, partial);
+    return hipGetLastError();
+}
+hipError_t launch_adam(float* w, const float* gw, float* m, float* v, long long n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, hipStream_t s) {
+    hipLaunchKernelGGL(adam_step, MSX_GRID(n), w, gw, m, v, n, lr, b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+#undef MSX_GRID
+
+}  // namespace unet
+}  // namespace msx

@@ -71,7 +71,10 @@ PROF_ROW, PROF_COL, PROF_SPLAT, PROF_DET = 0, 1, 2, 3
 EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_last_error", "msx_version",
            "msx_profile", "msx_profile_read", "msx_displace", "msx_philox4x32_10",
            "msx_wavelength", "msx_sigma", "msx_vzatom", "msx_featom", "msx_fft2d", "msx_potential",
-           "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image"]
+           "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image",
+           "msx_unet_create", "msx_unet_destroy", "msx_unet_param_count", "msx_unet_params",
+           "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
+           "msx_unet_conv"]
 
 _lib = None
 _lock = threading.Lock()
@@ -123,6 +126,12 @@ def load():
         "msx_stem": (i, [vp, C.POINTER(Stack), vp, C.POINTER(Probe), vp, i, C.POINTER(Detector), i, vp, i, d, i,
                          vp, i, i, i]),
         "msx_ctf_image": (i, [vp, vp, i, i, i, d, d, d, d, d, d, vp, vp, i, i]),
+        "msx_unet_create": (i, [vp, i, i, C.c_uint64, C.POINTER(vp)]), "msx_unet_destroy": (i, [vp]),
+        "msx_unet_param_count": (C.c_int64, [vp]), "msx_unet_params": (i, [vp, vp, i]),
+        "msx_unet_set_params": (i, [vp, vp, i]), "msx_unet_forward": (i, [vp, vp, vp, i, i, i, i]),
+        "msx_unet_gradient": (i, [vp, vp, vp, i, i, i, C.POINTER(d), vp, i]),
+        "msx_unet_train_step": (i, [vp, vp, vp, i, i, i, C.c_float, C.POINTER(d), i]),
+        "msx_unet_conv": (i, [vp, i, i, vp, vp, vp, vp, vp, vp, i, i, i, i, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -320,6 +329,106 @@ class Context:
         _check(load().msx_ctf_image(self.h, pw[0], nw, nx, ny, ax, by, kev, cs, df, aobj, pimg, pi[0], prec,
                                     pw[1]))
         return (inten, img) if want_wave else inten
+
+
+def unet_conv(ctx, mode, ks, out, w, x=None, bias=None, g=None, y=None, relu=False):
+    """One convolution of the surrogate's kernels (msx_unet_conv) on torch CUDA
+    float32 tensors: x [n, cin, h, w], w [cout, cin*ks*ks], g/y [n, cout, h, w];
+    out is written (mode 0, 1) or accumulated into (mode 2)."""
+    def ptr(t):
+        if t is None:
+            return None
+        if not (_is_torch(t) and t.is_cuda and t.is_contiguous() and str(t.dtype) == "torch.float32"):
+            raise ValueError("msx_unet_conv takes contiguous float32 CUDA tensors")
+        return C.c_void_p(t.data_ptr())
+    ref = x if x is not None else g
+    n, _, hh, ww = ref.shape
+    cout = w.shape[0]
+    cin = w.shape[1] // (ks * ks)
+    _check(load().msx_unet_conv(ctx.h, mode, ks, ptr(x), ptr(w), ptr(bias), ptr(g), ptr(y), ptr(out), n, cin, cout,
+                                hh, ww, int(relu)))
+    return out
+
+
+class UNet:
+    """The frozen-phonon surrogate network (msx_unet_*), resident on the
+    context's GPU.  Images are (n, h, w) float32: numpy (host) or torch CUDA
+    tensors (device); h, w multiples of 2**(levels-1)."""
+
+    def __init__(self, ctx, levels=4, base=32, seed=0):
+        self.ctx, self.levels, self.base = ctx, levels, base
+        h = C.c_void_p()
+        _check(load().msx_unet_create(ctx.h, levels, base, seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().msx_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nparam(self):
+        return int(load().msx_unet_param_count(self.h))
+
+    def params(self):
+        p = np.empty(self.nparam, np.float32)
+        _check(load().msx_unet_params(self.h, C.c_void_p(p.ctypes.data), HOST))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.nparam:
+            raise ValueError(f"{p.size} parameters given, the network has {self.nparam}")
+        _check(load().msx_unet_set_params(self.h, C.c_void_p(p.ctypes.data), HOST))
+
+    @staticmethod
+    def _img(x):
+        b = _buf(x, F32, complex_=False)
+        if len(b[2].shape) != 3:
+            raise ValueError("images must be (n, h, w)")
+        return b
+
+    def forward(self, x, out=None):
+        bx = self._img(x)
+        n, hh, ww = bx[2].shape
+        if out is None:
+            out = np.empty((n, hh, ww), np.float32) if bx[1] == HOST else x.new_empty((n, hh, ww))
+        bo = _buf(out, F32, complex_=False, writable=True)
+        where = _ptr_where([bx, bo])
+        _check(load().msx_unet_forward(self.h, bx[0], bo[0], n, hh, ww, where))
+        return out
+
+    def gradient(self, x, target):
+        """(loss, packed gradient as numpy float32)."""
+        bx, bt = self._img(x), self._img(target)
+        n, hh, ww = bx[2].shape
+        g = np.empty(self.nparam, np.float32)
+        loss = C.c_double()
+        where = _ptr_where([bx, bt])
+        if where == DEVICE:
+            import torch
+            gd = torch.empty(self.nparam, dtype=torch.float32, device=x.device)
+            _check(load().msx_unet_gradient(self.h, bx[0], bt[0], n, hh, ww, C.byref(loss),
+                                            C.c_void_p(gd.data_ptr()), DEVICE))
+            g = gd.cpu().numpy()
+        else:
+            _check(load().msx_unet_gradient(self.h, bx[0], bt[0], n, hh, ww, C.byref(loss),
+                                            C.c_void_p(g.ctypes.data), HOST))
+        return loss.value, g
+
+    def train_step(self, x, target, lr):
+        bx, bt = self._img(x), self._img(target)
+        n, hh, ww = bx[2].shape
+        loss = C.c_double()
+        _check(load().msx_unet_train_step(self.h, bx[0], bt[0], n, hh, ww, float(lr), C.byref(loss),
+                                          _ptr_where([bx, bt])))
+        return loss.value
 
 
 _default = None
