@@ -136,6 +136,37 @@ MSX_API int msx_fft2d(msx_ctx ctx, void* data, int nbatch, int nx, int ny, int s
 MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c,
                           int nx, int ny, int nslices, double radius, int table, int formula, void* V,
                           int prec, int where);
+/* Reciprocal-space projected potential from structure factors: Muslib
+ * Potential_rec_DT_DW (Muslib/Pot_rec_DT_DW.cpp:3-132) followed by the inverse
+ * transform of the phase-grating build (MSProjectReuse.cs:425-431).  Per slice
+ * (atoms assigned as in msx_potential):
+ *   V(g) = sum_atoms f_e(|g|/2) exp(-dw |g|^2/4) occ / area exp(-2 pi i g.r) (1 + i ads)
+ * with Doyle-Turner f_e (Curves.cpp:3-11, table Curves.cpp:83-179, Z 1..92),
+ * dw the Debye-Waller B (A^2) and ads the absorptive fraction of each atom
+ * (NULL: 0), over the reflections g = (h/a, k/b) of the unit cell a x b (angle
+ * gam, a along x) whose grid indices |h|*nmult < nx/2 and |k|*mmult < ny/2;
+ * the grid spans the supercell nmult*a x mmult*b (pass a, b of the UNIT cell
+ * and atoms inside it).  Then V(r) = sum_g V(g) exp(+2 pi i g.r) in FP64, and
+ * V = 2 pi a0 e Re V(r) (V*A, the units of msx_potential), Vabs = 2 pi a0 e
+ * Im V(r) (may be NULL).  Layout as msx_potential: [slice][ix][iy]. */
+/* Real-space projected potential from Doyle-Turner Gaussians over pixel boxes:
+ * Muslib Potential_rs_DT_DW_cutpot (periodic = 0, Pot_RS.cpp:111-155) and
+ * Potential_rs_DT_DW_periodic (periodic = 1, :350-486).  Every pixel of the box
+ * of an atom (fractional position xf, yf in the cell a x b at angle gam):
+ *   i in [max(ibb,0), min(iee,nx)), ibb = (long)((xf - atrad sin(gam)/a) nx),
+ *   iee = (long)((xf + atrad sin(gam)/a) nx), likewise j along b,
+ * gets occ * 4 pi sum_j a_j/(b_j+dw) exp(-4 pi^2 r^2/(b_j+dw)) (Curves.cpp:23-34)
+ * with r^2 = x (x + 2 y cos gam) + y^2 to the atom, and times ads in Vabs.  The
+ * periodic form adds the eight pieces of the box wrapped across the cell edges
+ * (atom image shifted by a and/or b; a piece wrapped from the high edge keeps
+ * one pixel more, as the reference does).  Scaled by 2 pi a0 e to V*A, layout
+ * [slice][ix][iy] as msx_potential; any nx, ny >= 1. */
+MSX_API int msx_potential_rs(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw,
+                             const double* ads, double a, double b, double c, double gam, double atrad, int periodic,
+                             int nx, int ny, int nslices, void* V, void* Vabs, int prec, int where);
+MSX_API int msx_potential_rec(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw,
+                              const double* ads, double a, double b, double c, double gam, int nmult, int mmult,
+                              int nx, int ny, int nslices, void* V, void* Vabs, int prec, int where);
 
 /* ---- transmission ----------------------------------------------------- */
 /* Transmission functions of nlayer potential layers (mulslice.cpp:414-421 and
@@ -146,6 +177,11 @@ MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, do
  * V: nlayer real images; t: nlayer complex images. */
 MSX_API int msx_transmission(msx_ctx ctx, const void* V, int nlayer, int nx, int ny, double ax, double by,
                              double kev, double bw, void* t, int prec, int where);
+/* As msx_transmission with an absorptive potential (Muslib phase grating,
+ * MSProjectReuse.cs:438-457): t = exp(i sigma V/1000) exp(-sigma Vabs/1000)
+ * before the band limit.  Vabs may be NULL. */
+MSX_API int msx_transmission_abs(msx_ctx ctx, const void* V, const void* Vabs, int nlayer, int nx, int ny,
+                                 double ax, double by, double kev, double bw, void* t, int prec, int where);
 
 /* ---- slice stack -------------------------------------------------------- */
 /* The specimen/optics description of one multislice run: the layers, their

@@ -299,13 +299,14 @@ template <typename C> int fft2d_dev(msx_ctx ctx, C* d, int nb, int nx, int ny, i
 // ---------------------------------------------------------------- transmission
 template <typename C>
 int transmission_dev(msx_ctx ctx, const typename real_of<C>::type* V, int nl, int nx, int ny, double ax, double by,
-                     double kev, double bw, C* t) {
+                     double kev, double bw, C* t, const typename real_of<C>::type* Vabs = nullptr) {
     using Re = typename real_of<C>::type;
     const C *twx, *twy;
     CHK(ctx->twiddles<C>(nx, &twx));
     CHK(ctx->twiddles<C>(ny, &twy));
     RowArgs<C> r = row_args<C>(t, nx, nl, twy);
     r.vreal = V;
+    r.vabs = Vabs;
     r.vscale = (Re)(msx_sigma(kev) / 1000.0);
     if (!(bw > 0)) {
         LAUNCH_ROW(ny, kRowNone, r);
@@ -683,6 +684,219 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     return MSX_OK;
 }
 
+// 2 pi a0 e (V A^2): projected potential in V A of the Fourier sum of f_e/area
+// (Kirkland's vzatom constant al = 4 pi^2 a0 e = 300.8242834, slicelib.cpp:2666-2699)
+constexpr double kVA2 = 300.8242834 / (2.0 * 3.14159265358979323846);
+
+int potential_rec_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw, const double* ads,
+                       double a, double b, double c, double gam, int nmult, int mmult, int nx, int ny, int nslices,
+                       void* V, void* Vabs, int prec, int where) {
+    const double dz = c / nslices;
+    const double sg = std::sin(gam), cg = std::cos(gam);
+    struct Item {
+        long long kz;
+        int Z;
+        double dw, ads;
+        RecAtom at;
+    };
+    std::vector<Item> items;
+    items.reserve(natoms);
+    for (int64_t i = 0; i < natoms; i++) {
+        const msx_atom& at = atoms[i];
+        if (at.Z < 1 || at.Z > 92) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..92 (Doyle-Turner table)",
+                                               (long long)i, at.Z);
+        long long kz = (long long)std::floor(at.z / dz);
+        if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
+        if (kz < 0 || kz >= nslices) continue;
+        // fractional coordinates: a along x, b at angle gam from a in the xy plane
+        const double yf = at.y / (b * sg);
+        const double xf = (at.x - at.y * cg / sg) / a;
+        items.push_back(Item{kz, at.Z, dw ? dw[i] : 0.0, ads ? ads[i] : 0.0,
+                             RecAtom{xf - std::floor(xf), yf - std::floor(yf), (double)at.occ}});
+    }
+    std::sort(items.begin(), items.end(), [](const Item& p, const Item& q) {
+        if (p.kz != q.kz) return p.kz < q.kz;
+        if (p.Z != q.Z) return p.Z < q.Z;
+        if (p.dw != q.dw) return p.dw < q.dw;
+        return p.ads < q.ads;
+    });
+    std::vector<RecAtom> rat(std::max<size_t>(1, items.size()));
+    std::vector<RecSeg> segs;
+    std::vector<int> slice_seg(nslices + 1, 0);
+    for (size_t i = 0; i < items.size(); i++) {
+        rat[i] = items[i].at;
+        const bool same = !segs.empty() && i > 0 && items[i - 1].kz == items[i].kz && items[i - 1].Z == items[i].Z &&
+                          items[i - 1].dw == items[i].dw && items[i - 1].ads == items[i].ads;
+        if (same) {
+            segs.back().end = (int)i + 1;
+        } else {
+            segs.push_back(RecSeg{(int)i, (int)i + 1, items[i].Z, 0, items[i].dw, items[i].ads});
+            slice_seg[items[i].kz + 1]++;
+        }
+    }
+    for (int k = 0; k < nslices; k++) slice_seg[k + 1] += slice_seg[k];
+    if (segs.empty()) segs.push_back(RecSeg{0, 0, 1, 0, 0.0, 0.0});
+    CHK(ctx->atoms.ensure(sizeof(RecAtom) * rat.size() + sizeof(RecSeg) * segs.size()));
+    CHK(ctx->cells.ensure(sizeof(int) * slice_seg.size()));
+    char* abase = static_cast<char*>(ctx->atoms.p);
+    RecSeg* d_segs = reinterpret_cast<RecSeg*>(abase + sizeof(RecAtom) * rat.size());
+    HIPCHK(hipMemcpyAsync(abase, rat.data(), sizeof(RecAtom) * rat.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(d_segs, segs.data(), sizeof(RecSeg) * segs.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->cells.p, slice_seg.data(), sizeof(int) * slice_seg.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    // slices in chunks of at most 1 GiB of complex128 coefficients
+    const size_t img = (size_t)nx * ny;
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>(nslices, ((size_t)1 << 30) / (img * 16)));
+    CHK(ctx->wave2.ensure(img * 16 * chunk));
+    const size_t rb = prec == MSX_F64 ? 8 : 4;
+    if (where == MSX_HOST) CHK(ctx->vtmp.ensure(rb * img * chunk * (Vabs ? 2 : 1)));
+    RecArgs ra;
+    ra.atoms = reinterpret_cast<const RecAtom*>(abase);
+    ra.segs = d_segs;
+    ra.slice_seg = static_cast<const int*>(ctx->cells.p);
+    ra.out = static_cast<double2*>(ctx->wave2.p);
+    ra.A = 1.0 / (a * sg);
+    ra.B = 1.0 / (b * sg);
+    ra.cg = cg;
+    ra.inv_area = 1.0 / (a * b * sg);
+    ra.nx = nx;
+    ra.ny = ny;
+    ra.nmult = nmult;
+    ra.mmult = mmult;
+    for (int k0 = 0; k0 < nslices; k0 += chunk) {
+        const int nk = std::min(chunk, nslices - k0);
+        ra.kz0 = k0;
+        CHK(ctx->timed(MSX_PROF_SPLAT, (double)img * nk * 16, [&] { return launch_rec_sf(ra, nk, ctx->stream); }));
+        // V(r) = sum_g V(g) exp(+2 pi i g.r): the unscaled +i transform
+        CHK(fft2d_dev<double2>(ctx, ra.out, nk, nx, ny, +1));
+        char* vd = where == MSX_HOST ? static_cast<char*>(ctx->vtmp.p) : static_cast<char*>(V) + rb * img * k0;
+        char* va = nullptr;
+        if (Vabs) va = where == MSX_HOST ? vd + rb * img * nk : static_cast<char*>(Vabs) + rb * img * k0;
+        HIPCHK(launch_split_potential(ra.out, (long long)img * nk, kVA2, vd, va, prec == MSX_F64, ctx->stream));
+        if (where == MSX_HOST) {
+            HIPCHK(hipMemcpyAsync(static_cast<char*>(V) + rb * img * k0, vd, rb * img * nk, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            if (Vabs)
+                HIPCHK(hipMemcpyAsync(static_cast<char*>(Vabs) + rb * img * k0, va, rb * img * nk,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+int potential_rs_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw, const double* ads,
+                      double a, double b, double c, double gam, double atrad, int periodic, int nx, int ny,
+                      int nslices, void* V, void* Vabs, int prec, int where) {
+    const double dz = c / nslices;
+    const double sg = std::sin(gam), cg = std::cos(gam);
+    const double drax = atrad * sg / a, dray = atrad * sg / b;
+    const int ntx = (nx + 15) / 16, nty = (ny + 15) / 16;
+    const long long ntiles = (long long)nslices * ntx * nty;
+    std::vector<RsAtom> rat;
+    std::vector<int> akz;
+    rat.reserve(natoms);
+    for (int64_t i = 0; i < natoms; i++) {
+        const msx_atom& at = atoms[i];
+        if (at.Z < 1 || at.Z > 92) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..92 (Doyle-Turner table)",
+                                               (long long)i, at.Z);
+        long long kz = (long long)std::floor(at.z / dz);
+        if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
+        if (kz < 0 || kz >= nslices) continue;
+        const double yf0 = at.y / (b * sg), xf0 = (at.x - at.y * cg / sg) / a;
+        const double xf = xf0 - std::floor(xf0), yf = yf0 - std::floor(yf0);
+        // Pot_RS.cpp:131-134: C conversions truncate toward zero
+        const long ibb = (long)((xf - drax) * nx), iee = (long)((xf + drax) * nx);
+        const long jbb = (long)((yf - dray) * ny), jee = (long)((yf + dray) * ny);
+        if (ibb < -(1L << 30) || iee > (1L << 30) || jbb < -(1L << 30) || jee > (1L << 30))
+            return fail(MSX_EINVAL, "atrad too large for the grid");
+        rat.push_back(RsAtom{xf, yf, (double)at.occ, dw ? dw[i] : 0.0, ads ? ads[i] : 0.0, at.Z, (int)ibb, (int)iee,
+                             (int)jbb, (int)jee, 0});
+        akz.push_back((int)kz);
+    }
+    // (atom, image) entries per (slice, tile): count, then fill
+    std::vector<int> start(ntiles + 1, 0);
+    auto boxes = [&](size_t k, auto&& emit) {
+        const RsAtom& r = rat[k];
+        for (int l = 0; l < (periodic ? 9 : 1); l++) {
+            const int sx = (l == 3 || l == 5 || l == 8) ? 1 : ((l == 4 || l == 6 || l == 7) ? -1 : 0);
+            const int sy = (l == 1 || l == 5 || l == 7) ? 1 : ((l == 2 || l == 6 || l == 8) ? -1 : 0);
+            if ((sx == 1 && !(r.ibb < 0)) || (sx == -1 && !(r.iee >= nx)) || (sy == 1 && !(r.jbb < 0)) ||
+                (sy == -1 && !(r.jee >= ny)))
+                continue;
+            int ib = std::max(r.ibb, 0), ie = std::min(r.iee, nx), jb = std::max(r.jbb, 0), je = std::min(r.jee, ny);
+            if (sx == 1) { ib = r.ibb + nx; ie = nx; }
+            if (sx == -1) { ib = 0; ie = r.iee - nx + 1; }
+            if (sy == 1) { jb = r.jbb + ny; je = ny; }
+            if (sy == -1) { jb = 0; je = r.jee - ny + 1; }
+            ib = std::max(ib, 0);
+            jb = std::max(jb, 0);
+            ie = std::min(ie, nx);
+            je = std::min(je, ny);
+            if (ib >= ie || jb >= je) continue;
+            for (int tx = ib / 16; tx <= (ie - 1) / 16; tx++)
+                for (int ty = jb / 16; ty <= (je - 1) / 16; ty++)
+                    emit(((long long)akz[k] * ntx + tx) * nty + ty, l);
+        }
+    };
+    for (size_t k = 0; k < rat.size(); k++) boxes(k, [&](long long t, int) { start[t + 1]++; });
+    for (long long t = 0; t < ntiles; t++) start[t + 1] += start[t];
+    if (start[ntiles] > (1 << 28)) return fail(MSX_EINVAL, "%d box-tile entries: atrad too large", start[ntiles]);
+    std::vector<int2> ent(std::max(1, start[ntiles]));
+    {
+        std::vector<int> fill(start.begin(), start.end() - 1);
+        for (size_t k = 0; k < rat.size(); k++) boxes(k, [&](long long t, int l) { ent[fill[t]++] = make_int2((int)k, l); });
+    }
+    if (rat.empty()) rat.push_back(RsAtom{});
+    const size_t abytes = sizeof(RsAtom) * rat.size(), ebytes = sizeof(int2) * ent.size();
+    CHK(ctx->atoms.ensure(abytes + ebytes));
+    CHK(ctx->cells.ensure(sizeof(int) * start.size()));
+    char* base = static_cast<char*>(ctx->atoms.p);
+    HIPCHK(hipMemcpyAsync(base, rat.data(), abytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(base + abytes, ent.data(), ebytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->cells.p, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice, ctx->stream));
+    const size_t rb = prec == MSX_F64 ? 8 : 4;
+    const size_t vbytes = rb * (size_t)nslices * nx * ny;
+    void* vd = V;
+    void* vad = Vabs;
+    if (where == MSX_HOST) {
+        CHK(ctx->vtmp.ensure(vbytes * (Vabs ? 2 : 1)));
+        vd = ctx->vtmp.p;
+        vad = Vabs ? static_cast<char*>(ctx->vtmp.p) + vbytes : nullptr;
+    }
+    RsArgs ra;
+    std::memset(&ra, 0, sizeof ra);
+    ra.atoms = reinterpret_cast<const RsAtom*>(base);
+    ra.entries = reinterpret_cast<const int2*>(base + abytes);
+    ra.tile_start = static_cast<const int*>(ctx->cells.p);
+    if (prec == MSX_F64) {
+        ra.v64 = static_cast<double*>(vd);
+        ra.a64 = static_cast<double*>(vad);
+    } else {
+        ra.v32 = static_cast<float*>(vd);
+        ra.a32 = static_cast<float*>(vad);
+    }
+    ra.a = a;
+    ra.b = b;
+    ra.cg = cg;
+    ra.dxa = a / nx;
+    ra.dyb = b / ny;
+    ra.scale = kVA2;
+    ra.nx = nx;
+    ra.ny = ny;
+    ra.ntx = ntx;
+    ra.nty = nty;
+    ra.nslices = nslices;
+    CHK(ctx->timed(MSX_PROF_SPLAT, (double)vbytes, [&] { return launch_rs_dt(ra, ctx->stream); }));
+    if (where == MSX_HOST) {
+        HIPCHK(hipMemcpyAsync(V, vd, vbytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (Vabs) HIPCHK(hipMemcpyAsync(Vabs, vad, vbytes, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
 }  // namespace
 
 int msx_ctx_device(const msx_ctx_s* ctx) { return ctx->device; }
@@ -895,8 +1109,46 @@ MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, do
     return potential_impl(ctx, atoms, natoms, a, b, c, nx, ny, nslices, radius, table, formula, V, prec, where);
 }
 
+MSX_API int msx_potential_rec(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw,
+                              const double* ads, double a, double b, double c, double gam, int nmult, int mmult,
+                              int nx, int ny, int nslices, void* V, void* Vabs, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_grid(nx, ny));
+    if (nslices <= 0) return fail(MSX_EINVAL, "nslices must be > 0");
+    if (!(a > 0) || !(b > 0) || !(c > 0)) return fail(MSX_EINVAL, "a, b, c must be > 0");
+    if (!(gam > 0.0 && gam < 3.14159265358979323846)) return fail(MSX_EINVAL, "gam must be in (0, pi)");
+    if (nmult < 1 || mmult < 1) return fail(MSX_EINVAL, "nmult and mmult must be >= 1");
+    if (natoms < 0 || (natoms > 0 && !atoms) || !V) return fail(MSX_EINVAL, "bad atoms/V");
+    if (natoms > (1LL << 30)) return fail(MSX_EINVAL, "too many atoms");
+    HIPCHK(hipSetDevice(ctx->device));
+    return potential_rec_impl(ctx, atoms, natoms, dw, ads, a, b, c, gam, nmult, mmult, nx, ny, nslices, V, Vabs, prec,
+                              where);
+}
+
+MSX_API int msx_potential_rs(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw,
+                             const double* ads, double a, double b, double c, double gam, double atrad, int periodic,
+                             int nx, int ny, int nslices, void* V, void* Vabs, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    if (nx < 1 || ny < 1 || nx > 16384 || ny > 16384 || nslices <= 0)
+        return fail(MSX_EINVAL, "bad grid %d x %d x %d", nx, ny, nslices);
+    if (!(a > 0) || !(b > 0) || !(c > 0) || !(atrad > 0)) return fail(MSX_EINVAL, "a, b, c, atrad must be > 0");
+    if (!(gam > 0.0 && gam < 3.14159265358979323846)) return fail(MSX_EINVAL, "gam must be in (0, pi)");
+    if (natoms < 0 || (natoms > 0 && !atoms) || !V) return fail(MSX_EINVAL, "bad atoms/V");
+    if (natoms > (1LL << 30)) return fail(MSX_EINVAL, "too many atoms");
+    HIPCHK(hipSetDevice(ctx->device));
+    return potential_rs_impl(ctx, atoms, natoms, dw, ads, a, b, c, gam, atrad, periodic, nx, ny, nslices, V, Vabs,
+                             prec, where);
+}
+
 MSX_API int msx_transmission(msx_ctx ctx, const void* V, int nlayer, int nx, int ny, double ax, double by, double kev,
                              double bw, void* t, int prec, int where) {
+    return msx_transmission_abs(ctx, V, nullptr, nlayer, nx, ny, ax, by, kev, bw, t, prec, where);
+}
+
+MSX_API int msx_transmission_abs(msx_ctx ctx, const void* V, const void* Vabs, int nlayer, int nx, int ny, double ax,
+                                 double by, double kev, double bw, void* t, int prec, int where) {
     if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
     CHK(check_prec_where(prec, where));
     CHK(check_grid(nx, ny));
@@ -905,20 +1157,25 @@ MSX_API int msx_transmission(msx_ctx ctx, const void* V, int nlayer, int nx, int
     const size_t n = (size_t)nx * ny * nlayer;
     const size_t vb = (prec == MSX_F64 ? 8 : 4) * n, tb = (prec == MSX_F64 ? 16 : 8) * n;
     const void* v_dev = V;
+    const void* a_dev = Vabs;
     void* t_dev = t;
     if (where == MSX_HOST) {
-        CHK(ctx->vtmp.ensure(vb));
+        CHK(ctx->vtmp.ensure(vb * (Vabs ? 2 : 1)));
         CHK(ctx->trans.ensure(tb));
         HIPCHK(hipMemcpyAsync(ctx->vtmp.p, V, vb, hipMemcpyHostToDevice, ctx->stream));
         v_dev = ctx->vtmp.p;
+        if (Vabs) {
+            a_dev = static_cast<char*>(ctx->vtmp.p) + vb;
+            HIPCHK(hipMemcpyAsync(const_cast<void*>(a_dev), Vabs, vb, hipMemcpyHostToDevice, ctx->stream));
+        }
         t_dev = ctx->trans.p;
     }
     if (prec == MSX_F64)
         CHK(transmission_dev<double2>(ctx, static_cast<const double*>(v_dev), nlayer, nx, ny, ax, by, kev, bw,
-                                      static_cast<double2*>(t_dev)));
+                                      static_cast<double2*>(t_dev), static_cast<const double*>(a_dev)));
     else
         CHK(transmission_dev<float2>(ctx, static_cast<const float*>(v_dev), nlayer, nx, ny, ax, by, kev, bw,
-                                     static_cast<float2*>(t_dev)));
+                                     static_cast<float2*>(t_dev), static_cast<const float*>(a_dev)));
     if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(t, t_dev, tb, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MSX_OK;

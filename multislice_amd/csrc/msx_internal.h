@@ -92,6 +92,46 @@ struct AtomIO {
 hipError_t launch_displace(const AtomIO* in, long long n, const double* u_rms, unsigned long long seed, int config,
                            AtomIO* out, hipStream_t s);
 
+// reciprocal-space potential (msx_potential.hip)
+struct RecAtom {
+    double x, y;  // fractional coordinates in the unit cell
+    double occ;
+};
+struct RecSeg {            // atoms [start, end) of one slice sharing (Z, B, ads)
+    int start, end, Z, pad;
+    double dw, ads;
+};
+struct RecArgs {
+    const RecAtom* atoms;
+    const RecSeg* segs;
+    const int* slice_seg;  // segments of slice kz: [slice_seg[kz], slice_seg[kz+1])
+    double2* out;          // V(g) of slices kz0 .. kz0+nslices-1: [slice][nx][ny]
+    double A, B, cg, inv_area;  // reciprocal lengths 1/(a sin gam), 1/(b sin gam); cos gam; 1/area
+    int nx, ny, nmult, mmult, kz0;
+};
+hipError_t launch_rec_sf(const RecArgs& a, int nslices, hipStream_t s);
+hipError_t launch_split_potential(const double2* w, long long n, double scale, void* V, void* Vabs, bool f64,
+                                  hipStream_t s);
+
+// real-space Doyle-Turner potential over pixel boxes (msx_potential.hip)
+struct RsAtom {
+    double xf, yf, occ, dw, ads;   // fractional position, occupancy, Debye-Waller B, absorptive fraction
+    int Z, ibb, iee, jbb, jee;     // box: pixel index bounds before clipping
+    int pad;
+};
+struct RsArgs {
+    const RsAtom* atoms;
+    const int2* entries;       // (atom, image 0..8) per tile, grouped by (slice, tile)
+    const int* tile_start;     // [nslices * ntx * nty + 1]
+    double* v64;               // outputs [nslices][nx][ny] (one of v64 / v32), scaled by `scale`
+    float* v32;
+    double* a64;               // absorptive part (optional)
+    float* a32;
+    double a, b, cg, dxa, dyb, scale;
+    int nx, ny, ntx, nty, nslices;
+};
+hipError_t launch_rs_dt(const RsArgs& a, hipStream_t s);
+
 // detector finish: out[d][p] += w * sum_g partial[d][b][g] (p = probe0 + b)
 hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups, double w, double* out,
                              long long out_stride_d, long long probe0, hipStream_t s);

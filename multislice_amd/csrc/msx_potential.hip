@@ -15,13 +15,18 @@
 #include "msx_internal.h"
 #include "msx_philox.h"
 #include "fparams_tables.inc"
+#include "doyle_turner.inc"
 
 namespace msx {
 
 __constant__ double c_fparams[2][103][12];
+__constant__ double c_dt[93][8];
 
 hipError_t upload_fparams(hipStream_t s) {
-    return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_fparams), MSX_FPARAMS, sizeof(MSX_FPARAMS), 0,
+    hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_fparams), MSX_FPARAMS, sizeof(MSX_FPARAMS), 0,
+                                          hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpyToSymbolAsync(HIP_SYMBOL(c_dt), MSX_DOYLE_TURNER, sizeof(MSX_DOYLE_TURNER), 0,
                                   hipMemcpyHostToDevice, s);
 }
 
@@ -150,6 +155,168 @@ hipError_t launch_displace(const AtomIO* in, long long n, const double* u_rms, u
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(displace_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in, n, u_rms, seed,
                        config, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Reciprocal-space potential (Muslib Potential_rec_DT_DW,
+// Pot_rec_DT_DW.cpp:3-132): Fourier coefficients of one slice,
+//   V(g) = sum_species F_s(g) sum_{atoms of s} occ exp(-2 pi i (h x_f + k y_f)) (1 + i ads_s),
+//   F_s(g) = sum_j a_j exp(-(b_j + B_s) |g|^2/4) / area   (Doyle-Turner, Debye-Waller B),
+// at the reflections whose grid index h*nmult (k*mmult) is below n/2 in
+// magnitude; every other coefficient (Nyquist, non-multiples) is 0.
+// One 16x16 tile of reflections per block.  The phase factor of an atom is
+// separable, exp(-2 pi i h x_f) exp(-2 pi i k y_f): per chunk of 8 atoms the
+// block evaluates the 16 + 16 factors of its tile row and column once (one
+// FP64 sincospi per thread, argument reduced to [-1/2, 1/2]) into LDS, and
+// each thread forms the 8 products for its reflection (4 FP64 FMA each).
+__device__ __forceinline__ bool refl_index(int i, int n, int mult, int* h) {
+    const int half = n / 2;
+    if (i < half) {
+        *h = i / mult;
+        return i % mult == 0;
+    }
+    const int j = n - i;
+    *h = -(j / mult);
+    return i > half && j % mult == 0;
+}
+
+__global__ __launch_bounds__(256) void rec_sf_kernel(RecArgs p) {
+    __shared__ double2 ph[8][32];
+    const int ty = threadIdx.x & 15, tx = threadIdx.x >> 4;
+    const int iy = blockIdx.x * 16 + ty, ix = blockIdx.y * 16 + tx;
+    const int kz = p.kz0 + (int)blockIdx.z;
+    int h, k;
+    const bool ok = refl_index(ix, p.nx, p.nmult, &h) & refl_index(iy, p.ny, p.mmult, &k);
+    const double gx = h * p.A, gy = k * p.B;
+    const double s2 = 0.25 * (gx * gx + gy * gy - 2.0 * gx * gy * p.cg);
+    // this thread's phase-factor slot: atom (tid / 32) of the chunk, column
+    // (tid % 32): 0..15 = h of tile column, 16..31 = k of tile row
+    const int slot_a = threadIdx.x >> 5, slot_c = threadIdx.x & 31;
+    int slot_q;
+    if (slot_c < 16) refl_index(blockIdx.y * 16 + slot_c, p.nx, p.nmult, &slot_q);
+    else refl_index(blockIdx.x * 16 + slot_c - 16, p.ny, p.mmult, &slot_q);
+    double vr = 0.0, vi = 0.0;
+    for (int sg = p.slice_seg[kz]; sg < p.slice_seg[kz + 1]; ++sg) {
+        const RecSeg seg = p.segs[sg];
+        double sr = 0.0, si = 0.0;
+        for (int a0 = seg.start; a0 < seg.end; a0 += 8) {
+            const int ai = a0 + slot_a;
+            if (ai < seg.end) {
+                const RecAtom at = p.atoms[ai];
+                double arg = (double)slot_q * (slot_c < 16 ? at.x : at.y);
+                arg -= rint(arg);
+                double sn, cs;
+                sincospi(-2.0 * arg, &sn, &cs);
+                if (slot_c < 16) {
+                    sn *= at.occ;
+                    cs *= at.occ;
+                }
+                ph[slot_a][slot_c] = make_double2(cs, sn);
+            }
+            __syncthreads();
+            const int na = min(8, seg.end - a0);
+            for (int j = 0; j < na; ++j) {
+                const double2 a = ph[j][tx], b = ph[j][16 + ty];
+                sr = fma(a.x, b.x, fma(-a.y, b.y, sr));
+                si = fma(a.x, b.y, fma(a.y, b.x, si));
+            }
+            __syncthreads();
+        }
+        const double* dt = c_dt[seg.Z];
+        double f = 0.0;
+        for (int j = 0; j < 4; ++j) f += dt[2 * j] * exp(-(dt[2 * j + 1] + seg.dw) * s2);
+        f *= p.inv_area;
+        vr += f * (sr - seg.ads * si);
+        vi += f * (si + seg.ads * sr);
+    }
+    if (!ok) vr = vi = 0.0;
+    p.out[((long long)blockIdx.z * p.nx + ix) * p.ny + iy] = make_double2(vr, vi);
+}
+
+hipError_t launch_rec_sf(const RecArgs& a, int nslices, hipStream_t s) {
+    dim3 grid(a.ny / 16, a.nx / 16, nslices);
+    hipLaunchKernelGGL(rec_sf_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// V = scale * Re(w), Vabs = scale * Im(w) for n values
+template <typename Re>
+__global__ void split_potential_kernel(const double2* w, long long n, double scale, Re* V, Re* Vabs) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 v = w[i];
+    V[i] = (Re)(scale * v.x);
+    if (Vabs) Vabs[i] = (Re)(scale * v.y);
+}
+hipError_t launch_split_potential(const double2* w, long long n, double scale, void* V, void* Vabs, bool f64,
+                                  hipStream_t s) {
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (f64)
+        hipLaunchKernelGGL(split_potential_kernel<double>, g, dim3(256), 0, s, w, n, scale, static_cast<double*>(V),
+                           static_cast<double*>(Vabs));
+    else
+        hipLaunchKernelGGL(split_potential_kernel<float>, g, dim3(256), 0, s, w, n, scale, static_cast<float*>(V),
+                           static_cast<float*>(Vabs));
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------
+// Real-space Doyle-Turner potential (Muslib Potential_rs_DT_DW_cutpot and
+// _periodic, Pot_RS.cpp:111-155, 350-486): every pixel of an atom's box (or of
+// a wrapped piece of it, image l) gets occ * 4 pi sum_j a_j/(b_j+B)
+// exp(-4 pi^2 r^2/(b_j+B)) (Curves.cpp:23-34), r^2 = x(x + 2y cos gam) + y^2
+// to the atom image.  The host lists, per 16x16 pixel tile, the (atom, image)
+// pairs whose box meets the tile; each thread tests its pixel against the
+// exact integer box of the pair.
+__device__ __forceinline__ double dt_rs2(int Z, double r2, double dw) {
+    const double* p = c_dt[Z];
+    double f = 0.0;
+    for (int j = 0; j < 4; ++j)
+        if (p[2 * j + 1] > 0) f += p[2 * j] / (p[2 * j + 1] + dw) * exp(-39.47841760435743 * r2 / (p[2 * j + 1] + dw));
+    return 12.566370614359172 * f;
+}
+
+__global__ __launch_bounds__(256) void rs_dt_kernel(RsArgs p) {
+    const int iy = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int ix = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int kz = blockIdx.z;
+    const int tile = (kz * p.ntx + (int)blockIdx.y) * p.nty + (int)blockIdx.x;
+    double v = 0.0, va = 0.0;
+    for (int e = p.tile_start[tile]; e < p.tile_start[tile + 1]; ++e) {
+        const int2 en = p.entries[e];
+        const RsAtom at = p.atoms[en.x];
+        const int l = en.y;
+        // image l: x part 0 main, +1 wrapped from the low edge (atom at +a),
+        // -1 wrapped from the high edge (atom at -a); same for y
+        const int sx = (l == 3 || l == 5 || l == 8) ? 1 : ((l == 4 || l == 6 || l == 7) ? -1 : 0);
+        const int sy = (l == 1 || l == 5 || l == 7) ? 1 : ((l == 2 || l == 6 || l == 8) ? -1 : 0);
+        int ib = max(at.ibb, 0), ie = min(at.iee, p.nx), jb = max(at.jbb, 0), je = min(at.jee, p.ny);
+        double x0 = at.xf * p.a, y0 = at.yf * p.b;
+        if (sx == 1) { ib = at.ibb + p.nx; ie = p.nx; x0 += p.a; }
+        if (sx == -1) { ib = 0; ie = at.iee - p.nx + 1; x0 -= p.a; }
+        if (sy == 1) { jb = at.jbb + p.ny; je = p.ny; y0 += p.b; }
+        if (sy == -1) { jb = 0; je = at.jee - p.ny + 1; y0 -= p.b; }
+        if (ix >= ib && ix < ie && iy >= jb && iy < je) {
+            const double x = ix * p.dxa - x0, y = iy * p.dyb - y0;
+            const double r2 = x * (x + 2.0 * y * p.cg) + y * y;
+            const double res = dt_rs2(at.Z, r2, at.dw) * at.occ;
+            v += res;
+            va += res * at.ads;
+        }
+    }
+    if (ix < p.nx && iy < p.ny) {
+        const long long o = ((long long)kz * p.nx + ix) * p.ny + iy;
+        if (p.v64) p.v64[o] = p.scale * v;
+        if (p.v32) p.v32[o] = (float)(p.scale * v);
+        if (p.a64) p.a64[o] = p.scale * va;
+        if (p.a32) p.a32[o] = (float)(p.scale * va);
+    }
+}
+
+hipError_t launch_rs_dt(const RsArgs& a, hipStream_t s) {
+    dim3 grid(a.nty, a.ntx, a.nslices);
+    hipLaunchKernelGGL(rs_dt_kernel, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

@@ -53,6 +53,7 @@ template <typename C> struct RowArgs {
     const C* tw;            // twiddles of length N
     const C* trans;         // transmission [nx][N] shared by all waves, or nullptr
     const Re* vreal;        // real potential [nb][nx][N] (replaces the input), or nullptr
+    const Re* vabs;         // optional absorptive potential, same layout: exp(-vscale*vabs) damping
     Re vscale;              // phase per unit of vreal (sigma/1000)
     Re scale;               // multiplies the data after the IN transform
     Re* inten;              // optional |psi|^2 output [nb][nx][N] (after the pointwise stage)
@@ -116,6 +117,14 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
                 Re s, c;
                 if constexpr (sizeof(Re) == 4) sincosf(p.vscale * vr[t + e * T], &s, &c);
                 else sincos(p.vscale * vr[t + e * T], &s, &c);
+                if (p.vabs) {
+                    const Re* va = p.vabs + ((long long)b * p.nx + ix) * N;
+                    Re d;
+                    if constexpr (sizeof(Re) == 4) d = expf(-p.vscale * va[t + e * T]);
+                    else d = exp(-p.vscale * va[t + e * T]);
+                    c *= d;
+                    s *= d;
+                }
                 v[e] = mk<C>(c, s);
             }
         } else {

@@ -74,7 +74,7 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_last_error", "msx
            "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image",
            "msx_unet_create", "msx_unet_destroy", "msx_unet_param_count", "msx_unet_params",
            "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
-           "msx_unet_conv"]
+           "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs"]
 
 _lib = None
 _lock = threading.Lock()
@@ -132,6 +132,9 @@ def load():
         "msx_unet_gradient": (i, [vp, vp, vp, i, i, i, C.POINTER(d), vp, i]),
         "msx_unet_train_step": (i, [vp, vp, vp, i, i, i, C.c_float, C.POINTER(d), i]),
         "msx_unet_conv": (i, [vp, i, i, vp, vp, vp, vp, vp, vp, i, i, i, i, i, i]),
+        "msx_potential_rec": (i, [vp, vp, C.c_int64, vp, vp, d, d, d, d, i, i, i, i, i, vp, vp, i, i]),
+        "msx_transmission_abs": (i, [vp, vp, vp, i, i, i, d, d, d, d, vp, i, i]),
+        "msx_potential_rs": (i, [vp, vp, C.c_int64, vp, vp, d, d, d, d, d, i, i, i, i, vp, vp, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -242,7 +245,43 @@ class Context:
                                     where))
         return out
 
-    def transmission(self, V, ax, by, kev, bw=2.0 / 3.0, out=None):
+    def potential_rs(self, atoms, cell, nx, ny, nslices, atrad, periodic=True, dw=None, ads=None,
+                     gamma=np.pi / 2, prec=F32, out=None, out_abs=None, absorptive=False):
+        """Real-space Doyle-Turner potential over pixel boxes, msx_potential_rs.
+        Returns V (and Vabs when absorptive=True), [nslices][nx][ny] in V*A."""
+        return self._dt_potential("rs", atoms, cell, nx, ny, nslices, dw, ads, gamma, (float(atrad), int(periodic)),
+                                  prec, out, out_abs, absorptive)
+
+    def _dt_potential(self, kind, atoms, cell, nx, ny, nslices, dw, ads, gamma, extra, prec, out, out_abs,
+                      absorptive):
+        at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        n = len(at)
+        arr = lambda v: None if v is None else np.ascontiguousarray(np.broadcast_to(v, (n,)), np.float64)  # noqa
+        dwa, ada = arr(dw), arr(ads)
+        dt = np.float64 if prec == F64 else np.float32
+        if out is None:
+            out = np.zeros((nslices, nx, ny), dt)
+        if absorptive and out_abs is None:
+            out_abs = np.zeros((nslices, nx, ny), dt) if not _is_torch(out) else out.new_empty(out.shape)
+        pv = _buf(out, prec, complex_=False, writable=True)
+        pa = _buf(out_abs, prec, complex_=False, writable=True) if out_abs is not None else (None, pv[1], None, None)
+        where = _ptr_where([pv, pa])
+        args = [self.h, C.c_void_p(at.ctypes.data), n, None if dwa is None else C.c_void_p(dwa.ctypes.data),
+                None if ada is None else C.c_void_p(ada.ctypes.data), float(cell[0]), float(cell[1]), float(cell[2]),
+                float(gamma), *extra, nx, ny, nslices, pv[0], pa[0], prec, where]
+        fn = load().msx_potential_rs if kind == "rs" else load().msx_potential_rec
+        _check(fn(*args))
+        return (out, out_abs) if out_abs is not None else out
+
+    def potential_rec(self, atoms, cell, nx, ny, nslices, dw=None, ads=None, gamma=np.pi / 2, nmult=1, mmult=1,
+                      prec=F32, out=None, out_abs=None, absorptive=False):
+        """Reciprocal-space (Doyle-Turner structure factor) potential, msx_potential_rec.
+        cell = (a, b, c) of the unit cell; the grid spans (nmult*a, mmult*b).
+        Returns V (and Vabs when absorptive=True), [nslices][nx][ny] in V*A."""
+        return self._dt_potential("rec", atoms, cell, nx, ny, nslices, dw, ads, gamma, (int(nmult), int(mmult)),
+                                  prec, out, out_abs, absorptive)
+
+    def transmission(self, V, ax, by, kev, bw=2.0 / 3.0, out=None, Vabs=None):
         """exp(i sigma V / 1000), band-limited; V: (nlayer, nx, ny) real."""
         prec = _prec_of(V)
         nl, nx, ny = V.shape
@@ -255,8 +294,14 @@ class Context:
             else:
                 out = np.zeros((nl, nx, ny), np.complex128 if prec == F64 else np.complex64)
         pt = _buf(out, prec, writable=True)
-        where = _ptr_where([pv, pt])
-        _check(load().msx_transmission(self.h, pv[0], nl, nx, ny, ax, by, kev, bw, pt[0], prec, where))
+        bufs = [pv, pt]
+        pa = None
+        if Vabs is not None:
+            pa = _buf(Vabs, prec, complex_=False)
+            bufs.append(pa)
+        where = _ptr_where(bufs)
+        _check(load().msx_transmission_abs(self.h, pv[0], None if pa is None else pa[0], nl, nx, ny, ax, by, kev, bw,
+                                           pt[0], prec, where))
         return out
 
     @staticmethod
@@ -288,13 +333,13 @@ class Context:
         return psi
 
     def stem(self, trans, cz, layer, ax, by, kev, probe, positions, detectors, checkpoints,
-             bw=2.0 / 3.0, batch=0, weight=1.0, out=None):
+             bw=2.0 / 3.0, batch=0, weight=1.0, out=None, tilt=(0.0, 0.0)):
         """STEM scan; returns out[ck][det][probe] (fraction of incident intensity).
         probe: dict(alpha_max, alpha_min=0, cs=0, df=0) (rad, Angstrom);
-        detectors: [(k_inner, k_outer)] in 1/Angstrom."""
+        detectors: [(k_inner, k_outer)] in 1/Angstrom; tilt: crystal tilt (rad)."""
         prec = _prec_of(trans)
         nl, nx, ny = trans.shape
-        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, (0.0, 0.0), bw)
+        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
         pt = _buf(trans, prec)
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 2)
         det = np.ascontiguousarray(detectors, np.float64).reshape(-1, 2)

@@ -38,6 +38,18 @@
  *   probe / detector: Parseval normalisation, integer shift = circular
  *       shift, annulus partition sums to the total.
  *   ctf: closed-form chi, aperture mask.
+ *   potential_rec (Doyle-Turner, reciprocal space): one atom at the origin
+ *       gives the real radial f_e(|g|/2) exp(-B|g|^2/4)/area at every
+ *       reflection; translation theorem; Hermitian symmetry without
+ *       absorption; the Fourier sum equals the closed-form real-space
+ *       Gaussian potential of Curves.cpp:23-34 summed over periodic images;
+ *       table rows against the reference's Curves.cpp bank.
+ *   potential_rs (Doyle-Turner, real-space boxes): brute force over all
+ *       pixels with the box rule written as explicit interval tests; a box
+ *       inside the cell gives the same sum in cut and periodic forms; the
+ *       periodic form of a box crossing an edge equals the cut form of the
+ *       shifted atom plus its wrapped remainder; integral of one atom's
+ *       potential = 4 pi^2 sum_j a_j (DT forward scattering f_e(0) / pi).
  */
 #include <cmath>
 #include <complex>
@@ -47,6 +59,7 @@
 #include <vector>
 
 #include "../multislice_amd/csrc/fparams_tables.inc"
+#include "../multislice_amd/csrc/doyle_turner.inc"
 
 #define ORC_API extern "C" __attribute__((visibility("default")))
 
@@ -758,4 +771,119 @@ ORC_API void orc_ctf_image_f64(const double *wave, int nx, int ny, double ax, do
     if (img_wave) std::memcpy(img_wave, w.data(), sizeof(double) * 2 * nx * ny);
     if (image)
         for (size_t j = 0; j < (size_t)nx * ny; j++) image[j] = std::norm(w[j]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Reciprocal-space potential (Muslib)                                 */
+/* ------------------------------------------------------------------ */
+/* scat_ampl_rec_DT_DW (Curves.cpp:3-11): f_e(s) = sum_j a_j exp(-b_j s^2) */
+ORC_API double orc_dt_fe(int Z, double s) {
+    const double *p = MSX_DOYLE_TURNER[Z];
+    double f = 0.0;
+    for (int j = 0; j < 4; j++) f += p[2 * j] * std::exp(-p[2 * j + 1] * s * s);
+    return f;
+}
+/* real-space counterpart, scat_ampl_rs_DT_DW (Curves.cpp:12-21) at distance r:
+ * 4 pi sum_j a_j/(b_j+dw) exp(-4 pi^2 r^2/(b_j+dw)) (terms with b_j = 0 skipped) */
+ORC_API double orc_dt_rs(int Z, double r, double dw) {
+    const double *p = MSX_DOYLE_TURNER[Z];
+    double f = 0.0;
+    for (int j = 0; j < 4; j++)
+        if (p[2 * j + 1] > 0) f += p[2 * j] / (p[2 * j + 1] + dw) * std::exp(-4.0 * M_PI * M_PI * r * r / (p[2 * j + 1] + dw));
+    return 4.0 * M_PI * f;
+}
+
+/* Potential_rec_DT_DW (Pot_rec_DT_DW.cpp:3-132), whole slice (SL = 0, EL = M):
+ * adds to gli, in the reference's layout (row jj of N interleaved complex
+ * values, x fastest), the Fourier coefficients
+ *   V(g) += f_e(|g|/2) exp(-dw |g|^2/4) occ / area * exp(-2 pi i (h xx + k yy)) * (1 + i ads)
+ * for every reflection (h, k) with |h|*nmult < N/2 and |k|*mmult < M/2 (integer
+ * halves), stored at column h*nmult mod N and row k*mmult mod M; reciprocal
+ * metric of the oblique cell |g|^2 = (hA)^2 + (kB)^2 - 2hkAB cos(gam),
+ * A = 1/(a sin gam), B = 1/(b sin gam), area = a b sin gam (:6-12, :30-35,
+ * :69-84).  xx, yy are fractional coordinates.  The reference enumerates the
+ * four sign quadrants explicitly; this loop visits the same set of signed
+ * reflections. */
+ORC_API void orc_potential_rec_dt(double a, double b, double gam, int nmult, int mmult, int N, int M, long na,
+                                  const double *xx, const double *yy, const double *dw, const double *ads,
+                                  const double *occ, const int *Z, double *gli) {
+    const double A = 1.0 / (a * std::sin(gam)), B = 1.0 / (b * std::sin(gam));
+    const double inv_area = 1.0 / (a * b * std::sin(gam));
+    const double cg = std::cos(gam);
+    const int hmax = (N / 2 - 1) / nmult, kmax = (M / 2 - 1) / mmult;  /* largest |h| with |h|*nmult < N/2 */
+    for (long t = 0; t < na; t++) {
+        for (int k = -kmax; k <= kmax; k++) {
+            const int row = ((k * mmult) % M + M) % M;
+            for (int h = -hmax; h <= hmax; h++) {
+                const int col = ((h * nmult) % N + N) % N;
+                const double x = h * A, y = k * B;
+                const double g2 = x * x + y * y - 2.0 * x * y * cg;
+                const double ss = 0.25 * g2;
+                const double res = orc_dt_fe(Z[t], std::sqrt(ss)) * std::exp(-dw[t] * ss) * inv_area * occ[t];
+                const double ph = -2.0 * M_PI * (h * xx[t] + k * yy[t]);
+                const double rr = std::cos(ph) * res, ri = std::sin(ph) * res;
+                double *o = gli + 2 * ((size_t)row * N + col);
+                o[0] += rr - ads[t] * ri;
+                o[1] += ri + ads[t] * rr;
+            }
+        }
+    }
+}
+
+/* scat_ampl_rs2_DT_DW (Curves.cpp:23-34), argument r^2 */
+ORC_API double orc_dt_rs2(int Z, double r2, double dw) {
+    const double *p = MSX_DOYLE_TURNER[Z];
+    double f = 0.0;
+    for (int j = 0; j < 4; j++)
+        if (p[2 * j + 1] > 0) f += p[2 * j] / (p[2 * j + 1] + dw) * std::exp(-4.0 * M_PI * M_PI * r2 / (p[2 * j + 1] + dw));
+    return 4.0 * M_PI * f;
+}
+
+/* Potential_rs_DT_DW_cutpot (Pot_RS.cpp:111-155) and Potential_rs_DT_DW_periodic
+ * (Pot_RS.cpp:350-486): adds occ * scat_ampl_rs2_DT_DW(Z, r^2, dw) (and times
+ * ads to the imaginary part) over a box of pixels around every atom.  Box of
+ * atom (xx, yy) (fractional): ibb = (long)((xx - drax) N), iee = (long)((xx +
+ * drax) N) with drax = atrad sin(gam)/a (C truncation toward zero), the same
+ * along y; pixels i in [max(ibb,0), min(iee,N)), j likewise, distance to the
+ * atom at (xx a, yy b) with r^2 = x (x + 2 y cos gam) + y^2.  The periodic form
+ * adds the eight wrapped pieces of the box with the atom image shifted by a
+ * and/or b; a piece wrapped from the far edge keeps one more pixel (je =
+ * jee - M + 1, ie = iee - N + 1).  Layout: row j of N interleaved complex. */
+ORC_API void orc_potential_rs_dt(double a, double b, double gam, double atrad, int N, int M, long na,
+                                 const double *xx, const double *yy, const double *dw, const double *ads,
+                                 const double *occ, const int *Z, int periodic, double *gli) {
+    const double sg = std::sin(gam), cg = std::cos(gam);
+    const double drax = atrad * sg / a, dray = atrad * sg / b;
+    const double dxa = a / (double)N, dyb = b / (double)M;
+    for (long k = 0; k < na; k++) {
+        const long ibb = (long)((xx[k] - drax) * N), iee = (long)((xx[k] + drax) * N);
+        const long jbb = (long)((yy[k] - dray) * M), jee = (long)((yy[k] + dray) * M);
+        for (int l = 0; l < (periodic ? 9 : 1); l++) {
+            long ib = std::max(ibb, 0L), ie = std::min(iee, (long)N);
+            long jb = std::max(jbb, 0L), je = std::min(jee, (long)M);
+            double x0 = xx[k] * a, y0 = yy[k] * b;
+            const bool lo_i = ibb < 0, hi_i = iee >= N, lo_j = jbb < 0, hi_j = jee >= M;
+            /* x part: 0 = main, +1 = wrapped from the left (atom image at +a),
+             * -1 = wrapped from the right (image at -a); same for y */
+            static const int sx[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
+            static const int sy[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+            if ((sx[l] == 1 && !lo_i) || (sx[l] == -1 && !hi_i) || (sy[l] == 1 && !lo_j) || (sy[l] == -1 && !hi_j))
+                continue;
+            if (sx[l] == 1) { ib = ibb + N; ie = N; x0 += a; }
+            if (sx[l] == -1) { ib = 0; ie = iee - N + 1; x0 -= a; }
+            if (sy[l] == 1) { jb = jbb + M; je = M; y0 += b; }
+            if (sy[l] == -1) { jb = 0; je = jee - M + 1; y0 -= b; }
+            for (long j = jb; j < je; j++) {
+                const double y = j * dyb - y0;
+                for (long i = ib; i < ie; i++) {
+                    const double x = i * dxa - x0;
+                    const double r2 = x * (x + 2.0 * y * cg) + y * y;
+                    const double res = orc_dt_rs2(Z[k], r2, dw[k]) * occ[k];
+                    double *o = gli + 2 * ((size_t)j * N + i);
+                    o[0] += res;
+                    o[1] += res * ads[k];
+                }
+            }
+        }
+    }
 }

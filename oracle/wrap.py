@@ -48,6 +48,10 @@ def lib():
             ("orc_stem_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
             ("orc_stem_f32", i, [_fp, i, _fp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
             ("orc_ctf_image_f64", None, [_dp, i, i, d, d, d, d, d, d, C.c_void_p, C.c_void_p]),
+            ("orc_dt_fe", d, [i, d]), ("orc_dt_rs", d, [i, d, d]),
+            ("orc_potential_rec_dt", None, [d, d, d, i, i, i, i, l, _dp, _dp, _dp, _dp, _dp, _ip, _dp]),
+            ("orc_dt_rs2", d, [i, d, d]),
+            ("orc_potential_rs_dt", None, [d, d, d, d, i, i, l, _dp, _dp, _dp, _dp, _dp, _ip, i, _dp]),
         ]:
             fn = getattr(L, name)
             fn.restype = res
@@ -254,3 +258,34 @@ class Oracle:
         lib().orc_ctf_image_f64(w.view(np.float64).reshape(-1), nx, ny, ax, by, kev, cs, df, aobj,
                                 img_w.ctypes.data, img.ctypes.data)
         return img_w, img
+
+    @staticmethod
+    def dt_fe(Z, s):
+        return lib().orc_dt_fe(int(Z), float(s))
+
+    @staticmethod
+    def dt_rs(Z, r, dw=0.0):
+        return lib().orc_dt_rs(int(Z), float(r), float(dw))
+
+    @staticmethod
+    def potential_rec_g(Z, xf, yf, a, b, N, M, gam=np.pi / 2, nmult=1, mmult=1, dw=None, ads=None, occ=None):
+        """Potential_rec_DT_DW Fourier coefficients of one slice in the
+        reference's layout: complex array [M][N] (x fastest).  xf, yf are
+        fractional coordinates."""
+        n = len(Z)
+        z = lambda v, d=0.0: np.ascontiguousarray(np.full(n, d) if v is None else v, np.float64)  # noqa: E731
+        g = np.zeros((M, N), np.complex128)
+        lib().orc_potential_rec_dt(a, b, gam, nmult, mmult, N, M, n, z(xf), z(yf), z(dw), z(ads), z(occ, 1.0),
+                                   np.ascontiguousarray(Z, np.int32), g.view(np.float64).reshape(-1))
+        return g
+
+    @staticmethod
+    def potential_rs_g(Z, xf, yf, a, b, N, M, atrad, periodic=True, gam=np.pi / 2, dw=None, ads=None, occ=None):
+        """Potential_rs_DT_DW_{periodic,cutpot} of one slice in the reference's
+        layout: complex array [M][N] (x fastest), Muslib units."""
+        n = len(Z)
+        z = lambda v, d=0.0: np.ascontiguousarray(np.full(n, d) if v is None else v, np.float64)  # noqa: E731
+        g = np.zeros((M, N), np.complex128)
+        lib().orc_potential_rs_dt(a, b, gam, atrad, N, M, n, z(xf), z(yf), z(dw), z(ads), z(occ, 1.0),
+                                  np.ascontiguousarray(Z, np.int32), int(periodic), g.view(np.float64).reshape(-1))
+        return g
