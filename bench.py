@@ -90,6 +90,44 @@ def cpu_baseline(trans_host, cell, nthreads):
                       f"oracle FP32 loop, {nthreads} OpenMP threads, {dt:.1f} s"}
 
 
+def surrogate_rate(msx, torch, ctx, value):
+    """Converged ADF images/hr of this workload (128x128 scan, 100 slices):
+    plain 64-config multislice vs the surrogate (4 configs + one U-Net pass,
+    4 levels / 32 channels, timed here on a 128x128 image).  The surrogate's
+    accuracy comes from the evaluation report committed under profiles/ (its
+    own scaled-down specimen set, named there)."""
+    import glob
+    net = msx.UNet(ctx, 4, 32, seed=0)
+    x = torch.rand((1, SCAN, SCAN), device="cuda") + 0.5
+    for _ in range(3):
+        net.forward(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        net.forward(x)
+    torch.cuda.synchronize()
+    t_unet = (time.perf_counter() - t0) / reps
+    net.close()
+    per_config = SCAN * SCAN * NSLICE / value
+    out = {"plain_64config": round(3600.0 / (64 * per_config), 3),
+           "surrogate_4config": round(3600.0 / (4 * per_config + t_unet), 3),
+           "gpu_time_ratio": round(64 * per_config / (4 * per_config + t_unet), 2),
+           "unet_forward_ms_128px": round(1e3 * t_unet, 3)}
+    reps_ = sorted(glob.glob(os.path.join(ROOT, "profiles", "surrogate_eval_*.json")))
+    if reps_:
+        with open(reps_[-1]) as f:
+            ev = json.load(f)
+        out["accuracy"] = {"source": os.path.relpath(reps_[-1], ROOT),
+                           "held_out_specimens": ev["eval_specimens"],
+                           "surrogate_rel_rms_mean": round(ev["eval"]["surrogate_rel_rms_mean"], 4),
+                           "input_4config_rel_rms_mean": round(ev["eval"]["input_rel_rms_mean"], 4),
+                           "fraction_within_2pct": round(ev["eval"]["fraction_within_2pct_surrogate"], 3),
+                           "setup": "SrTiO3/Si/Au-on-aC, 5-40 nm, 128^2 grid, 32x32 ADF scans (40 mrad to the "
+                                    "band limit), judged against plain 64-config multislice"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +136,8 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--prof-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch slice loops directly (no HIP graph replay)")
+    ap.add_argument("--no-tune", action="store_true", help="built-in launch shapes instead of the autotuner's")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the path shards over probes/configs "
@@ -107,6 +147,11 @@ def main():
     import multislice_amd as msx
     import torch
     ctx = msx.Context(0)
+    ctx.set_graphs(not a.no_graphs)
+    tuning = None
+    if not a.no_tune:
+        tuning = ctx.autotune(N, prec=msx.F32, max_batch=a.batch)
+        log(f"autotuner (512^2 FP32): {tuning}")
     log("building potentials + transmissions for %d configs..." % NCONF)
     trans, cell, t_build, natoms = build_specimen(ctx, torch)
     log(f"  built in {t_build:.2f} s ({natoms} atoms/config)")
@@ -152,6 +197,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
                 "per_kernel": per_kernel}
 
+    surrogate = surrogate_rate(msx, torch, ctx, value)
+
     cpu = None
     if not a.no_cpu:
         nthr = max(1, min(16, os.cpu_count() or 1))
@@ -167,10 +214,11 @@ def main():
                                "300 kV, 21 mrad, Cs 1 um, 128x128 scan, BF 0-10 / ADF 70-200 mrad, "
                                "8 frozen-phonon configs resident",
                    "grid": N, "slices": NSLICE, "probes_per_launch": B, "frozen_phonon_configs": NCONF,
-                   "scan": [SCAN, SCAN]},
+                   "scan": [SCAN, SCAN], "launch_shapes": tuning, "hip_graphs": not a.no_graphs},
         "ms_per_slice": round(1e3 * dt / (a.steps * NSLICE), 4),
         "plain_64config_images_per_hr": round(value * 3600 / (SCAN * SCAN * NSLICE * 64), 3),
         "potential_and_transmission_build_s_per_config": round(t_build / NCONF, 3),
+        "converged_adf_images_per_hr": surrogate,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

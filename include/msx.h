@@ -70,6 +70,26 @@ MSX_API int msx_destroy(msx_ctx ctx);
 MSX_API int msx_set_stream(msx_ctx ctx, void* hip_stream);
 MSX_API const char* msx_last_error(void);
 MSX_API int msx_version(void);
+/* Replay slice loops (msx_multislice, msx_stem batches) from cached HIP graphs
+ * (the reference relaunched every kernel and rebuilt FFT plans each slice,
+ * ModelSimulated.cu:60-120).  The first call with a given set of launch
+ * arguments captures the per-slice sequence; identical later calls replay it
+ * with one graph launch.  Off by default; ignored while profiling. */
+MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
+/* Autotuner: times the slice loop of an nx x ny grid on synthetic waves for
+ * candidate launch shapes and keeps the fastest in the context (used by every
+ * later slice loop and msx_stem of that grid and precision): waves per
+ * work-group of the row and column sweeps (0 = built-in heuristic) and the
+ * STEM batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
+ * ms_per_slice is the time of one wave through one slice at that batch.
+ * msx_set_tuning installs a saved result (NULL clears it). */
+typedef struct msx_tuning {
+    int32_t row_wpb, col_wpb, batch;
+    double ms_per_slice;
+} msx_tuning;
+MSX_API int msx_autotune(msx_ctx ctx, int nx, int ny, int prec, int max_batch, msx_tuning* best);
+MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuning* t);
+MSX_API int msx_graph_stats(msx_ctx ctx, int64_t* captures, int64_t* replays);
 
 /* Kernel timing (the reference timed its loops with cudaEvents and printed
  * them, ModelPotential.cu:46-50,92-96 and ModelSimulated.cu:43-47,178-182).
@@ -210,6 +230,22 @@ typedef struct msx_stack {
  * wave) and psi is output only. */
 MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
                            void* psi, int nwave, int prec, int where);
+
+/* Many independent HRTEM jobs of uneven sizes and thicknesses in as few
+ * launches as possible (the reference ran one job per process,
+ * statistics/getStatistics.py:34 -> LMultiSlice main).  Each job is a full
+ * msx_multislice of one wave.  Jobs with the same grid, cell, kV, tilt, band
+ * limit and slice thicknesses (over their common slices) advance together:
+ * one row and one column sweep per slice for all of their waves still
+ * running, each wave with its own transmission.  Device buffers only
+ * (where = MSX_DEVICE); psi0 may be NULL (psi holds the incident wave). */
+typedef struct msx_job {
+    const msx_stack* stack;
+    const void* trans;  /* [nlayer][nx][ny] complex */
+    const void* psi0;   /* incident wave [nx][ny], or NULL */
+    void* psi;          /* exit wave out (and incident wave in when psi0 is NULL) */
+} msx_job;
+MSX_API int msx_multislice_batch(msx_ctx ctx, const msx_job* jobs, int njobs, int prec, int where);
 
 /* ---- STEM --------------------------------------------------------------- */
 /* Probe of Musli ConvergedBeam/Xi (MSProject.cs:458-514) in Kirkland's

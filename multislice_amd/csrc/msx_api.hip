@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/msx.h"
@@ -147,6 +148,31 @@ struct msx_ctx_s {
     std::map<int, DevBuf> tw32, tw64;
     DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp;
 
+    // launch shapes picked by msx_autotune / msx_set_tuning, per (nx, ny, prec)
+    std::map<std::tuple<int, int, int>, msx_tuning> tune;
+    const msx_tuning* tuning(int nx, int ny, int prec) const {
+        auto it = tune.find(std::make_tuple(nx, ny, prec));
+        return it == tune.end() ? nullptr : &it->second;
+    }
+
+    // hipGraph replay of slice loops (msx_set_graphs): a keying pass records
+    // the kind and argument bytes of every launch instead of launching; the
+    // key selects a cached executable graph captured from the same sequence.
+    bool graphs = false;
+    bool keying = false;
+    std::string gkey;
+    std::map<std::string, hipGraphExec_t> gcache;
+    long long graph_hits = 0, graph_captures = 0;
+    void key_add(int kind, int n, int mode, const void* args, size_t bytes) {
+        const int hdr[3] = {kind, n, mode};
+        gkey.append(reinterpret_cast<const char*>(hdr), sizeof hdr);
+        gkey.append(static_cast<const char*>(args), bytes);
+    }
+    void drop_graphs() {
+        for (auto& kv : gcache) hipGraphExecDestroy(kv.second);
+        gcache.clear();
+    }
+
     template <typename C> int twiddles(int n, const C** out) {
         auto& m = (sizeof(C) == 8) ? tw32 : tw64;
         DevBuf& b = m[n];
@@ -161,6 +187,7 @@ struct msx_ctx_s {
     ~msx_ctx_s() {
         for (auto& kv : tw32) kv.second.release();
         for (auto& kv : tw64) kv.second.release();
+        drop_graphs();
         for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp}) b->release();
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
@@ -264,18 +291,31 @@ template <typename C> double row_bytes(const RowArgs<C>& a, int n, double band) 
     const double px = (double)a.nx * n;
     const double fin = (a.prune & 1) ? band : 1.0, fout = (a.prune & 2) ? band : 1.0;
     return px * a.nb * sizeof(C) * (fin + fout) + (a.trans ? px * sizeof(C) : 0.0) +
+           (a.trans_tab ? px * a.nb * sizeof(C) : 0.0) +
            (a.vreal ? px * a.nb * (0.5 * (double)sizeof(C) - (double)sizeof(C)) : 0.0);
 }
 template <typename C> double col_bytes(const ColArgs<C>& a, int n, double band) {
     const double px = (double)a.ny * n * a.nb * (a.prune ? band : 1.0);
     return px * (((a.ops & kColGen) ? 0.0 : sizeof(C)) + ((a.ops & kColStore) ? sizeof(C) : 0.0));
 }
-#define LAUNCH_ROW(n, mode, args) \
-    CHK(ctx->timed(MSX_PROF_ROW, ctx->prof ? row_bytes<C>((args), (n), ctx->band) : 0.0, \
-                   [&] { return launch_row<C>((n), (mode), (args), ctx->stream); }))
-#define LAUNCH_COL(n, mode, args) \
-    CHK(ctx->timed(MSX_PROF_COL, ctx->prof ? col_bytes<C>((args), (n), ctx->band) : 0.0, \
-                   [&] { return launch_col<C>((n), (mode), (args), ctx->stream); }))
+#define LAUNCH_ROW(n, mode, args)                                                                       \
+    do {                                                                                                \
+        if (ctx->keying) {                                                                              \
+            ctx->key_add(MSX_PROF_ROW, (n), (mode), &(args), sizeof(args));                             \
+            break;                                                                                      \
+        }                                                                                               \
+        CHK(ctx->timed(MSX_PROF_ROW, ctx->prof ? row_bytes<C>((args), (n), ctx->band) : 0.0,            \
+                       [&] { return launch_row<C>((n), (mode), (args), ctx->stream); }));               \
+    } while (0)
+#define LAUNCH_COL(n, mode, args)                                                                       \
+    do {                                                                                                \
+        if (ctx->keying) {                                                                              \
+            ctx->key_add(MSX_PROF_COL, (n), (mode), &(args), sizeof(args));                             \
+            break;                                                                                      \
+        }                                                                                               \
+        CHK(ctx->timed(MSX_PROF_COL, ctx->prof ? col_bytes<C>((args), (n), ctx->band) : 0.0,            \
+                       [&] { return launch_col<C>((n), (mode), (args), ctx->stream); }));               \
+    } while (0)
 
 // ---------------------------------------------------------------- fft2d
 template <typename C> int fft2d_dev(msx_ctx ctx, C* d, int nb, int nx, int ny, int sign) {
@@ -350,8 +390,8 @@ struct DetPlan {
 // With det != nullptr the loop stops after the last checkpoint; otherwise
 // it ends with the final inverse row sweep (exit waves in real space).
 template <typename C>
-int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-               const C* src, long long src_bstride, int start, DetPlan* det) {
+int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
+                    const C* src, long long src_bstride, int start, DetPlan* det) {
     using Re = typename real_of<C>::type;
     const int nx = st->nx, ny = st->ny;
     const long long img = (long long)nx * ny;
@@ -360,6 +400,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
     CHK(ctx->twiddles<C>(ny, &twy));
     const Re inv = (Re)1 / ((Re)nx * (Re)ny);
     const int groups = col_groups<C>(nx, ny);
+    const msx_tuning* tu = ctx->tuning(nx, ny, sizeof(Re) == 8 ? MSX_F64 : MSX_F32);
     int ick = 0;
     for (int s = 0; s < st->nslice; s++) {
         const int l = st->layer[s];
@@ -372,6 +413,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
         r.prune = (s == 0) ? 2 : 3;
+        if (tu) r.wpb = tu->row_wpb;
         if (s == 0 && start == 0) {
             r.src = src;
             r.src_bstride = src_bstride;
@@ -388,6 +430,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         c.px = tb.px + (long long)l * nx;
         c.py = tb.py + (long long)l * ny;
         c.ops = kColProp | kColStore;
+        if (tu) c.wpb = tu->col_wpb;
         bool is_ck = det && ick < det->nck && det->ck[ick] == s + 1;
         if (is_ck) {
             c.ops |= kColDetect;
@@ -406,11 +449,17 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
             LAUNCH_COL(nx, kColSlice, c);
         }
         if (is_ck) {
-            CHK(ctx->timed(MSX_PROF_DET, 8.0 * det->ndet * nb * (groups + 1), [&] {
-                return launch_det_finish(c.partial, det->ndet, nb, groups, 1.0 / ((double)nx * ny),
-                                         det->out + (long long)ick * det->ndet * det->nprobe, det->nprobe,
-                                         det->probe0, ctx->stream);
-            }));
+            double* dout = det->out + (long long)ick * det->ndet * det->nprobe;
+            if (ctx->keying) {
+                const long long fa[4] = {(long long)(intptr_t)c.partial, (long long)(intptr_t)dout, det->nprobe,
+                                         det->probe0};
+                ctx->key_add(MSX_PROF_DET, groups, det->ndet, fa, sizeof fa);
+            } else {
+                CHK(ctx->timed(MSX_PROF_DET, 8.0 * det->ndet * nb * (groups + 1), [&] {
+                    return launch_det_finish(c.partial, det->ndet, nb, groups, 1.0 / ((double)nx * ny), dout,
+                                             det->nprobe, det->probe0, ctx->stream);
+                }));
+            }
             ick++;
         }
         if (last) return MSX_OK;
@@ -422,6 +471,272 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         r.k2max = tb.k2max;
         r.prune = 1;
         LAUNCH_ROW(ny, kRowInv, r);
+    }
+    return MSX_OK;
+}
+
+// The slice loop, replayed from a cached hipGraph when graphs are enabled
+// (and profiling is off).  Kernel arguments are baked into a graph, so the
+// key is the byte string of every launch's arguments: any change of pointer,
+// size, table or detector set selects (or captures) another graph.  A capture
+// that fails falls back to launching the loop directly.
+template <typename C>
+int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
+               const C* src, long long src_bstride, int start, DetPlan* det) {
+    if (!ctx->graphs || ctx->prof) return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+    ctx->gkey.clear();
+    ctx->keying = true;
+    const int rk = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+    ctx->keying = false;
+    CHK(rk);
+    auto it = ctx->gcache.find(ctx->gkey);
+    if (it == ctx->gcache.end()) {
+        if (ctx->gcache.size() >= 64) ctx->drop_graphs();
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();  // e.g. the legacy default stream: launch directly
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+        }
+        const int rc = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        hipGraphExec_t ex = nullptr;
+        hipError_t ei = (rc == MSX_OK && ec == hipSuccess && g) ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0)
+                                                                : hipErrorUnknown;
+        if (g) hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            (void)hipGetLastError();
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+        }
+        it = ctx->gcache.emplace(ctx->gkey, ex).first;
+        ctx->graph_captures++;
+    } else {
+        ctx->graph_hits++;
+    }
+    HIPCHK(hipGraphLaunch(it->second, ctx->stream));
+    return MSX_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Times the slice loop on synthetic waves (zero data: the kernels do the same
+// work whatever the values) for each candidate launch shape, with the
+// per-launch event timing of the profiling mode, and keeps the fastest:
+// waves per work-group of the row and of the column sweep (1, 2, 4, 8; each
+// sweep on its own), then the STEM batch: the smallest candidate within 3% of
+// the best probe-slice rate.
+template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_batch, msx_tuning* res) {
+    using Re = typename real_of<C>::type;
+    const int prec = sizeof(Re) == 8 ? MSX_F64 : MSX_F32;
+    const long long img = (long long)nx * ny;
+    const long long cap = std::max(1LL, (2LL << 30) / (long long)(sizeof(C) * img));
+    const int batches[] = {16, 32, 64, 128, 256, 512};
+    int bmax = (int)std::min<long long>(max_batch > 0 ? max_batch : 512, cap);
+    const int nslice = 6;
+    std::vector<double> cz(1, 2.0);
+    std::vector<int32_t> lay(nslice, 0);
+    msx_stack st{};
+    st.nx = nx;
+    st.ny = ny;
+    st.ax = 0.1 * nx;
+    st.by = 0.1 * ny;
+    st.kev = 300.0;
+    st.bw = 2.0 / 3.0;
+    st.nlayer = 1;
+    st.cz = cz.data();
+    st.nslice = nslice;
+    st.layer = lay.data();
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, &st, true, &tb));
+    CHK(ctx->wave.ensure(sizeof(C) * img * bmax));
+    CHK(ctx->trans.ensure(sizeof(C) * img));
+    HIPCHK(hipMemsetAsync(ctx->wave.p, 0, sizeof(C) * img * bmax, ctx->stream));
+    HIPCHK(hipMemsetAsync(ctx->trans.p, 0, sizeof(C) * img, ctx->stream));
+    const bool prof0 = ctx->prof, graphs0 = ctx->graphs;
+    if (!ctx->ev0) {
+        HIPCHK(hipEventCreate(&ctx->ev0));
+        HIPCHK(hipEventCreate(&ctx->ev1));
+    }
+    ctx->graphs = false;
+    auto key = std::make_tuple(nx, ny, prec);
+    const bool had = ctx->tune.count(key) > 0;
+    const msx_tuning saved = had ? ctx->tune[key] : msx_tuning{};
+    // times (ms per launch) of the row and column sweeps for one shape
+    auto trial = [&](int rw, int cw, int nb, double* row_ms, double* col_ms) -> int {
+        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0};
+        ctx->prof = true;
+        for (int rep = 0; rep < 2; rep++) {  // the first pass warms up
+            for (int k = 0; k < MSX_PROF_KINDS; k++) ctx->prof_n[k] = 0, ctx->prof_ms[k] = 0, ctx->prof_bytes[k] = 0;
+            CHK(slice_loop_body<C>(ctx, &st, tb, static_cast<const C*>(ctx->trans.p), static_cast<C*>(ctx->wave.p),
+                                   nb, nullptr, 0, 0, nullptr));
+        }
+        *row_ms = ctx->prof_ms[MSX_PROF_ROW] / std::max(1LL, ctx->prof_n[MSX_PROF_ROW]);
+        *col_ms = ctx->prof_ms[MSX_PROF_COL] / std::max(1LL, ctx->prof_n[MSX_PROF_COL]);
+        return MSX_OK;
+    };
+    int rc = MSX_OK;
+    msx_tuning best{1, 1, 0, 0.0};
+    do {
+        const int nb = std::min(bmax, 256);
+        double brow = 1e30, bcol = 1e30;
+        for (int w : {1, 2, 4, 8}) {
+            double rm, cm;
+            if ((rc = trial(w, w, nb, &rm, &cm)) != MSX_OK) break;
+            if (rm < brow) brow = rm, best.row_wpb = w;
+            if (cm < bcol) bcol = cm, best.col_wpb = w;
+        }
+        if (rc != MSX_OK) break;
+        double rate[6] = {0}, top = 0.0;
+        for (int i = 0; i < 6; i++) {
+            if (batches[i] > bmax) break;
+            double rm, cm;
+            if ((rc = trial(best.row_wpb, best.col_wpb, batches[i], &rm, &cm)) != MSX_OK) break;
+            rate[i] = batches[i] / (rm + cm);
+            top = std::max(top, rate[i]);
+        }
+        if (rc != MSX_OK) break;
+        for (int i = 0; i < 6 && batches[i] <= bmax; i++)
+            if (rate[i] >= 0.97 * top) {
+                best.batch = batches[i];
+                best.ms_per_slice = 1.0 / rate[i];
+                break;
+            }
+    } while (0);
+    ctx->prof = prof0;
+    ctx->graphs = graphs0;
+    for (int k = 0; k < MSX_PROF_KINDS; k++) ctx->prof_n[k] = 0, ctx->prof_ms[k] = 0, ctx->prof_bytes[k] = 0;
+    if (rc != MSX_OK) {
+        if (had) ctx->tune[key] = saved; else ctx->tune.erase(key);
+        return rc;
+    }
+    ctx->tune[key] = best;
+    if (res) *res = best;
+    return MSX_OK;
+}
+
+// ---------------------------------------------------------------- job batches
+// Uneven HRTEM jobs (msx_multislice_batch).  Jobs whose propagator tables
+// agree (grid, cell, kV, tilt, band limit, and slice thicknesses over the
+// common slices) form one group; within a group the waves are ordered by
+// slice count (longest first), so the waves still running at slice s are a
+// prefix: every row and column sweep advances all of them in one launch, the
+// row sweep reading each wave's transmission through a per-wave pointer table.
+// A job's final inverse row sweep runs right after its last slice.
+struct BatchItem {
+    int job;
+    const msx_stack* st;
+};
+
+template <typename C>
+int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g) {
+    using Re = typename real_of<C>::type;
+    std::stable_sort(g.begin(), g.end(),
+                     [](const BatchItem& a, const BatchItem& b) { return a.st->nslice > b.st->nslice; });
+    const msx_stack* ref = g[0].st;  // the longest: its thicknesses cover every member's slices
+    const int nx = ref->nx, ny = ref->ny, nb = (int)g.size(), S = ref->nslice;
+    const long long img = (long long)nx * ny;
+    // synthetic stack: one layer per slice of the longest job (its thicknesses)
+    std::vector<double> cz(S);
+    std::vector<int32_t> lay(S);
+    for (int s = 0; s < S; s++) {
+        cz[s] = ref->cz[ref->layer[s]];
+        lay[s] = s;
+    }
+    msx_stack st = *ref;
+    st.nlayer = S;
+    st.cz = cz.data();
+    st.nslice = S;
+    st.layer = lay.data();
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, &st, true, &tb));
+    // wave slots and per-slice transmission pointers
+    CHK(ctx->wave.ensure(sizeof(C) * img * nb));
+    C* w = static_cast<C*>(ctx->wave.p);
+    std::vector<int> active(S + 1, 0);
+    for (int s = 0; s <= S; s++)
+        for (int i = 0; i < nb; i++) active[s] += g[i].st->nslice > s;
+    std::vector<const C*> tab((size_t)S * nb, nullptr);
+    for (int s = 0; s < S; s++)
+        for (int i = 0; i < active[s]; i++) {
+            const msx_stack* js = g[i].st;
+            tab[(size_t)s * nb + i] = static_cast<const C*>(jobs[g[i].job].trans) + (long long)js->layer[s] * img;
+        }
+    CHK(ctx->aux.ensure(sizeof(C*) * tab.size()));
+    const C* const* dtab = static_cast<const C* const*>(ctx->aux.p);
+    HIPCHK(hipMemcpyAsync(ctx->aux.p, tab.data(), sizeof(C*) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+    for (int i = 0; i < nb; i++) {
+        const msx_job& j = jobs[g[i].job];
+        HIPCHK(hipMemcpyAsync(w + i * img, j.psi0 ? j.psi0 : j.psi, sizeof(C) * img, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    }
+    const C *twx, *twy;
+    CHK(ctx->twiddles<C>(nx, &twx));
+    CHK(ctx->twiddles<C>(ny, &twy));
+    const Re inv = (Re)1 / ((Re)nx * (Re)ny);
+    for (int s = 0; s < S; s++) {
+        const int na = active[s];
+        RowArgs<C> r = row_args<C>(w, nx, na, twy);
+        r.trans_tab = dtab + (size_t)s * nb;
+        r.ky2 = tb.ky2;
+        r.k2max = tb.k2max;
+        r.prune = (s == 0) ? 2 : 3;
+        if (s == 0) {
+            LAUNCH_ROW(ny, kRowFwd, r);
+        } else {
+            r.scale = inv;
+            LAUNCH_ROW(ny, kRowSlice, r);
+        }
+        ColArgs<C> c = col_args<C>(w, ny, na, twx);
+        c.prune = 1;
+        c.kx2 = tb.kx2;
+        c.ky2 = tb.ky2;
+        c.k2max = tb.k2max;
+        c.px = tb.px + (long long)s * nx;
+        c.py = tb.py + (long long)s * ny;
+        c.ops = kColProp | kColStore;
+        LAUNCH_COL(nx, kColSlice, c);
+        const int done = na - active[s + 1];  // jobs whose last slice this was
+        if (done > 0) {
+            RowArgs<C> f = row_args<C>(w + (long long)active[s + 1] * img, nx, done, twy);
+            f.scale = inv;
+            f.ky2 = tb.ky2;
+            f.k2max = tb.k2max;
+            f.prune = 1;
+            LAUNCH_ROW(ny, kRowInv, f);
+        }
+    }
+    for (int i = 0; i < nb; i++)
+        HIPCHK(hipMemcpyAsync(jobs[g[i].job].psi, w + i * img, sizeof(C) * img, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+int check_stack(const msx_stack* st);
+
+template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs, int njobs) {
+    std::vector<bool> used(njobs, false);
+    for (int i = 0; i < njobs; i++) CHK(check_stack(jobs[i].stack));
+    auto same_optics = [](const msx_stack* a, const msx_stack* b) {
+        return a->nx == b->nx && a->ny == b->ny && a->ax == b->ax && a->by == b->by && a->kev == b->kev &&
+               a->ctilt_x == b->ctilt_x && a->ctilt_y == b->ctilt_y && a->bw == b->bw;
+    };
+    auto same_slices = [](const msx_stack* a, const msx_stack* b) {
+        const int n = std::min(a->nslice, b->nslice);
+        for (int s = 0; s < n; s++)
+            if (a->cz[a->layer[s]] != b->cz[b->layer[s]]) return false;
+        return true;
+    };
+    for (int i = 0; i < njobs; i++) {
+        if (used[i]) continue;
+        std::vector<BatchItem> g{{i, jobs[i].stack}};
+        used[i] = true;
+        for (int k = i + 1; k < njobs; k++) {
+            if (used[k] || !same_optics(jobs[i].stack, jobs[k].stack)) continue;
+            bool ok = true;
+            for (const BatchItem& m : g) ok = ok && same_slices(m.st, jobs[k].stack);
+            if (!ok) continue;
+            g.push_back({k, jobs[k].stack});
+            used[k] = true;
+        }
+        CHK(batch_group<C>(ctx, jobs, g));
     }
     return MSX_OK;
 }
@@ -475,6 +790,8 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
     using Re = typename real_of<C>::type;
     const int nx = st->nx, ny = st->ny;
     const long long img = (long long)nx * ny;
+    const msx_tuning* tu = ctx->tuning(nx, ny, sizeof(Re) == 8 ? MSX_F64 : MSX_F32);
+    if (batch <= 0 && tu && tu->batch > 0) batch = tu->batch;
     if (batch <= 0) {
         // default: enough waves to fill the GPU, bounded by ~2 GiB of waves
         long long b = (2LL << 30) / (long long)(sizeof(C) * img);
@@ -966,6 +1283,46 @@ MSX_API int msx_profile_read(msx_ctx ctx, int kind, int64_t* launches, double* t
     return MSX_OK;
 }
 
+MSX_API int msx_autotune(msx_ctx ctx, int nx, int ny, int prec, int max_batch, msx_tuning* best) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, MSX_DEVICE));
+    CHK(check_grid(nx, ny));
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (prec == MSX_F64) return autotune_impl<double2>(ctx, nx, ny, max_batch, best);
+    return autotune_impl<float2>(ctx, nx, ny, max_batch, best);
+}
+
+MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuning* t) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, MSX_DEVICE));
+    CHK(check_grid(nx, ny));
+    if (!t) {
+        ctx->tune.erase(std::make_tuple(nx, ny, prec));
+        return MSX_OK;
+    }
+    if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < 0 || t->col_wpb > 64 || t->batch < 0)
+        return fail(MSX_EINVAL, "bad tuning values");
+    ctx->tune[std::make_tuple(nx, ny, prec)] = *t;
+    return MSX_OK;
+}
+
+MSX_API int msx_set_graphs(msx_ctx ctx, int enable) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (!enable) ctx->drop_graphs();
+    ctx->graphs = enable != 0;
+    return MSX_OK;
+}
+
+MSX_API int msx_graph_stats(msx_ctx ctx, int64_t* captures, int64_t* replays) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    if (captures) *captures = ctx->graph_captures;
+    if (replays) *replays = ctx->graph_hits;
+    return MSX_OK;
+}
+
 MSX_API int msx_set_stream(msx_ctx ctx, void* s) {
     if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
     ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own;
@@ -1179,6 +1536,18 @@ MSX_API int msx_transmission_abs(msx_ctx ctx, const void* V, const void* Vabs, i
     if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(t, t_dev, tb, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MSX_OK;
+}
+
+MSX_API int msx_multislice_batch(msx_ctx ctx, const msx_job* jobs, int njobs, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    if (where != MSX_DEVICE) return fail(MSX_EINVAL, "msx_multislice_batch takes device buffers (MSX_DEVICE)");
+    if (njobs < 0 || (njobs > 0 && !jobs)) return fail(MSX_EINVAL, "bad jobs");
+    for (int i = 0; i < njobs; i++)
+        if (!jobs[i].stack || !jobs[i].trans || !jobs[i].psi) return fail(MSX_EINVAL, "job %d: stack, trans, psi required", i);
+    HIPCHK(hipSetDevice(ctx->device));
+    if (prec == MSX_F64) return multislice_batch_impl<double2>(ctx, jobs, njobs);
+    return multislice_batch_impl<float2>(ctx, jobs, njobs);
 }
 
 MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0, void* psi,

@@ -52,6 +52,7 @@ template <typename C> struct RowArgs {
     long long src_bstride;  // elements between waves of src (0: one input shared by all waves)
     const C* tw;            // twiddles of length N
     const C* trans;         // transmission [nx][N] shared by all waves, or nullptr
+    const C* const* trans_tab;  // per-wave transmission [nx][N] (wave b: trans_tab[b]), or nullptr
     const Re* vreal;        // real potential [nb][nx][N] (replaces the input), or nullptr
     const Re* vabs;         // optional absorptive potential, same layout: exp(-vscale*vabs) damping
     Re vscale;              // phase per unit of vreal (sigma/1000)
@@ -160,12 +161,18 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
             load(b, a);
             mask_in(a);
         }
+        if (p.trans_tab) {
+            // jobs batched in one launch: each wave has its own transmission
+            const C* trow = p.trans_tab[b] + (long long)ix * N;
+#pragma unroll
+            for (int e = 0; e < E; ++e) tr[e] = trow[t + e * T];
+        }
         if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
         if (p.scale != (Re)1) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = cscale(a[e], p.scale);
         }
-        if (p.trans) {
+        if (p.trans || p.trans_tab) {
             // cfpix::operator*= (cfpix.cpp:296-318): (wr*tr - wi*ti, wr*ti + wi*tr)
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = cmul(a[e], tr[e]);

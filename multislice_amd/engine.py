@@ -62,19 +62,29 @@ class Probe(C.Structure):
     _fields_ = [("alpha_max", C.c_double), ("alpha_min", C.c_double), ("cs", C.c_double), ("df", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("row_wpb", C.c_int32), ("col_wpb", C.c_int32), ("batch", C.c_int32), ("ms_per_slice", C.c_double)]
+
+
+class Job(C.Structure):
+    _fields_ = [("stack", C.c_void_p), ("trans", C.c_void_p), ("psi0", C.c_void_p), ("psi", C.c_void_p)]
+
+
 class Detector(C.Structure):
     _fields_ = [("k_inner", C.c_double), ("k_outer", C.c_double)]
 
 
 PROF_ROW, PROF_COL, PROF_SPLAT, PROF_DET = 0, 1, 2, 3
 
-EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_last_error", "msx_version",
+EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx_graph_stats", "msx_autotune",
+           "msx_set_tuning", "msx_last_error", "msx_version",
            "msx_profile", "msx_profile_read", "msx_displace", "msx_philox4x32_10",
            "msx_wavelength", "msx_sigma", "msx_vzatom", "msx_featom", "msx_fft2d", "msx_potential",
            "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image",
            "msx_unet_create", "msx_unet_destroy", "msx_unet_param_count", "msx_unet_params",
            "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
-           "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs"]
+           "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
+           "msx_multislice_batch"]
 
 _lib = None
 _lock = threading.Lock()
@@ -112,6 +122,8 @@ def load():
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     sig = {
         "msx_create": (i, [i, C.POINTER(vp)]), "msx_destroy": (i, [vp]), "msx_set_stream": (i, [vp, vp]),
+        "msx_set_graphs": (i, [vp, i]),
+        "msx_autotune": (i, [vp, i, i, i, i, C.POINTER(Tuning)]), "msx_set_tuning": (i, [vp, i, i, i, C.POINTER(Tuning)]), "msx_graph_stats": (i, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "msx_last_error": (C.c_char_p, []), "msx_version": (i, []),
         "msx_profile": (i, [vp, i]),
         "msx_profile_read": (i, [vp, i, C.POINTER(C.c_int64), C.POINTER(d), C.POINTER(d)]),
@@ -135,6 +147,7 @@ def load():
         "msx_potential_rec": (i, [vp, vp, C.c_int64, vp, vp, d, d, d, d, i, i, i, i, i, vp, vp, i, i]),
         "msx_transmission_abs": (i, [vp, vp, vp, i, i, i, d, d, d, d, vp, i, i]),
         "msx_potential_rs": (i, [vp, vp, C.c_int64, vp, vp, d, d, d, d, d, i, i, i, i, vp, vp, i, i]),
+        "msx_multislice_batch": (i, [vp, vp, i, i, i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -212,6 +225,28 @@ class Context:
         """Queue work on an external stream (torch.cuda.Stream or raw handle)."""
         raw = getattr(stream, "cuda_stream", stream)
         _check(load().msx_set_stream(self.h, C.c_void_p(raw) if raw else None))
+
+    def autotune(self, nx, ny=None, prec=F32, max_batch=0):
+        """Pick launch shapes for an nx x ny grid (msx_autotune); returns them as a dict."""
+        t = Tuning()
+        _check(load().msx_autotune(self.h, nx, ny or nx, prec, max_batch, C.byref(t)))
+        return dict(row_wpb=t.row_wpb, col_wpb=t.col_wpb, batch=t.batch, ms_per_slice=t.ms_per_slice)
+
+    def set_tuning(self, nx, ny=None, prec=F32, tuning=None):
+        """Install a saved autotuner result (dict as returned by autotune), or clear it (None)."""
+        t = None if tuning is None else C.byref(Tuning(tuning["row_wpb"], tuning["col_wpb"], tuning["batch"],
+                                                       tuning.get("ms_per_slice", 0.0)))
+        _check(load().msx_set_tuning(self.h, nx, ny or nx, prec, t))
+
+    def set_graphs(self, enable=True):
+        """Replay slice loops from cached HIP graphs (msx_set_graphs)."""
+        _check(load().msx_set_graphs(self.h, int(enable)))
+
+    def graph_stats(self):
+        """(captures, replays) of slice-loop graphs so far."""
+        c, r = C.c_int64(), C.c_int64()
+        _check(load().msx_graph_stats(self.h, C.byref(c), C.byref(r)))
+        return c.value, r.value
 
     def profile(self, enable=True):
         """Turn per-kernel event timing on/off (resets the totals)."""
@@ -331,6 +366,28 @@ class Context:
         nwave = int(np.prod(psi.shape[:-2])) if len(psi.shape) > 2 else 1
         _check(load().msx_multislice(self.h, C.byref(st), pt[0], p0, pw[0], nwave, prec, where))
         return psi
+
+    def multislice_batch(self, jobs):
+        """Uneven HRTEM jobs in shared launches (msx_multislice_batch).  jobs:
+        list of dicts with trans (torch CUDA (nlayer, nx, ny) complex), psi (torch
+        CUDA (nx, ny), incident in / exit out), ax, by, kev, cz, layer, and
+        optional tilt, bw, psi0."""
+        if not jobs:
+            return
+        prec = _prec_of(jobs[0]["trans"])
+        keep, arr = [], (Job * len(jobs))()
+        for k, j in enumerate(jobs):
+            nl, nx, ny = j["trans"].shape
+            st, kp = self._stack(nx, ny, j["ax"], j["by"], j["kev"], j["cz"], j["layer"], j.get("tilt", (0.0, 0.0)),
+                                 j.get("bw", 2.0 / 3.0))
+            keep += [st, kp, j]
+            for key in ("trans", "psi") + (("psi0",) if j.get("psi0") is not None else ()):
+                t = j[key]
+                if not (_is_torch(t) and t.is_cuda and t.is_contiguous() and _prec_of(t) == prec):
+                    raise ValueError(f"job {k}: {key} must be a contiguous CUDA tensor of the batch precision")
+            arr[k] = Job(C.addressof(st), j["trans"].data_ptr(),
+                         j["psi0"].data_ptr() if j.get("psi0") is not None else None, j["psi"].data_ptr())
+        _check(load().msx_multislice_batch(self.h, arr, len(jobs), prec, DEVICE))
 
     def stem(self, trans, cz, layer, ax, by, kev, probe, positions, detectors, checkpoints,
              bw=2.0 / 3.0, batch=0, weight=1.0, out=None, tilt=(0.0, 0.0)):

@@ -1,0 +1,76 @@
+"""Uneven HRTEM jobs batched into shared launches (msx_multislice_batch).
+
+Jobs of mixed grids (including radix-3 sizes), voltages and slice counts:
+every exit wave must equal, bit for bit, the same job run alone through
+msx_multislice (itself checked against the oracle), and one job is checked
+against the oracle directly.  The per-job FP32 error against an FP64 run of
+the same batch is reported and bounded (1e-5 relative L2).
+"""
+import numpy as np
+import pytest
+
+import multislice_amd as msx
+from oracle import Oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_jobs(rng, njobs, sizes, dtype):
+    import torch
+    jobs = []
+    for k in range(njobs):
+        n = int(rng.choice(sizes))
+        L = 0.1 * n
+        kev = float(rng.choice([200.0, 300.0]))
+        ns = int(rng.integers(3, 25))
+        nl = min(ns, 4)
+        V = np.abs(rng.standard_normal((nl, n, n))).cumsum(axis=1) % 25.0  # smooth-ish positive potentials
+        Vt = torch.from_numpy(V.astype(np.float64 if dtype == "f64" else np.float32)).cuda()
+        jobs.append(dict(V=Vt, n=n, ax=L, by=L, kev=kev, cz=np.full(nl, 2.0), layer=np.arange(ns) % nl))
+    return jobs
+
+
+def run_batch(ctx, jobs, cdtype):
+    import torch
+    out = []
+    for j in jobs:
+        j["trans"] = ctx.transmission(j["V"], j["ax"], j["by"], j["kev"])
+        j["psi"] = torch.ones((j["n"], j["n"]), dtype=cdtype, device="cuda")
+        out.append(j)
+    ctx.multislice_batch(out)
+    return [j["psi"].cpu().numpy() for j in out]
+
+
+def test_batch_equals_single_jobs(ctx):
+    import torch
+    rng = np.random.default_rng(4)
+    jobs = make_jobs(rng, 14, [64, 128, 384], "f32")
+    got = run_batch(ctx, jobs, torch.complex64)
+    for j, g in zip(jobs, got):
+        psi = torch.ones((j["n"], j["n"]), dtype=torch.complex64, device="cuda")
+        ctx.multislice(j["trans"], j["cz"], j["layer"], psi, j["ax"], j["by"], j["kev"])
+        np.testing.assert_array_equal(g, psi.cpu().numpy())
+    # one job against the oracle (FP32 restatement)
+    j = jobs[0]
+    ref = O.multislice(j["trans"].cpu().numpy(), j["cz"], j["layer"], np.ones((j["n"], j["n"]), np.complex64),
+                       j["ax"], j["by"], j["kev"], f32=True)
+    ref = ref[0] if isinstance(ref, tuple) else ref
+    assert np.linalg.norm(got[0] - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_batch_fp32_vs_fp64_per_job(ctx):
+    import torch
+    jobs32 = make_jobs(np.random.default_rng(7), 10, [128, 256, 384], "f32")
+    jobs64 = make_jobs(np.random.default_rng(7), 10, [128, 256, 384], "f64")
+    g32 = run_batch(ctx, jobs32, torch.complex64)
+    g64 = run_batch(ctx, jobs64, torch.complex128)
+    errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(g32, g64)]
+    print("per-job FP32 vs FP64 rel-L2:", ", ".join(f"{e:.1e}" for e in errs))
+    assert max(errs) < 1e-5
+
+
+def test_batch_rejects_host_buffers(ctx):
+    with pytest.raises(msx.MsxError):
+        st, _ = msx.Context._stack(64, 64, 6.4, 6.4, 200.0, [2.0], [0])
+        arr = (msx.engine.Job * 1)(msx.engine.Job(None, None, None, None))
+        msx.engine._check(msx.load().msx_multislice_batch(ctx.h, arr, 1, msx.F32, msx.HOST))

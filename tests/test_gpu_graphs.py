@@ -1,0 +1,99 @@
+"""Runtime: slice loops replayed from cached HIP graphs (msx_set_graphs) give the same
+bits as direct launches: every replay is checked against a direct run, with
+direct work on another context between replays; a changed argument (another
+transmission stack) captures a new graph instead of replaying a stale one.
+The autotuner's launch shapes change only how waves are grouped into
+work-groups: results stay bit-identical (a different STEM batch regroups the
+detector sums, so it is compared to 1e-12)."""
+import numpy as np
+import pytest
+
+import multislice_amd as msx
+from multislice_amd import specimen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def si_stack():
+    import torch
+    ctx = msx.Context(0)
+    at, cell = specimen.si_001((4, 4, 5))
+    V = torch.empty((20, 256, 256), dtype=torch.float32, device="cuda")
+    ctx.potential(at, cell, 256, 256, 20, 3.0, prec=msx.F32, out=V)
+    t = ctx.transmission(V, cell[0], cell[1], 200.0)
+    V2 = V * 1.1
+    t2 = ctx.transmission(V2, cell[0], cell[1], 200.0)
+    torch.cuda.synchronize()
+    yield ctx, t, t2, cell
+    ctx.close()
+
+
+def test_multislice_graph_replays_match_direct(si_stack):
+    import torch
+    ctx_d, t, t2, cell = si_stack
+    g = msx.Context(0)
+    g.set_graphs(True)
+    cz = np.full(20, cell[2] / 20)
+    layer = np.arange(20)
+    outs = []
+    # one wave buffer each: the graph key holds the wave pointer
+    psi_g = torch.empty((256, 256), dtype=torch.complex64, device="cuda")
+    psi_d = torch.empty((256, 256), dtype=torch.complex64, device="cuda")
+    for rep in range(4):
+        tt = t if rep != 2 else t2
+        psi_g.fill_(1)
+        g.multislice(tt, cz, layer, psi_g, cell[0], cell[1], 200.0)
+        psi_d.fill_(1)
+        ctx_d.multislice(tt, cz, layer, psi_d, cell[0], cell[1], 200.0)  # direct launches in between
+        assert torch.equal(psi_g, psi_d), f"replay {rep} differs"
+        outs.append(psi_g.cpu())
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[3]) and not torch.equal(outs[0], outs[2])
+    captures, replays = g.graph_stats()
+    assert captures == 2 and replays == 2, (captures, replays)
+    g.close()
+
+
+def test_stem_graph_replays_match_direct(si_stack):
+    ctx_d, t, _, cell = si_stack
+    g = msx.Context(0)
+    g.set_graphs(True)
+    cz = np.full(20, cell[2] / 20)
+    layer = np.arange(20)
+    lam = msx.wavelength(200.0)
+    det = [(0.0, 0.01 / lam), (0.04 / lam, 0.09 / lam)]
+    probe = dict(alpha_max=0.02, cs=1e4)
+    rng = np.random.default_rng(0)
+    for rep in range(3):
+        pos = rng.uniform(0, cell[0], (64, 2))  # same buffer sizes, new positions each call
+        a = g.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [10, 20], batch=64)
+        b = ctx_d.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [10, 20], batch=64)
+        np.testing.assert_array_equal(a, b)
+    captures, replays = g.graph_stats()
+    assert captures == 1 and replays == 2, (captures, replays)
+    g.close()
+
+
+def test_autotune_keeps_results_and_reports_shapes(si_stack):
+    ctx_d, t, _, cell = si_stack
+    c = msx.Context(0)
+    cz = np.full(20, cell[2] / 20)
+    layer = np.arange(20)
+    lam = msx.wavelength(200.0)
+    det = [(0.0, 0.01 / lam), (0.04 / lam, 0.09 / lam)]
+    probe = dict(alpha_max=0.02, cs=1e4)
+    pos = np.random.default_rng(1).uniform(0, cell[0], (100, 2))
+    ref = c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20], batch=50)
+    best = c.autotune(256, prec=msx.F32, max_batch=256)
+    print("tuned 256^2 FP32:", best)
+    assert best["row_wpb"] in (1, 2, 4, 8) and best["col_wpb"] in (1, 2, 4, 8)
+    assert 16 <= best["batch"] <= 256 and best["ms_per_slice"] > 0
+    got = c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20], batch=50)
+    np.testing.assert_array_equal(got, ref)
+    # an explicit shape, and the tuned default batch
+    c.set_tuning(256, prec=msx.F32, tuning=dict(row_wpb=1, col_wpb=8, batch=32))
+    np.testing.assert_array_equal(c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20], batch=50), ref)
+    np.testing.assert_allclose(c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20]), ref,
+                               rtol=1e-12, atol=0)
+    c.set_tuning(256, prec=msx.F32, tuning=None)
+    c.close()
