@@ -90,6 +90,32 @@ def cpu_baseline(trans_host, cell, nthreads):
                       f"oracle FP32 loop, {nthreads} OpenMP threads, {dt:.1f} s"}
 
 
+def copy_probe():
+    """A plain copy of known size: 512 MiB read and 512 MiB written per
+    launch.  Profiled with rocprofv3 --pmc FETCH_SIZE (then WRITE_SIZE), the
+    counters' readings for it scale the readings for the sweep kernels to
+    bytes."""
+    import torch
+    n = 512 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": 5}))
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the column sweep on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
+    probe); None when it is absent."""
+    path = os.path.join(ROOT, "profiles", "col_sweep_stem512_b256_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["hbm_bytes_per_launch"]
+
+
 def surrogate_rate(msx, torch, ctx, value):
     """Converged ADF images/hr of this workload (128x128 scan, 100 slices):
     plain 64-config multislice vs the surrogate (4 configs + one U-Net pass,
@@ -138,7 +164,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch slice loops directly (no HIP graph replay)")
     ap.add_argument("--no-tune", action="store_true", help="built-in launch shapes instead of the autotuner's")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 512 MiB device to device 5 times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the path shards over probes/configs "
                                    "across GPUs by design but no multi-GPU run is provided"}))
@@ -193,8 +224,9 @@ def main():
     per_kernel = {k: {"launches": v[0], "avg_us": 1e3 * v[1] / max(1, v[0]),
                       "algorithmic_MB_per_launch": v[2] / max(1, v[0]) / 1e6,
                       "GBps": v[2] / max(1e-12, v[1] * 1e-3) / 1e9} for k, v in kinds.items()}
+    traffic = traffic_from_profile() if dom == "col_sweep" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "per_kernel": per_kernel}
 
     surrogate = surrogate_rate(msx, torch, ctx, value)
