@@ -1,0 +1,135 @@
+"""HRTEM with frozen phonons, and batches of uneven HRTEM jobs.
+
+`frozen_phonon_image` is the large-HRTEM path of the reference (LMultiSlice
+imageCalculation per configuration, ModelSimulated.cu:21-192, with Kirkland's
+objective lens, mulslice.cpp:612-656): every configuration's atoms are
+displaced on the GPU (msx_displace), its potential and transmission stack are
+built on the GPU, all configurations advance through one batched slice loop
+(msx_multislice_batch: one wave per configuration, each with its own
+transmission), and the objective-lens images are averaged.
+
+    python -m multislice_amd.hrtem c2      # SrTiO3 52x52x50, 2048^2, 100 slices, 16 configurations
+    python -m multislice_amd.hrtem c4      # 256 uneven jobs, batched vs one by one
+
+print one JSON line of timings each.
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import engine, specimen
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+    return time.perf_counter()
+
+
+def frozen_phonon_image(ctx, atoms, cell, n, nslices, kev, u_rms, nconf, seed, cs, df, aobj, radius=3.0,
+                        prec=engine.F32):
+    """Mean objective-lens image (n x n, torch float64 on the GPU) over nconf
+    frozen-phonon configurations, and the seconds spent per stage."""
+    import torch
+    rdt, cdt = (torch.float64, torch.complex128) if prec == engine.F64 else (torch.float32, torch.complex64)
+    cz = np.full(nslices, cell[2] / nslices)
+    layer = np.arange(nslices)
+    atd = torch.from_numpy(np.ascontiguousarray(atoms, engine.ATOM_DTYPE).view(np.uint8).copy()).cuda()
+    V = torch.empty((nslices, n, n), dtype=rdt, device="cuda")
+    tm = {}
+    t0 = _sync()
+    trans = []
+    for c in range(nconf):
+        at = engine.displace(atd, u_rms, seed, c, ctx=ctx).cpu().numpy().view(engine.ATOM_DTYPE)
+        ctx.potential(at, cell, n, n, nslices, radius, prec=prec, out=V)
+        trans.append(ctx.transmission(V, cell[0], cell[1], kev))
+    t1 = _sync()
+    tm["displace_potential_transmission"] = t1 - t0
+    psis = [torch.ones((n, n), dtype=cdt, device="cuda") for _ in range(nconf)]
+    t1 = _sync()
+    ctx.multislice_batch([dict(trans=t, psi=p, ax=cell[0], by=cell[1], kev=kev, cz=cz, layer=layer)
+                          for t, p in zip(trans, psis)])
+    t2 = _sync()
+    tm["multislice_batched"] = t2 - t1
+    img = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    for p in psis:
+        img += ctx.ctf_image(p, cell[0], cell[1], kev, cs, df, aobj).double()
+    img /= nconf
+    tm["ctf_average"] = _sync() - t2
+    return img, tm
+
+
+def c2(ctx):
+    """The large-HRTEM configuration: SrTiO3 [001] 52x52x50 cells, 2048^2,
+    100 slices of 1.9525 A, 300 kV, 16 configurations, Cs 1.2 mm, defocus
+    -60 nm, 20 mrad aperture (one warm-up run, then a timed run)."""
+    at, cell = specimen.srtio3_001((52, 52, 50))
+    args = (ctx, at, cell, 2048, 100, 300.0, specimen.SRTIO3_U, 16, 2024, 1.2e7, -600.0, 0.020)
+    frozen_phonon_image(*args)
+    img, tm = frozen_phonon_image(*args)
+    ms = tm["multislice_batched"]
+    return {"workload": "C2 frozen-phonon HRTEM, SrTiO3 52x52x50 (676,000 atoms), 2048^2, 100 slices, 16 configs",
+            "seconds": {k: round(v, 4) for k, v in tm.items()}, "total_s": round(sum(tm.values()), 3),
+            "ms_per_wave_slice": round(1e3 * ms / (16 * 100), 4), "image_mean": float(img.mean())}
+
+
+def uneven_jobs(rng, njobs, sizes=(256, 384, 512, 768, 1024)):
+    """Synthetic uneven jobs: grid n from `sizes`, lateral size 0.1 A * n,
+    20-200 slices of 2 A, 200 or 300 kV, four distinct smooth random phase
+    layers per job (device tensors)."""
+    import torch
+    jobs = []
+    for _ in range(njobs):
+        n = int(rng.choice(sizes))
+        ns = int(rng.integers(20, 201))
+        kev = float(rng.choice([200.0, 300.0]))
+        V = torch.rand((4, n, n), device="cuda") * 20.0
+        jobs.append(dict(n=n, V=V, ax=0.1 * n, by=0.1 * n, kev=kev, cz=np.full(4, 2.0), layer=np.arange(ns) % 4))
+    return jobs
+
+
+def c4(ctx, njobs=256, seed=5):
+    """Uneven jobs: one msx_multislice_batch call against one msx_multislice
+    call per job (same jobs, same results), after a warm-up of each."""
+    import torch
+    jobs = uneven_jobs(np.random.default_rng(seed), njobs)
+    for j in jobs:
+        j["trans"] = ctx.transmission(j["V"], j["ax"], j["by"], j["kev"])
+        del j["V"]
+    wave_slices = sum(len(j["layer"]) for j in jobs)
+
+    def batched():
+        for j in jobs:
+            j["psi"] = torch.ones((j["n"], j["n"]), dtype=torch.complex64, device="cuda")
+        t0 = _sync()
+        ctx.multislice_batch(jobs)
+        return _sync() - t0
+
+    def one_by_one():
+        t0 = _sync()
+        for j in jobs:
+            psi = torch.ones((j["n"], j["n"]), dtype=torch.complex64, device="cuda")
+            ctx.multislice(j["trans"], j["cz"], j["layer"], psi, j["ax"], j["by"], j["kev"])
+        return _sync() - t0
+
+    batched(), one_by_one()
+    tb, ts = batched(), one_by_one()
+    return {"workload": f"C4 {njobs} uneven HRTEM jobs, n in 256..1024, 20-200 slices, 200/300 kV",
+            "wave_slices": wave_slices, "batched_s": round(tb, 3), "one_by_one_s": round(ts, 3),
+            "speedup": round(ts / tb, 2), "wave_slices_per_s_batched": round(wave_slices / tb)}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m multislice_amd.hrtem")
+    ap.add_argument("workload", choices=["c2", "c4"])
+    a = ap.parse_args(argv)
+    ctx = engine.Context(0)
+    print(json.dumps(c2(ctx) if a.workload == "c2" else c4(ctx)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
+    sys.exit(0)
