@@ -188,6 +188,23 @@ MSX_API int msx_potential_rec(msx_ctx ctx, const msx_atom* atoms, int64_t natoms
                               const double* ads, double a, double b, double c, double gam, int nmult, int mmult,
                               int nx, int ny, int nslices, void* V, void* Vabs, int prec, int where);
 
+/* The tabulated-curve forms of the two potentials above: Muslib
+ * Potential_rec_curve_DW (Pot_rec_curve_DW.cpp:7-164) and
+ * Potential_rs_curve_DW_cutpot / _periodic (Pot_RS.cpp:5-55, 209-349).  Each
+ * atom has a type (0..ntypes-1) selecting a row of curves[ntypes][npoints],
+ * sampled at spacing dds and interpolated linearly; nothing is added past the
+ * last point.  Reciprocal space: the row is sampled at s = |g|/2 and already
+ * holds the Debye-Waller factor and 1/area (MSProject.cs:850-863); real space:
+ * sampled at r.  Output and scaling as msx_potential_rec / msx_potential_rs. */
+MSX_API int msx_potential_rec_curve(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const int32_t* type,
+                                    const double* curves, int ntypes, int npoints, double dds, const double* ads,
+                                    double a, double b, double c, double gam, int nmult, int mmult, int nx, int ny,
+                                    int nslices, void* V, void* Vabs, int prec, int where);
+MSX_API int msx_potential_rs_curve(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const int32_t* type,
+                                   const double* curves, int ntypes, int npoints, double dds, const double* ads,
+                                   double a, double b, double c, double gam, double atrad, int periodic, int nx,
+                                   int ny, int nslices, void* V, void* Vabs, int prec, int where);
+
 /* ---- transmission ----------------------------------------------------- */
 /* Transmission functions of nlayer potential layers (mulslice.cpp:414-421 and
  * the anti-aliasing pass :515-522; LMultiSlice phaseObject kernel.cu:4-18):

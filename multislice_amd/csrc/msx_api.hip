@@ -146,7 +146,7 @@ struct msx_ctx_s {
         return MSX_OK;
     }
     std::map<int, DevBuf> tw32, tw64;
-    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp;
+    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf;
 
     // launch shapes picked by msx_autotune / msx_set_tuning, per (nx, ny, prec)
     std::map<std::tuple<int, int, int>, msx_tuning> tune;
@@ -188,7 +188,8 @@ struct msx_ctx_s {
         for (auto& kv : tw32) kv.second.release();
         for (auto& kv : tw64) kv.second.release();
         drop_graphs();
-        for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp}) b->release();
+        for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf})
+            b->release();
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (own) hipStreamDestroy(own);
@@ -1005,9 +1006,34 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
 // (Kirkland's vzatom constant al = 4 pi^2 a0 e = 300.8242834, slicelib.cpp:2666-2699)
 constexpr double kVA2 = 300.8242834 / (2.0 * 3.14159265358979323846);
 
+// tabulated scattering curves (Muslib *_curve_* potentials): per-atom type
+// index into curves[ntypes][npoints], spacing dds
+struct CurveSpec {
+    const int32_t* type;
+    const double* curves;
+    int ntypes, npoints;
+    double dds;
+};
+int check_curves(const CurveSpec* cs, int64_t natoms) {
+    if (!cs->type || !cs->curves || cs->ntypes < 1 || cs->npoints < 2 || !(cs->dds > 0) ||
+        (long long)cs->ntypes * cs->npoints > (1LL << 26))
+        return fail(MSX_EINVAL, "curves: type, table (ntypes >= 1, npoints >= 2) and dds > 0 required");
+    for (int64_t i = 0; i < natoms; i++)
+        if (cs->type[i] < 0 || cs->type[i] >= cs->ntypes)
+            return fail(MSX_EINVAL, "atom %lld: type %d outside 0..%d", (long long)i, cs->type[i], cs->ntypes - 1);
+    return MSX_OK;
+}
+int upload_curves(msx_ctx ctx, const CurveSpec* cs, const double** dev) {
+    const size_t bytes = sizeof(double) * (size_t)cs->ntypes * cs->npoints;
+    CHK(ctx->curvebuf.ensure(bytes));
+    HIPCHK(hipMemcpyAsync(ctx->curvebuf.p, cs->curves, bytes, hipMemcpyHostToDevice, ctx->stream));
+    *dev = static_cast<const double*>(ctx->curvebuf.p);
+    return MSX_OK;
+}
+
 int potential_rec_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw, const double* ads,
                        double a, double b, double c, double gam, int nmult, int mmult, int nx, int ny, int nslices,
-                       void* V, void* Vabs, int prec, int where) {
+                       void* V, void* Vabs, int prec, int where, const CurveSpec* cs = nullptr) {
     const double dz = c / nslices;
     const double sg = std::sin(gam), cg = std::cos(gam);
     struct Item {
@@ -1020,15 +1046,15 @@ int potential_rec_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const
     items.reserve(natoms);
     for (int64_t i = 0; i < natoms; i++) {
         const msx_atom& at = atoms[i];
-        if (at.Z < 1 || at.Z > 92) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..92 (Doyle-Turner table)",
-                                               (long long)i, at.Z);
+        if (!cs && (at.Z < 1 || at.Z > 92))
+            return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..92 (Doyle-Turner table)", (long long)i, at.Z);
         long long kz = (long long)std::floor(at.z / dz);
         if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
         if (kz < 0 || kz >= nslices) continue;
         // fractional coordinates: a along x, b at angle gam from a in the xy plane
         const double yf = at.y / (b * sg);
         const double xf = (at.x - at.y * cg / sg) / a;
-        items.push_back(Item{kz, at.Z, dw ? dw[i] : 0.0, ads ? ads[i] : 0.0,
+        items.push_back(Item{kz, cs ? cs->type[i] : at.Z, (dw && !cs) ? dw[i] : 0.0, ads ? ads[i] : 0.0,
                              RecAtom{xf - std::floor(xf), yf - std::floor(yf), (double)at.occ}});
     }
     std::sort(items.begin(), items.end(), [](const Item& p, const Item& q) {
@@ -1080,6 +1106,14 @@ int potential_rec_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const
     ra.ny = ny;
     ra.nmult = nmult;
     ra.mmult = mmult;
+    ra.curves = nullptr;
+    ra.npoints = 0;
+    ra.dds = 1.0;
+    if (cs) {
+        CHK(upload_curves(ctx, cs, &ra.curves));
+        ra.npoints = cs->npoints;
+        ra.dds = cs->dds;
+    }
     for (int k0 = 0; k0 < nslices; k0 += chunk) {
         const int nk = std::min(chunk, nslices - k0);
         ra.kz0 = k0;
@@ -1105,7 +1139,7 @@ int potential_rec_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const
 
 int potential_rs_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw, const double* ads,
                       double a, double b, double c, double gam, double atrad, int periodic, int nx, int ny,
-                      int nslices, void* V, void* Vabs, int prec, int where) {
+                      int nslices, void* V, void* Vabs, int prec, int where, const CurveSpec* cs = nullptr) {
     const double dz = c / nslices;
     const double sg = std::sin(gam), cg = std::cos(gam);
     const double drax = atrad * sg / a, dray = atrad * sg / b;
@@ -1116,8 +1150,8 @@ int potential_rs_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const 
     rat.reserve(natoms);
     for (int64_t i = 0; i < natoms; i++) {
         const msx_atom& at = atoms[i];
-        if (at.Z < 1 || at.Z > 92) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..92 (Doyle-Turner table)",
-                                               (long long)i, at.Z);
+        if (!cs && (at.Z < 1 || at.Z > 92))
+            return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..92 (Doyle-Turner table)", (long long)i, at.Z);
         long long kz = (long long)std::floor(at.z / dz);
         if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
         if (kz < 0 || kz >= nslices) continue;
@@ -1128,8 +1162,8 @@ int potential_rs_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const 
         const long jbb = (long)((yf - dray) * ny), jee = (long)((yf + dray) * ny);
         if (ibb < -(1L << 30) || iee > (1L << 30) || jbb < -(1L << 30) || jee > (1L << 30))
             return fail(MSX_EINVAL, "atrad too large for the grid");
-        rat.push_back(RsAtom{xf, yf, (double)at.occ, dw ? dw[i] : 0.0, ads ? ads[i] : 0.0, at.Z, (int)ibb, (int)iee,
-                             (int)jbb, (int)jee, 0});
+        rat.push_back(RsAtom{xf, yf, (double)at.occ, (dw && !cs) ? dw[i] : 0.0, ads ? ads[i] : 0.0,
+                             cs ? cs->type[i] : at.Z, (int)ibb, (int)iee, (int)jbb, (int)jee, 0});
         akz.push_back((int)kz);
     }
     // (atom, image) entries per (slice, tile): count, then fill
@@ -1205,6 +1239,12 @@ int potential_rs_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const 
     ra.ntx = ntx;
     ra.nty = nty;
     ra.nslices = nslices;
+    ra.dds = 1.0;
+    if (cs) {
+        CHK(upload_curves(ctx, cs, &ra.curves));
+        ra.npoints = cs->npoints;
+        ra.dds = cs->dds;
+    }
     CHK(ctx->timed(MSX_PROF_SPLAT, (double)vbytes, [&] { return launch_rs_dt(ra, ctx->stream); }));
     if (where == MSX_HOST) {
         HIPCHK(hipMemcpyAsync(V, vd, vbytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -1497,6 +1537,42 @@ MSX_API int msx_potential_rs(msx_ctx ctx, const msx_atom* atoms, int64_t natoms,
     HIPCHK(hipSetDevice(ctx->device));
     return potential_rs_impl(ctx, atoms, natoms, dw, ads, a, b, c, gam, atrad, periodic, nx, ny, nslices, V, Vabs,
                              prec, where);
+}
+
+MSX_API int msx_potential_rec_curve(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const int32_t* type,
+                                    const double* curves, int ntypes, int npoints, double dds, const double* ads,
+                                    double a, double b, double c, double gam, int nmult, int mmult, int nx, int ny,
+                                    int nslices, void* V, void* Vabs, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_grid(nx, ny));
+    if (nslices <= 0 || !(a > 0) || !(b > 0) || !(c > 0) || !(gam > 0.0 && gam < 3.14159265358979323846) ||
+        nmult < 1 || mmult < 1)
+        return fail(MSX_EINVAL, "bad cell, slices or multipliers");
+    if (natoms < 0 || (natoms > 0 && !atoms) || !V || natoms > (1LL << 30)) return fail(MSX_EINVAL, "bad atoms/V");
+    const CurveSpec cs{type, curves, ntypes, npoints, dds};
+    CHK(check_curves(&cs, natoms));
+    HIPCHK(hipSetDevice(ctx->device));
+    return potential_rec_impl(ctx, atoms, natoms, nullptr, ads, a, b, c, gam, nmult, mmult, nx, ny, nslices, V, Vabs,
+                              prec, where, &cs);
+}
+
+MSX_API int msx_potential_rs_curve(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const int32_t* type,
+                                   const double* curves, int ntypes, int npoints, double dds, const double* ads,
+                                   double a, double b, double c, double gam, double atrad, int periodic, int nx,
+                                   int ny, int nslices, void* V, void* Vabs, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    if (nx < 1 || ny < 1 || nx > 16384 || ny > 16384 || nslices <= 0)
+        return fail(MSX_EINVAL, "bad grid %d x %d x %d", nx, ny, nslices);
+    if (!(a > 0) || !(b > 0) || !(c > 0) || !(atrad > 0) || !(gam > 0.0 && gam < 3.14159265358979323846))
+        return fail(MSX_EINVAL, "bad cell or atrad");
+    if (natoms < 0 || (natoms > 0 && !atoms) || !V || natoms > (1LL << 30)) return fail(MSX_EINVAL, "bad atoms/V");
+    const CurveSpec cs{type, curves, ntypes, npoints, dds};
+    CHK(check_curves(&cs, natoms));
+    HIPCHK(hipSetDevice(ctx->device));
+    return potential_rs_impl(ctx, atoms, natoms, nullptr, ads, a, b, c, gam, atrad, periodic, nx, ny, nslices, V, Vabs,
+                             prec, where, &cs);
 }
 
 MSX_API int msx_transmission(msx_ctx ctx, const void* V, int nlayer, int nx, int ny, double ax, double by, double kev,

@@ -108,6 +108,11 @@ struct RecArgs {
     double2* out;          // V(g) of slices kz0 .. kz0+nslices-1: [slice][nx][ny]
     double A, B, cg, inv_area;  // reciprocal lengths 1/(a sin gam), 1/(b sin gam); cos gam; 1/area
     int nx, ny, nmult, mmult, kz0;
+    // tabulated curves (instead of Doyle-Turner): [type][npoints] at |g|/2 = i*dds,
+    // already including the Debye-Waller factor and 1/area; seg.Z = type
+    const double* curves;
+    int npoints;
+    double dds;
 };
 hipError_t launch_rec_sf(const RecArgs& a, int nslices, hipStream_t s);
 hipError_t launch_split_potential(const double2* w, long long n, double scale, void* V, void* Vabs, bool f64,
@@ -129,6 +134,10 @@ struct RsArgs {
     float* a32;
     double a, b, cg, dxa, dyb, scale;
     int nx, ny, ntx, nty, nslices;
+    // tabulated curves (instead of Doyle-Turner): [type][npoints] at r = i*dds; atom.Z = type
+    const double* curves;
+    int npoints;
+    double dds;
 };
 hipError_t launch_rs_dt(const RsArgs& a, hipStream_t s);
 

@@ -181,6 +181,16 @@ __device__ __forceinline__ bool refl_index(int i, int n, int mult, int* h) {
     return i > half && j % mult == 0;
 }
 
+// linear interpolation in a curve table, nothing past the end (Pot_RS.cpp:40-46,
+// Pot_rec_curve_DW.cpp:38-44)
+__device__ __forceinline__ bool curve_at(const double* c, int npoints, double u, double* out) {
+    const double f = floor(u);
+    if (f >= (double)(npoints - 1)) return false;
+    const int j = (int)f;
+    *out = c[j] + (c[j + 1] - c[j]) * (u - (double)j);
+    return true;
+}
+
 __global__ __launch_bounds__(256) void rec_sf_kernel(RecArgs p) {
     __shared__ double2 ph[8][32];
     const int ty = threadIdx.x & 15, tx = threadIdx.x >> 4;
@@ -223,10 +233,14 @@ __global__ __launch_bounds__(256) void rec_sf_kernel(RecArgs p) {
             }
             __syncthreads();
         }
-        const double* dt = c_dt[seg.Z];
         double f = 0.0;
-        for (int j = 0; j < 4; ++j) f += dt[2 * j] * exp(-(dt[2 * j + 1] + seg.dw) * s2);
-        f *= p.inv_area;
+        if (p.curves) {
+            curve_at(p.curves + (long long)seg.Z * p.npoints, p.npoints, sqrt(s2) / p.dds, &f);
+        } else {
+            const double* dt = c_dt[seg.Z];
+            for (int j = 0; j < 4; ++j) f += dt[2 * j] * exp(-(dt[2 * j + 1] + seg.dw) * s2);
+            f *= p.inv_area;
+        }
         vr += f * (sr - seg.ads * si);
         vi += f * (si + seg.ads * sr);
     }
@@ -300,9 +314,15 @@ __global__ __launch_bounds__(256) void rs_dt_kernel(RsArgs p) {
         if (ix >= ib && ix < ie && iy >= jb && iy < je) {
             const double x = ix * p.dxa - x0, y = iy * p.dyb - y0;
             const double r2 = x * (x + 2.0 * y * p.cg) + y * y;
-            const double res = dt_rs2(at.Z, r2, at.dw) * at.occ;
-            v += res;
-            va += res * at.ads;
+            double val = 0.0;
+            bool hit = true;
+            if (p.curves) hit = curve_at(p.curves + (long long)at.Z * p.npoints, p.npoints, sqrt(r2) / p.dds, &val);
+            else val = dt_rs2(at.Z, r2, at.dw);
+            if (hit) {
+                const double res = val * at.occ;
+                v += res;
+                va += res * at.ads;
+            }
         }
     }
     if (ix < p.nx && iy < p.ny) {

@@ -84,7 +84,7 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx
            "msx_unet_create", "msx_unet_destroy", "msx_unet_param_count", "msx_unet_params",
            "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
-           "msx_multislice_batch"]
+           "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve"]
 
 _lib = None
 _lock = threading.Lock()
@@ -148,6 +148,10 @@ def load():
         "msx_transmission_abs": (i, [vp, vp, vp, i, i, i, d, d, d, d, vp, i, i]),
         "msx_potential_rs": (i, [vp, vp, C.c_int64, vp, vp, d, d, d, d, d, i, i, i, i, vp, vp, i, i]),
         "msx_multislice_batch": (i, [vp, vp, i, i, i]),
+        "msx_potential_rec_curve": (i, [vp, vp, C.c_int64, vp, vp, i, i, d, vp, d, d, d, d, i, i, i, i, i, vp, vp, i,
+                                        i]),
+        "msx_potential_rs_curve": (i, [vp, vp, C.c_int64, vp, vp, i, i, d, vp, d, d, d, d, d, i, i, i, i, vp, vp, i,
+                                       i]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -287,8 +291,23 @@ class Context:
         return self._dt_potential("rs", atoms, cell, nx, ny, nslices, dw, ads, gamma, (float(atrad), int(periodic)),
                                   prec, out, out_abs, absorptive)
 
+    def potential_curve(self, kind, atoms, types, curves, dds, cell, nx, ny, nslices, ads=None, gamma=np.pi / 2,
+                        nmult=1, mmult=1, atrad=None, periodic=True, prec=F32, out=None, out_abs=None,
+                        absorptive=False):
+        """Tabulated-curve potentials: kind "rec" (msx_potential_rec_curve) or "rs"
+        (msx_potential_rs_curve); types: per-atom row of curves[ntypes][npoints]."""
+        tp = np.ascontiguousarray(types, np.int32)
+        cv = np.ascontiguousarray(curves, np.float64)
+        if cv.ndim != 2:
+            raise ValueError("curves must be (ntypes, npoints)")
+        self._keep_curves = (tp, cv)
+        head = (C.c_void_p(tp.ctypes.data), C.c_void_p(cv.ctypes.data), cv.shape[0], cv.shape[1], float(dds))
+        extra = (int(nmult), int(mmult)) if kind == "rec" else (float(atrad), int(periodic))
+        return self._dt_potential(kind + "_curve", atoms, cell, nx, ny, nslices, None, ads, gamma, extra, prec, out,
+                                  out_abs, absorptive, head=head)
+
     def _dt_potential(self, kind, atoms, cell, nx, ny, nslices, dw, ads, gamma, extra, prec, out, out_abs,
-                      absorptive):
+                      absorptive, head=None):
         at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
         n = len(at)
         arr = lambda v: None if v is None else np.ascontiguousarray(np.broadcast_to(v, (n,)), np.float64)  # noqa
@@ -301,10 +320,15 @@ class Context:
         pv = _buf(out, prec, complex_=False, writable=True)
         pa = _buf(out_abs, prec, complex_=False, writable=True) if out_abs is not None else (None, pv[1], None, None)
         where = _ptr_where([pv, pa])
-        args = [self.h, C.c_void_p(at.ctypes.data), n, None if dwa is None else C.c_void_p(dwa.ctypes.data),
-                None if ada is None else C.c_void_p(ada.ctypes.data), float(cell[0]), float(cell[1]), float(cell[2]),
-                float(gamma), *extra, nx, ny, nslices, pv[0], pa[0], prec, where]
-        fn = load().msx_potential_rs if kind == "rs" else load().msx_potential_rec
+        ap = None if ada is None else C.c_void_p(ada.ctypes.data)
+        if head is None:
+            args = [self.h, C.c_void_p(at.ctypes.data), n, None if dwa is None else C.c_void_p(dwa.ctypes.data), ap]
+        else:
+            args = [self.h, C.c_void_p(at.ctypes.data), n, *head, ap]
+        args += [float(cell[0]), float(cell[1]), float(cell[2]), float(gamma), *extra, nx, ny, nslices, pv[0], pa[0],
+                 prec, where]
+        fn = {"rs": load().msx_potential_rs, "rec": load().msx_potential_rec,
+              "rs_curve": load().msx_potential_rs_curve, "rec_curve": load().msx_potential_rec_curve}[kind]
         _check(fn(*args))
         return (out, out_abs) if out_abs is not None else out
 

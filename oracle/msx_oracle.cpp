@@ -50,6 +50,9 @@
  *       periodic form of a box crossing an edge equals the cut form of the
  *       shifted atom plus its wrapped remainder; integral of one atom's
  *       potential = 4 pi^2 sum_j a_j (DT forward scattering f_e(0) / pi).
+ *   potential_rec_curve / potential_rs_curve (tabulated curves): with a
+ *       fine table sampled from the Doyle-Turner formula they reproduce the
+ *       Doyle-Turner potentials to the linear-interpolation error (h^2 f''/8).
  */
 #include <cmath>
 #include <complex>
@@ -879,6 +882,90 @@ ORC_API void orc_potential_rs_dt(double a, double b, double gam, double atrad, i
                     const double x = i * dxa - x0;
                     const double r2 = x * (x + 2.0 * y * cg) + y * y;
                     const double res = orc_dt_rs2(Z[k], r2, dw[k]) * occ[k];
+                    double *o = gli + 2 * ((size_t)j * N + i);
+                    o[0] += res;
+                    o[1] += res * ads[k];
+                }
+            }
+        }
+    }
+}
+
+/* linear interpolation in a curve table with spacing dds; 0 past the end
+ * (Pot_RS.cpp:40-46, Pot_rec_curve_DW.cpp:38-44) */
+static bool curve_at(const double *c, int npoints, double u, double *out) {
+    const double f = std::floor(u);
+    if (f >= (double)(npoints - 1)) return false;
+    const long j = (long)f;
+    const double rem = u - (double)j;
+    *out = c[j] + (c[j + 1] - c[j]) * rem;
+    return true;
+}
+
+/* Potential_rec_curve_DW (Pot_rec_curve_DW.cpp:7-164): as orc_potential_rec_dt
+ * with the factor f_e(|g|/2) exp(-dw |g|^2/4) / area replaced by the curve of
+ * the atom's type at s = |g|/2 (table spacing dds; the table already holds the
+ * Debye-Waller factor and 1/area, MSProject.cs:850-863), times occ. */
+ORC_API void orc_potential_rec_curve(double a, double b, double gam, int nmult, int mmult, int N, int M, long na,
+                                     const double *xx, const double *yy, const double *ads, const double *occ,
+                                     const int *type, const double *curves, int npoints, double dds, double *gli) {
+    const double A = 1.0 / (a * std::sin(gam)), B = 1.0 / (b * std::sin(gam));
+    const double cg = std::cos(gam);
+    const int hmax = (N / 2 - 1) / nmult, kmax = (M / 2 - 1) / mmult;
+    for (long t = 0; t < na; t++) {
+        const double *c = curves + (size_t)type[t] * npoints;
+        for (int k = -kmax; k <= kmax; k++) {
+            const int row = ((k * mmult) % M + M) % M;
+            for (int h = -hmax; h <= hmax; h++) {
+                const int col = ((h * nmult) % N + N) % N;
+                const double x = h * A, y = k * B;
+                const double g2 = x * x + y * y - 2.0 * x * y * cg;
+                double v;
+                if (!curve_at(c, npoints, 0.5 * std::sqrt(g2) / dds, &v)) continue;
+                const double res = v * occ[t];
+                const double ph = -2.0 * M_PI * (h * xx[t] + k * yy[t]);
+                const double rr = std::cos(ph) * res, ri = std::sin(ph) * res;
+                double *o = gli + 2 * ((size_t)row * N + col);
+                o[0] += rr - ads[t] * ri;
+                o[1] += ri + ads[t] * rr;
+            }
+        }
+    }
+}
+
+/* Potential_rs_curve_DW_cutpot / _periodic (Pot_RS.cpp:5-55, 209-349): the
+ * boxes of orc_potential_rs_dt with the value curve(type, r / dds) * occ. */
+ORC_API void orc_potential_rs_curve(double a, double b, double gam, double atrad, int N, int M, long na,
+                                    const double *xx, const double *yy, const double *ads, const double *occ,
+                                    const int *type, const double *curves, int npoints, double dds, int periodic,
+                                    double *gli) {
+    const double sg = std::sin(gam), cg = std::cos(gam);
+    const double drax = atrad * sg / a, dray = atrad * sg / b;
+    const double dxa = a / (double)N, dyb = b / (double)M;
+    static const int sx[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
+    static const int sy[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+    for (long k = 0; k < na; k++) {
+        const double *c = curves + (size_t)type[k] * npoints;
+        const long ibb = (long)((xx[k] - drax) * N), iee = (long)((xx[k] + drax) * N);
+        const long jbb = (long)((yy[k] - dray) * M), jee = (long)((yy[k] + dray) * M);
+        for (int l = 0; l < (periodic ? 9 : 1); l++) {
+            long ib = std::max(ibb, 0L), ie = std::min(iee, (long)N);
+            long jb = std::max(jbb, 0L), je = std::min(jee, (long)M);
+            double x0 = xx[k] * a, y0 = yy[k] * b;
+            if ((sx[l] == 1 && !(ibb < 0)) || (sx[l] == -1 && !(iee >= N)) || (sy[l] == 1 && !(jbb < 0)) ||
+                (sy[l] == -1 && !(jee >= M)))
+                continue;
+            if (sx[l] == 1) { ib = ibb + N; ie = N; x0 += a; }
+            if (sx[l] == -1) { ib = 0; ie = iee - N + 1; x0 -= a; }
+            if (sy[l] == 1) { jb = jbb + M; je = M; y0 += b; }
+            if (sy[l] == -1) { jb = 0; je = jee - M + 1; y0 -= b; }
+            for (long j = jb; j < je; j++) {
+                const double y = j * dyb - y0;
+                for (long i = ib; i < ie; i++) {
+                    const double x = i * dxa - x0;
+                    double v;
+                    if (!curve_at(c, npoints, std::sqrt(x * (x + 2.0 * y * cg) + y * y) / dds, &v)) continue;
+                    const double res = v * occ[k];
                     double *o = gli + 2 * ((size_t)j * N + i);
                     o[0] += res;
                     o[1] += res * ads[k];

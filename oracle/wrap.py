@@ -52,6 +52,8 @@ def lib():
             ("orc_potential_rec_dt", None, [d, d, d, i, i, i, i, l, _dp, _dp, _dp, _dp, _dp, _ip, _dp]),
             ("orc_dt_rs2", d, [i, d, d]),
             ("orc_potential_rs_dt", None, [d, d, d, d, i, i, l, _dp, _dp, _dp, _dp, _dp, _ip, i, _dp]),
+            ("orc_potential_rec_curve", None, [d, d, d, i, i, i, i, l, _dp, _dp, _dp, _dp, _ip, _dp, i, d, _dp]),
+            ("orc_potential_rs_curve", None, [d, d, d, d, i, i, l, _dp, _dp, _dp, _dp, _ip, _dp, i, d, i, _dp]),
         ]:
             fn = getattr(L, name)
             fn.restype = res
@@ -288,4 +290,41 @@ class Oracle:
         g = np.zeros((M, N), np.complex128)
         lib().orc_potential_rs_dt(a, b, gam, atrad, N, M, n, z(xf), z(yf), z(dw), z(ads), z(occ, 1.0),
                                   np.ascontiguousarray(Z, np.int32), int(periodic), g.view(np.float64).reshape(-1))
+        return g
+
+    @staticmethod
+    def dt_curves_rec(Zs, dws, npoints, dds, area):
+        """CreatePots_rec_DT_DW (Curves.cpp:47-62): per type f_e(s) exp(-dw s^2) / area at s = i dds."""
+        s = np.arange(npoints) * dds
+        return np.stack([np.array([Oracle.dt_fe(z, v) for v in s]) * np.exp(-dw * s * s) / area
+                         for z, dw in zip(Zs, dws)])
+
+    @staticmethod
+    def dt_curves_rs(Zs, dws, npoints, dds):
+        """CreatePots_rs_DT_DW (Curves.cpp:63-81): per type the real-space Gaussian sum at r = i dds."""
+        r = np.arange(npoints) * dds
+        return np.stack([np.array([Oracle.dt_rs(z, v, dw) for v in r]) for z, dw in zip(Zs, dws)])
+
+    @staticmethod
+    def potential_rec_curve_g(types, xf, yf, a, b, N, M, curves, dds, gam=np.pi / 2, nmult=1, mmult=1, ads=None,
+                              occ=None):
+        n = len(types)
+        z = lambda v, d=0.0: np.ascontiguousarray(np.full(n, d) if v is None else v, np.float64)  # noqa: E731
+        curves = np.ascontiguousarray(curves, np.float64)
+        g = np.zeros((M, N), np.complex128)
+        lib().orc_potential_rec_curve(a, b, gam, nmult, mmult, N, M, n, z(xf), z(yf), z(ads), z(occ, 1.0),
+                                      np.ascontiguousarray(types, np.int32), curves.reshape(-1), curves.shape[1],
+                                      dds, g.view(np.float64).reshape(-1))
+        return g
+
+    @staticmethod
+    def potential_rs_curve_g(types, xf, yf, a, b, N, M, atrad, curves, dds, periodic=True, gam=np.pi / 2,
+                             ads=None, occ=None):
+        n = len(types)
+        z = lambda v, d=0.0: np.ascontiguousarray(np.full(n, d) if v is None else v, np.float64)  # noqa: E731
+        curves = np.ascontiguousarray(curves, np.float64)
+        g = np.zeros((M, N), np.complex128)
+        lib().orc_potential_rs_curve(a, b, gam, atrad, N, M, n, z(xf), z(yf), z(ads), z(occ, 1.0),
+                                     np.ascontiguousarray(types, np.int32), curves.reshape(-1), curves.shape[1], dds,
+                                     int(periodic), g.view(np.float64).reshape(-1))
         return g

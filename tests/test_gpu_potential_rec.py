@@ -164,3 +164,45 @@ def test_potential_rs_periodic_matches_rec(ctx):
     Vrs = ctx.potential_rs(at, (a, a, 2.0), 256, 256, 1, 5.99, dw=1.5, prec=msx.F64)
     Vrec = ctx.potential_rec(at, (a, a, 2.0), 256, 256, 1, dw=1.5, prec=msx.F64)
     assert np.abs(Vrs - Vrec).max() < 1e-6 * np.abs(Vrec).max()
+
+
+@pytest.mark.parametrize("kind", ["rec", "rs"])
+def test_curve_potentials_match_oracle(ctx, kind):
+    rng = np.random.default_rng(17 if kind == "rec" else 18)
+    a, b, c, gam, nx, ny, ns = 9.0, 8.0, 4.0, np.deg2rad(96.0), 64, 64, 2
+    at = random_atoms(rng, 30, a, b, c, [6])
+    types = rng.integers(0, 3, 30).astype(np.int32)
+    Zs, dws = [8, 38, 79], [0.5, 0.3, 0.6]
+    ads = rng.choice([0.0, 0.07], 30)
+    if kind == "rec":
+        npts = 300
+        dds = 0.5 * np.hypot(nx / 2 / a, ny / 2 / b) / 250  # the table ends inside the band: both branches run
+        curves = O.dt_curves_rec(Zs, dws, npts, dds, a * b * np.sin(gam))
+    else:
+        npts, atrad = 400, 2.5
+        dds = atrad * 1.2 / 350
+        curves = O.dt_curves_rs(Zs, dws, npts, dds)
+    dz = c / ns
+    kz = np.floor(at["z"] / dz).astype(int)
+    xf = (at["x"] - at["y"] * np.cos(gam) / np.sin(gam)) / a
+    yf = at["y"] / (b * np.sin(gam))
+    Vr, Var = np.zeros((ns, nx, ny)), np.zeros((ns, nx, ny))
+    for s in range(ns):
+        m = kz == s
+        occ = at["occ"][m].astype(np.float64)
+        if kind == "rec":
+            g = O.potential_rec_curve_g(types[m], xf[m] % 1, yf[m] % 1, a, b, nx, ny, curves, dds, gam=gam,
+                                        nmult=2, ads=ads[m], occ=occ)
+            v = np.fft.ifft2(g) * (nx * ny)
+        else:
+            v = O.potential_rs_curve_g(types[m], xf[m] % 1, yf[m] % 1, a, b, nx, ny, atrad, curves, dds,
+                                       periodic=True, gam=gam, ads=ads[m], occ=occ)
+        Vr[s], Var[s] = KVA2 * v.real.T, KVA2 * v.imag.T
+    kw = dict(nmult=2) if kind == "rec" else dict(atrad=atrad, periodic=True)
+    V, Va = ctx.potential_curve(kind, at, types, curves, dds, (a, b, c), nx, ny, ns, ads=ads, gamma=gam,
+                                prec=msx.F64, absorptive=True, **kw)
+    e, ea = rel(V, Vr), rel(Va, Var)
+    print(f"{kind} curves: V {e:.2e}, Vabs {ea:.2e}")
+    assert e < 1e-11 and ea < 1e-11
+    with pytest.raises(msx.MsxError):  # a type outside the table
+        ctx.potential_curve(kind, at, np.full(30, 3), curves, dds, (a, b, c), nx, ny, ns, **kw)

@@ -194,3 +194,47 @@ def test_rs_periodic_equals_reciprocal_space():
     vrec = (np.fft.ifft2(grec) * N * N).real
     err = np.abs(grs.real - vrec).max() / np.abs(vrec).max()
     assert err < 1e-6, err
+
+
+# ---------------------------------------------------------------- tabulated curves
+def test_rec_curve_from_fine_dt_table_matches_dt():
+    # a curve table sampled from the Doyle-Turner formula (CreatePots_rec_DT_DW)
+    # reproduces the formula up to the linear-interpolation error h^2 f''/8
+    a, b, N, M = 8.0, 7.0, 32, 32
+    Z, dws = [14, 79], [0.4, 0.7]
+    xf, yf = [0.1, 0.55], [0.3, 0.8]
+    smax = 0.5 * np.hypot(N / 2 / a, M / 2 / b) * 1.05
+    errs = []
+    for npts in (4000, 400):
+        dds = smax / (npts - 2)
+        curves = O.dt_curves_rec(Z, dws, npts, dds, a * b)
+        gc = O.potential_rec_curve_g([0, 1], xf, yf, a, b, N, M, curves, dds)
+        gd = O.potential_rec_g(Z, xf, yf, a, b, N, M, dw=dws)
+        errs.append(np.abs(gc - gd).max() / np.abs(gd).max())
+    # second-order convergence: 10x the points, ~100x smaller error
+    assert errs[0] < 1e-5 and 40 < errs[1] / errs[0] < 250, errs
+
+
+def test_rs_curve_from_fine_dt_table_matches_dt():
+    a, b, N, M, atrad = 9.0, 8.0, 48, 40, 2.5
+    Z, dws = [8, 38], [0.5, 0.3]
+    xf, yf = [0.05, 0.6], [0.9, 0.4]
+    rmax = atrad * np.sqrt(2) * 1.05
+    errs = []
+    for npts in (8000, 800):
+        dds = rmax / (npts - 2)
+        curves = O.dt_curves_rs(Z, dws, npts, dds)
+        gc = O.potential_rs_curve_g([0, 1], xf, yf, a, b, N, M, atrad, curves, dds)
+        gd = O.potential_rs_g(Z, xf, yf, a, b, N, M, atrad, dw=dws)
+        errs.append(np.abs(gc - gd).max() / np.abs(gd).max())
+    assert errs[0] < 1e-5 and 40 < errs[1] / errs[0] < 250, errs
+
+
+def test_curve_past_table_end_contributes_nothing():
+    # a table shorter than the reach: the reflections / pixels past the last
+    # point get nothing (Pot_rec_curve_DW.cpp:40, Pot_RS.cpp:41)
+    a, N = 8.0, 32
+    curves = np.ones((1, 5))
+    g = O.potential_rec_curve_g([0], [0.0], [0.0], a, a, N, N, curves, dds=0.1)
+    s = 0.5 * np.hypot(*np.meshgrid(np.fft.fftfreq(N, a / N), np.fft.fftfreq(N, a / N)))
+    assert np.all(g[s / 0.1 >= 4] == 0) and np.all(g[(s / 0.1 < 4) & (s > 0)].real == 1.0)
