@@ -26,6 +26,15 @@ constexpr int row_lpw() { return kT >= 256 ? 1 : 256 / kT; }
 #ifndef MSX_COL_TW
 #define MSX_COL_TW 16
 #endif
+// widest tile tried at N = 512, measured on gfx950: two 8-column work-groups per
+// CU (4 wavefronts each, 74 KB of double-buffered LDS each) run the 256-wave
+// slice sweep in 180 us against 198 us for one 16-column group of 8
+// wavefronts; the halves of each 128-B line that neighbouring groups read meet
+// in L2 because neighbouring groups run on one XCD.  At 256 the 16-column tile
+// is faster; from 1024 on the LDS bound picks the width.
+#ifndef MSX_COL_TW512
+#define MSX_COL_TW512 8
+#endif
 #ifndef MSX_COL_DB
 #define MSX_COL_DB 1
 #endif
@@ -33,7 +42,7 @@ constexpr int row_lpw() { return kT >= 256 ? 1 : 256 / kT; }
 #define MSX_COL_THREADS 512
 #endif
 template <typename C> constexpr int col_tw() {
-    int tw = MSX_COL_TW;
+    int tw = kN == 512 ? MSX_COL_TW512 : MSX_COL_TW;
     while (tw > 1 && (kTc * tw > MSX_COL_THREADS || (long)kN * (tw + 1) * (long)sizeof(C) > 80 * 1024)) tw /= 2;
     return tw;
 }
