@@ -150,7 +150,7 @@ def surrogate_rate(msx, torch, ctx, value):
                            "input_4config_rel_rms_mean": round(ev["eval"]["input_rel_rms_mean"], 4),
                            "fraction_within_2pct": round(ev["eval"]["fraction_within_2pct_surrogate"], 3),
                            "setup": "SrTiO3/Si/Au-on-aC, 5-40 nm, 128^2 grid, 32x32 ADF scans (40 mrad to the "
-                                    "band limit), judged against plain 64-config multislice"}
+                                    "band limit), measured against plain 64-config multislice"}
     return out
 
 

@@ -1,7 +1,7 @@
 #!/bin/bash
 # Build the reference's own CPU code into oracle/_ref/ (test infrastructure
 # only; outputs are git-ignored).  Needs the reference tree, so it runs in the
-# development container, never on a GPU box (which uses the prebuilt files).
+# development container, never where the reference sources are absent (the prebuilt files are used there).
 #   libkirkland_ref.so : ImageSimulation/multislice/slicelib.cpp + ref_harness.cpp
 #   libfftw2f_ref.so   : ImageSimulation/fftw-2.1.5/fftw/*.c (float mode)
 #                        + ref_fftw2_harness.c

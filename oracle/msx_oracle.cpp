@@ -38,6 +38,19 @@
  *   probe / detector: Parseval normalisation, integer shift = circular
  *       shift, annulus partition sums to the total.
  *   ctf: closed-form chi, aperture mask.
+ *   illum_angles: the Gauss circle count of lattice points (exact integer
+ *       arithmetic) for cone radii away from lattice points; rings as
+ *       differences of disks.
+ *   defocus_steps: closed-form Gaussian weights, step count floor(2.5 s/d).
+ *   chi_astig: the compiled reference's multipole chi() (slicelib.cpp:
+ *       258-318, multiMode 1) with C12a = -dfa2 sin(2 phi2), C12b = dfa2
+ *       cos(2 phi2), C23a = -dfa3 sin(3 phi3), C23b = dfa3 cos(3 phi3), times
+ *       2 pi/lambda -- another parametrisation of the same function.
+ *   partial_image: one angle and one defocus reduce it to ctf_image of the
+ *       coherent exit wave; vacuum gives 1; the defocus sum equals the
+ *       weighted mean of single-defocus images; a four-angle ring equals
+ *       the mean of coherent images of incident waves built from the
+ *       compiled reference's freqn positions.
  *   potential_rec (Doyle-Turner, reciprocal space): one atom at the origin
  *       gives the real radial f_e(|g|/2) exp(-B|g|^2/4)/area at every
  *       reflection; translation theorem; Hermitian symmetry without
@@ -774,6 +787,162 @@ ORC_API void orc_ctf_image_f64(const double *wave, int nx, int ny, double ax, do
     if (img_wave) std::memcpy(img_wave, w.data(), sizeof(double) * 2 * nx * ny);
     if (image)
         for (size_t j = 0; j < (size_t)nx * ny; j++) image[j] = std::norm(w[j]);
+}
+
+/* ------------------------------------------------------------------ */
+/* Partial-coherence TEM image (mulslice.cpp:539-676)                  */
+/* ------------------------------------------------------------------ */
+/* Illumination angles, mulslice.cpp:556-583, in the reference's float
+ * arithmetic (the discrete choices -- how many angles, which defocus steps --
+ * must come out identical):  rx = 1/ax, nacx = (int)(acmax/(wavlen rx) + 1.5),
+ * q2max = (acmax/wavlen)^2, q2min = (acmin/wavlen)^2; loop iqy outer,
+ * iqx inner over [-nac, nac]; keep q2 = (iqx rx)^2 + (iqy ry)^2 with
+ * q2min <= q2 <= q2max.  Writes (iqx, iqy) pairs in loop order; returns
+ * the count (entries beyond maxn are counted, not written). */
+ORC_API int orc_illum_angles(int nx, int ny, double ax, double by, double kev, double acmin, double acmax,
+                             int *iq, int maxn) {
+    (void)nx;
+    (void)ny;
+    const float axf = (float)ax, byf = (float)by;
+    const float wavlen = (float)wavelength((double)(float)kev);
+    const float rx = 1.0F / axf, ry = 1.0F / byf;
+    const float acmaxf = (float)acmax, acminf = (float)acmin;
+    const int nacx = (int)((acmaxf / (wavlen * rx)) + 1.5F);
+    const int nacy = (int)((acmaxf / (wavlen * ry)) + 1.5F);
+    float q2max = acmaxf / wavlen;
+    q2max = q2max * q2max;
+    float q2min = acminf / wavlen;
+    q2min = q2min * q2min;
+    int n = 0;
+    for (int iqy = -nacy; iqy <= nacy; iqy++) {
+        const float qy = iqy * ry;
+        const float qy2 = qy * qy;
+        for (int iqx = -nacx; iqx <= nacx; iqx++) {
+            const float qx = iqx * rx;
+            const float q2 = qx * qx + qy2;
+            if ((q2 <= q2max) && (q2 >= q2min)) {
+                if (n < maxn) {
+                    iq[2 * n] = iqx;
+                    iq[2 * n + 1] = iqy;
+                }
+                n++;
+            }
+        }
+    }
+    return n;
+}
+
+/* Defocus steps and weights, mulslice.cpp:556,617-645: ndf =
+ * (int)(2.5 sigmaf / dfdelt) in float; df_i = df0 + i dfdelt for
+ * i = -ndf..ndf with weight exp(-0.5 ((df_i - df0)/sigmaf)^2) (evaluated in
+ * double here).  sigmaf <= 0 gives the single step df0 with weight 1 (the
+ * reference divides 0 by 0 there).  Writes offsets i*dfdelt and weights;
+ * returns 2 ndf + 1, or -1 for dfdelt <= 0 with sigmaf > 0. */
+ORC_API int orc_defocus_steps(double sigmaf, double dfdelt, double *offset, double *weight, int maxn) {
+    if (!(sigmaf > 0)) {
+        if (maxn > 0) {
+            offset[0] = 0.0;
+            weight[0] = 1.0;
+        }
+        return 1;
+    }
+    if (!(dfdelt > 0)) return -1;
+    const int ndf = (int)((2.5F * (float)sigmaf) / (float)dfdelt);
+    int n = 0;
+    for (int idf = -ndf; idf <= ndf; idf++, n++) {
+        if (n >= maxn) continue;
+        const double x = (idf * dfdelt) / sigmaf;
+        offset[n] = idf * dfdelt;
+        weight[n] = std::exp(-0.5 * x * x);
+    }
+    return n;
+}
+
+/* Aberration phase of the partial-coherence lens, mulslice.cpp:620-628:
+ *   chi = pi lambda k^2 (0.5 Cs lambda^2 k^2 - df
+ *         + dfa2 sin(2 (phi - dfa2phi)) + 2 dfa3 lambda |k| sin(3 (phi - dfa3phi)) / 3),
+ * phi = atan2(ky, kx).  FP64. */
+inline double chi_astig(double kx, double ky, double lam, double cs, double df, double dfa2, double dfa2phi,
+                        double dfa3, double dfa3phi) {
+    const double k2 = kx * kx + ky * ky;
+    const double phi = std::atan2(ky, kx);
+    return M_PI * lam * k2 *
+           (0.5 * cs * lam * lam * k2 - df + dfa2 * std::sin(2.0 * (phi - dfa2phi)) +
+            2.0 * dfa3 * lam * std::sqrt(k2) * std::sin(3.0 * (phi - dfa3phi)) / 3.0);
+}
+ORC_API double orc_chi_astig(double kx, double ky, double kev, double cs, double df, double dfa2, double dfa2phi,
+                             double dfa3, double dfa3phi) {
+    return chi_astig(kx, ky, wavelength(kev), cs, df, dfa2, dfa2phi, dfa3, dfa3phi);
+}
+
+/* The partial-coherence image, FP64 (mulslice.cpp:539-676): for every
+ * illumination angle (iqx, iqy) the incident wave
+ *   exp(2 pi i (qx xpos[ix] + qy ypos[iy])),  qx = iqx/ax,
+ *   xpos[ix] = ix ax/(nx-1)  (freqn, slicelib.cpp:433-450: note nx-1),
+ * runs through the coherent slice loop; its spectrum Psi = fft(psi) is then,
+ * for every defocus step, multiplied by exp(-i chi) inside the objective
+ * aperture (kx2+ky2 <= (aobj/lambda)^2, float sum as the reference) and
+ * transformed back, and w_i |ifft(...)|^2 is summed.  The sum is divided by
+ * (number of angles) * (sum of weights).  image: nx*ny doubles.  Returns the
+ * number of angles (0: none inside the illumination cone; -1: bad input). */
+ORC_API int orc_partial_image_f64(const double *trans, int nlayer, const double *cz, const int *layer, int nslice,
+                                  int nx, int ny, double ax, double by, double kev, double ctiltx, double ctilty,
+                                  double bw, double cs, double df0, double aobj, double dfa2, double dfa2phi,
+                                  double dfa3, double dfa3phi, double acmin, double acmax, double sigmaf,
+                                  double dfdelt, double *image) {
+    if (nx < 2 || ny < 2) return -1;
+    const int nill = orc_illum_angles(nx, ny, ax, by, kev, acmin, acmax, nullptr, 0);
+    std::vector<int> iq(2 * (size_t)std::max(nill, 1));
+    orc_illum_angles(nx, ny, ax, by, kev, acmin, acmax, iq.data(), nill);
+    const int ndfs = orc_defocus_steps(sigmaf, dfdelt, nullptr, nullptr, 0);
+    if (ndfs < 1) return -1;
+    std::vector<double> off(ndfs), wt(ndfs);
+    orc_defocus_steps(sigmaf, dfdelt, off.data(), wt.data(), ndfs);
+    double wsum = 0;
+    for (double w : wt) wsum += w;
+    Freq f(nx, ny, ax, by);
+    const double lam = wavelength(kev);
+    const float k2maxo = (float)((aobj / lam) * (aobj / lam));
+    const size_t img = (size_t)nx * ny;
+    std::vector<double> pix(img, 0.0);
+    std::vector<std::complex<double>> w(img), Psi(img);
+    for (int a = 0; a < nill; a++) {
+        const long iqx = iq[2 * a], iqy = iq[2 * a + 1];
+        for (int ix = 0; ix < nx; ix++)
+            for (int iy = 0; iy < ny; iy++) {
+                // 2 pi (iqx ix/(nx-1) + iqy iy/(ny-1)), reduced exactly in integers
+                const long mx = ((iqx * ix) % (nx - 1) + (nx - 1)) % (nx - 1);
+                const long my = ((iqy * iy) % (ny - 1) + (ny - 1)) % (ny - 1);
+                const double t = 2.0 * M_PI * ((double)mx / (nx - 1) + (double)my / (ny - 1));
+                w[(size_t)ix * ny + iy] = std::complex<double>(std::cos(t), std::sin(t));
+            }
+        if (orc_multislice_f64(trans, nlayer, cz, layer, nslice, nx, ny, ax, by, kev, ctiltx, ctilty, bw,
+                               reinterpret_cast<double *>(w.data()), nullptr) != 0)
+            return -1;
+        fft2d(w.data(), nx, ny, +1);
+        for (int d = 0; d < ndfs; d++) {
+            const double df = df0 + off[d];
+            for (int ix = 0; ix < nx; ix++) {
+                const double kx = (double)freq_index(ix, nx) / ax;
+                for (int iy = 0; iy < ny; iy++) {
+                    const double ky = (double)freq_index(iy, ny) / by;
+                    const size_t j = (size_t)ix * ny + iy;
+                    if (f.kx2[ix] + f.ky2[iy] <= k2maxo) {
+                        const double c = chi_astig(kx, ky, lam, cs, df, dfa2, dfa2phi, dfa3, dfa3phi);
+                        Psi[j] = w[j] * std::complex<double>(std::cos(c), -std::sin(c));
+                    } else {
+                        Psi[j] = 0;
+                    }
+                }
+            }
+            fft2d(Psi.data(), nx, ny, -1);
+            scale_inv(Psi.data(), nx, ny);
+            for (size_t j = 0; j < img; j++) pix[j] += wt[d] * std::norm(Psi[j]);
+        }
+    }
+    const double sc = nill > 0 ? 1.0 / ((double)nill * wsum) : 0.0;
+    for (size_t j = 0; j < img; j++) image[j] = pix[j] * sc;
+    return nill;
 }
 
 /* ------------------------------------------------------------------ */

@@ -32,5 +32,19 @@ REF_API double ref_chi_csdf(double cs, double df, double alx, double aly) {
     p[pDEFOCUS] = (float)df;
     return chi(p, alx, aly, 0);
 }
+/* chi with Cs, defocus and the multipole two- and three-fold astigmatism
+ * coefficients (multiMode 1): slicelib.cpp:258-318 */
+REF_API double ref_chi_multi(double cs, double df, double c12a, double c12b, double c23a, double c23b, double alx,
+                             double aly) {
+    float p[64];
+    for (int i = 0; i < 64; i++) p[i] = 0.0f;
+    p[pCS] = (float)cs;
+    p[pDEFOCUS] = (float)df;
+    p[pC12a] = (float)c12a;
+    p[pC12b] = (float)c12b;
+    p[pC23a] = (float)c23a;
+    p[pC23b] = (float)c23b;
+    return chi(p, alx, aly, 1);
+}
 REF_API double ref_rangauss(unsigned long *seed) { return rangauss(seed); }
 REF_API double ref_ranflat(unsigned long *seed) { return ranflat(seed); }

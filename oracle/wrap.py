@@ -54,6 +54,11 @@ def lib():
             ("orc_potential_rs_dt", None, [d, d, d, d, i, i, l, _dp, _dp, _dp, _dp, _dp, _ip, i, _dp]),
             ("orc_potential_rec_curve", None, [d, d, d, i, i, i, i, l, _dp, _dp, _dp, _dp, _ip, _dp, i, d, _dp]),
             ("orc_potential_rs_curve", None, [d, d, d, d, i, i, l, _dp, _dp, _dp, _dp, _ip, _dp, i, d, i, _dp]),
+            ("orc_illum_angles", i, [i, i, d, d, d, d, d, C.c_void_p, i]),
+            ("orc_defocus_steps", i, [d, d, C.c_void_p, C.c_void_p, i]),
+            ("orc_chi_astig", d, [d, d, d, d, d, d, d, d, d]),
+            ("orc_partial_image_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, d, d, d, d, d, d, d,
+                                          d, d, d, d, _dp]),
         ]:
             fn = getattr(L, name)
             fn.restype = res
@@ -77,6 +82,7 @@ def ref():
             ("ref_vzatom", d, [i, d]), ("ref_featom", d, [i, d]),
             ("ref_freqn", None, [_fp, _fp, _fp, i, d]),
             ("ref_chi_csdf", d, [d, d, d, d]),
+            ("ref_chi_multi", d, [d, d, d, d, d, d, d, d]),
             ("ref_rangauss", d, [C.POINTER(C.c_ulong)]),
             ("ref_ranflat", d, [C.POINTER(C.c_ulong)]),
         ]:
@@ -260,6 +266,45 @@ class Oracle:
         lib().orc_ctf_image_f64(w.view(np.float64).reshape(-1), nx, ny, ax, by, kev, cs, df, aobj,
                                 img_w.ctypes.data, img.ctypes.data)
         return img_w, img
+
+    @staticmethod
+    def illum_angles(nx, ny, ax, by, kev, acmin, acmax):
+        """(iqx, iqy) pairs of the partial-coherence illumination integral, in the reference's loop order."""
+        n = lib().orc_illum_angles(nx, ny, ax, by, kev, acmin, acmax, None, 0)
+        iq = np.zeros((max(n, 1), 2), np.int32)
+        lib().orc_illum_angles(nx, ny, ax, by, kev, acmin, acmax, iq.ctypes.data, n)
+        return iq[:n]
+
+    @staticmethod
+    def defocus_steps(sigmaf, dfdelt):
+        """(offsets from df0, weights) of the defocus integral."""
+        n = lib().orc_defocus_steps(sigmaf, dfdelt, None, None, 0)
+        if n < 1:
+            raise ValueError("dfdelt must be > 0")
+        off = np.zeros(n)
+        w = np.zeros(n)
+        lib().orc_defocus_steps(sigmaf, dfdelt, off.ctypes.data, w.ctypes.data, n)
+        return off, w
+
+    @staticmethod
+    def chi_astig(kx, ky, kev, cs, df, dfa2=0.0, dfa2phi=0.0, dfa3=0.0, dfa3phi=0.0):
+        return lib().orc_chi_astig(kx, ky, kev, cs, df, dfa2, dfa2phi, dfa3, dfa3phi)
+
+    @staticmethod
+    def partial_image(trans, cz, layer, ax, by, kev, lens, acmin, acmax, sigmaf, dfdelt, ctilt=(0.0, 0.0), bw=2.0 / 3.0):
+        """Partial-coherence image (FP64); lens: dict(cs, df, aobj, dfa2, dfa2phi, dfa3, dfa3phi).
+        Returns (image, number of illumination angles)."""
+        t = _c128(trans)
+        nl, nx, ny = t.shape
+        img = np.zeros((nx, ny), np.float64)
+        g = lambda k: float(lens.get(k, 0.0))  # noqa: E731
+        n = lib().orc_partial_image_f64(t.view(np.float64).reshape(-1), nl, np.ascontiguousarray(cz, np.float64),
+                                        np.ascontiguousarray(layer, np.int32), len(layer), nx, ny, ax, by, kev,
+                                        ctilt[0], ctilt[1], bw, g("cs"), g("df"), g("aobj"), g("dfa2"), g("dfa2phi"),
+                                        g("dfa3"), g("dfa3phi"), acmin, acmax, sigmaf, dfdelt, img.reshape(-1))
+        if n < 0:
+            raise ValueError("orc_partial_image_f64 rejected its input")
+        return img, n
 
     @staticmethod
     def dt_fe(Z, s):

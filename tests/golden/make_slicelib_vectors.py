@@ -47,6 +47,12 @@ def main():
     out["chi"] = [[cs, df, ax, ay, R.ref_chi_csdf(cs, df, ax, ay)]
                   for cs in (0.0, 1.2e7, 1e4) for df in (0.0, -600.0, 300.0)
                   for ax, ay in ((0.01, 0.0), (0.003, -0.004), (0.021, 0.001))]
+    out["chi_multi"] = [[cs, df, c12a, c12b, c23a, c23b, ax, ay,
+                         R.ref_chi_multi(cs, df, c12a, c12b, c23a, c23b, ax, ay)]
+                        for cs, df in ((1.2e7, -600.0), (0.0, 0.0))
+                        for c12a, c12b, c23a, c23b in ((0.0, 0.0, 0.0, 0.0), (-35.3, 48.5, 0.0, 0.0),
+                                                       (0.0, 0.0, 812.0, -1410.0), (12.0, -7.0, -300.0, 95.0))
+                        for ax, ay in ((0.01, 0.0), (0.003, -0.004), (-0.012, 0.007), (0.0, -0.015))]
     seed = C.c_ulong(12345)
     out["rangauss"] = {"seed": 12345, "values": [R.ref_rangauss(C.byref(seed)) for _ in range(16)]}
     rng = np.random.default_rng(7)

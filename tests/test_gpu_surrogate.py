@@ -5,10 +5,10 @@ suite: 128 random Si [001] specimens 5-10 nm thick (random tilt <= 10 mrad,
 random scan window), 128^2 grid, 32x32 ADF scans (40 mrad to the band limit).
 A 3-level, 16-channel U-Net is trained on the GPU by the library's kernels on
 engine-generated pairs (4-config mean -> mean of 8 further configs, 2000 Adam
-steps) and judged on 12 held-out specimens against plain 32-config
+steps) and measured on 12 held-out specimens against plain 32-config
 multislice.  With per-config noise sigma, the 4-config mean alone errs by
 0.47 sigma and an ideal estimator of the infinite mean by 0.17 sigma (the
-32-config judge's own scatter).  Measured: the surrogate removes about a third
+32-config mean's own scatter).  Measured: the surrogate removes about a third
 of the input's error (ratio 0.66); the test requires a ratio below 0.85.
 """
 import numpy as np
