@@ -138,6 +138,17 @@ MSX_API int msx_displace(msx_ctx ctx, const msx_atom* in, int64_t natoms, const 
  * for known-answer tests of the generator. */
 MSX_API void msx_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* ---- specimen generator (no reference counterpart) ---------------------- */
+/* Amorphous specimen for synthetic workloads (the uneven-job set: amorphous
+ * carbon at a given density with a minimum spacing): random sequential
+ * addition of up to natoms atoms of element Z, uniform in the periodic box
+ * a x b x c (Angstrom), each at least min_dist from every earlier one
+ * (minimum image); stops after max_candidates draws.  Deterministic in seed
+ * (Philox-4x32-10 per candidate).  out: host array of natoms; *nplaced
+ * receives the count placed.  Host only (no context). */
+MSX_API int msx_amorphous(double a, double b, double c, int32_t Z, int64_t natoms, double min_dist, uint64_t seed,
+                          int64_t max_candidates, msx_atom* out, int64_t* nplaced);
+
 /* ---- 2D FFT ----------------------------------------------------------- */
 /* cfpix::fft() / cfpix::ifft() (cfpix.cpp:115-145), batched: sign = +1 is
  * fft() (exponent +i, unscaled), sign = -1 is ifft() (exponent -i, scaled by
@@ -304,6 +315,34 @@ MSX_API int msx_stem(msx_ctx ctx, const msx_stack* stack, const void* trans, con
 MSX_API int msx_ctf_image(msx_ctx ctx, const void* psi, int nwave, int nx, int ny, double ax, double by,
                           double kev, double cs, double df, double aobj, void* psi_img, void* intensity,
                           int prec, int where);
+
+/* ---- partial-coherence TEM image ------------------------------------------ */
+/* Objective lens with the astigmatism terms of mulslice.cpp:620-628:
+ * chi = pi lambda k^2 (0.5 Cs lambda^2 k^2 - df + dfa2 sin(2(phi - dfa2phi))
+ *       + 2 dfa3 lambda |k| sin(3(phi - dfa3phi))/3), phi = atan2(ky, kx);
+ * Cs, df, dfa2, dfa3 in Angstrom, angles and aobj in rad. */
+typedef struct msx_lens {
+    double cs, df, aobj;
+    double dfa2, dfa2phi;
+    double dfa3, dfa3phi;
+} msx_lens;
+
+/* Replaces Kirkland mulslice's partial-coherence branch (mulslice.cpp:
+ * 539-676, lpartl == 1): the image integrated over illumination angles
+ * (acmin..acmax, rad) and a Gaussian defocus spread (sigmaf, sampled every
+ * dfdelt over +-2.5 sigmaf; Angstrom).  Every angle (iqx/ax, iqy/by) of the
+ * grid inside the cone starts a wave exp(2 pi i (qx xpos + qy ypos)) (freqn
+ * positions, xpos = ix ax/(nx-1)); the waves run `batch` at a time (0 =
+ * choose) through the slice loop of msx_multislice; each exit wave is imaged
+ * through the lens at every defocus step and the weighted intensities are
+ * averaged:  image = sum_angles sum_steps w |psi_img|^2 / (nangles sum w).
+ * The angle set and step count follow the reference's float arithmetic.
+ * sigmaf == 0 images at df alone (the reference divides 0 by 0 there).
+ * image: nx*ny doubles (host or device per `where`, like trans).  nangles
+ * and nsteps (may be NULL) receive the number of angles and defocus steps. */
+MSX_API int msx_partial_coherence(msx_ctx ctx, const msx_stack* stack, const void* trans, const msx_lens* lens,
+                                  double acmin, double acmax, double sigmaf, double dfdelt, int batch,
+                                  double* image, int32_t* nangles, int32_t* nsteps, int prec, int where);
 
 /* ---- frozen-phonon surrogate (no reference counterpart) --------------------
  * A U-Net that maps the mean ADF image of a few frozen-phonon configurations

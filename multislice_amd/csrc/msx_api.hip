@@ -289,9 +289,11 @@ template <typename C> ColArgs<C> col_args(C* psi, int ny, int nb, const C* tw) {
 // Algorithmic HBM bytes of one sweep launch: every wave pixel read and
 // written once (unless not stored), plus one shared transmission image.
 template <typename C> double row_bytes(const RowArgs<C>& a, int n, double band) {
+    using Re = typename real_of<C>::type;
     const double px = (double)a.nx * n;
-    const double fin = (a.prune & 1) ? band : 1.0, fout = (a.prune & 2) ? band : 1.0;
+    const double fin = (a.prune & 1) ? band : 1.0, fout = a.nostore ? 0.0 : ((a.prune & 2) ? band : 1.0);
     return px * a.nb * sizeof(C) * (fin + fout) + (a.trans ? px * sizeof(C) : 0.0) +
+           (a.inten ? px * a.nb * sizeof(Re) * (a.iacc == 2 ? 2.0 : 1.0) : 0.0) +
            (a.trans_tab ? px * a.nb * sizeof(C) : 0.0) +
            (a.vreal ? px * a.nb * (0.5 * (double)sizeof(C) - (double)sizeof(C)) : 0.0);
 }
@@ -389,10 +391,13 @@ struct DetPlan {
 // (or src is a shared incident wave), 1 = psi holds k-space probes already
 // through the column inverse (the first row sweep starts with IFFT_y).
 // With det != nullptr the loop stops after the last checkpoint; otherwise
-// it ends with the final inverse row sweep (exit waves in real space).
+// it ends with the final inverse row sweep (exit waves in real space), or,
+// with mixed_out, just before it: psi then holds IFFT_x(P FFT_x FFT_y(t psi))
+// of the last slice, i.e. FFT_y of the exit wave times nx, in the columns
+// ky2 < k2max (the others are not written).
 template <typename C>
 int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-                    const C* src, long long src_bstride, int start, DetPlan* det) {
+                    const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false) {
     using Re = typename real_of<C>::type;
     const int nx = st->nx, ny = st->ny;
     const long long img = (long long)nx * ny;
@@ -465,7 +470,7 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         }
         if (last) return MSX_OK;
     }
-    if (!det) {
+    if (!det && !mixed_out) {
         RowArgs<C> r = row_args<C>(psi, nx, nb, twy);
         r.scale = inv;
         r.ky2 = tb.ky2;
@@ -483,11 +488,12 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
 // that fails falls back to launching the loop directly.
 template <typename C>
 int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-               const C* src, long long src_bstride, int start, DetPlan* det) {
-    if (!ctx->graphs || ctx->prof) return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+               const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false) {
+    if (!ctx->graphs || ctx->prof)
+        return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
     ctx->gkey.clear();
     ctx->keying = true;
-    const int rk = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+    const int rk = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
     ctx->keying = false;
     CHK(rk);
     auto it = ctx->gcache.find(ctx->gkey);
@@ -496,9 +502,9 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             (void)hipGetLastError();  // e.g. the legacy default stream: launch directly
-            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
         }
-        const int rc = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+        const int rc = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         hipGraphExec_t ex = nullptr;
         hipError_t ei = (rc == MSX_OK && ec == hipSuccess && g) ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0)
@@ -506,7 +512,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         if (g) hipGraphDestroy(g);
         if (ei != hipSuccess) {
             (void)hipGetLastError();
-            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det);
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
         }
         it = ctx->gcache.emplace(ctx->gkey, ex).first;
         ctx->graph_captures++;
@@ -935,6 +941,152 @@ int ctf_impl(msx_ctx ctx, const void* psi, int nwave, int nx, int ny, double ax,
         if (psi_img) HIPCHK(hipMemcpyAsync(psi_img, w, wbytes, hipMemcpyDeviceToHost, ctx->stream));
         if (inten) HIPCHK(hipMemcpyAsync(inten, i_dev, ibytes, hipMemcpyDeviceToHost, ctx->stream));
     }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+// Illumination angles (iqx, iqy) of the partial-coherence integral in the
+// float arithmetic of mulslice.cpp:556-583 (loop iqy outer, iqx inner).
+std::vector<int2> illum_angles(const msx_stack* st, double acmin, double acmax) {
+    const float wavlen = (float)msx_wavelength((double)(float)st->kev);
+    const float rx = 1.0F / (float)st->ax, ry = 1.0F / (float)st->by;
+    const float acmaxf = (float)acmax, acminf = (float)acmin;
+    const int nacx = (int)((acmaxf / (wavlen * rx)) + 1.5F);
+    const int nacy = (int)((acmaxf / (wavlen * ry)) + 1.5F);
+    float q2max = acmaxf / wavlen;
+    q2max = q2max * q2max;
+    float q2min = acminf / wavlen;
+    q2min = q2min * q2min;
+    std::vector<int2> out;
+    for (int iqy = -nacy; iqy <= nacy; iqy++) {
+        const float qy = iqy * ry;
+        const float qy2 = qy * qy;
+        for (int iqx = -nacx; iqx <= nacx; iqx++) {
+            const float qx = iqx * rx;
+            const float q2 = qx * qx + qy2;
+            if ((q2 <= q2max) && (q2 >= q2min)) out.push_back(make_int2(iqx, iqy));
+        }
+    }
+    return out;
+}
+
+template <typename C>
+int partial_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_lens* lens, double acmin,
+                 double acmax, double sigmaf, double dfdelt, int batch, double* image, int32_t* nangles,
+                 int32_t* nsteps, int where) {
+    using Re = typename real_of<C>::type;
+    const int nx = st->nx, ny = st->ny;
+    const long long img = (long long)nx * ny;
+    const std::vector<int2> iq = illum_angles(st, acmin, acmax);
+    if (iq.empty()) return fail(MSX_EINVAL, "no illumination angle of the grid lies in [acmin, acmax]");
+    // defocus steps (mulslice.cpp:556,617-645): ndf from the float expression,
+    // weights exp(-0.5 ((df - df0)/sigmaf)^2)
+    std::vector<double> off, wt;
+    if (sigmaf > 0) {
+        const int ndf = (int)((2.5F * (float)sigmaf) / (float)dfdelt);
+        for (int i = -ndf; i <= ndf; i++) {
+            const double x = (i * dfdelt) / sigmaf;
+            off.push_back(i * dfdelt);
+            wt.push_back(std::exp(-0.5 * x * x));
+        }
+    } else {
+        off.push_back(0.0);
+        wt.push_back(1.0);
+    }
+    double wsum = 0.0;
+    for (double w : wt) wsum += w;
+    const int nang = (int)iq.size();
+    if (nangles) *nangles = nang;
+    if (nsteps) *nsteps = (int)off.size();
+
+    const msx_tuning* tu = ctx->tuning(nx, ny, sizeof(Re) == 8 ? MSX_F64 : MSX_F32);
+    if (batch <= 0 && tu && tu->batch > 0) batch = tu->batch;
+    if (batch <= 0) {
+        // two wave buffers and one intensity buffer: about 2 GiB in all
+        long long b = (2LL << 30) / (long long)((2 * sizeof(C) + sizeof(Re)) * img);
+        batch = (int)std::max(1LL, std::min<long long>(b, 256));
+    }
+    batch = std::min(batch, nang);
+    const C* t_dev = static_cast<const C*>(trans);
+    if (where == MSX_HOST) {
+        const size_t tbytes = sizeof(C) * img * st->nlayer;
+        CHK(ctx->trans.ensure(tbytes));
+        HIPCHK(hipMemcpyAsync(ctx->trans.p, trans, tbytes, hipMemcpyHostToDevice, ctx->stream));
+        t_dev = static_cast<const C*>(ctx->trans.p);
+    }
+    CHK(ctx->wave.ensure(sizeof(C) * img * batch));
+    CHK(ctx->wave2.ensure(sizeof(C) * img * batch));
+    CHK(ctx->vtmp.ensure(sizeof(Re) * img * batch));
+    C* w = static_cast<C*>(ctx->wave.p);
+    C* w2 = static_cast<C*>(ctx->wave2.p);
+    Re* ib = static_cast<Re*>(ctx->vtmp.p);
+    const size_t ibytes = sizeof(double) * img, iq_off = (ibytes + 255) & ~(size_t)255;
+    CHK(ctx->outbuf.ensure(iq_off + sizeof(int2) * iq.size()));
+    double* pix = where == MSX_DEVICE ? image : static_cast<double*>(ctx->outbuf.p);
+    int2* iq_dev = reinterpret_cast<int2*>(static_cast<char*>(ctx->outbuf.p) + iq_off);
+    HIPCHK(hipMemcpyAsync(iq_dev, iq.data(), sizeof(int2) * iq.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemsetAsync(pix, 0, ibytes, ctx->stream));
+
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, st, true, &tb));
+    const C *twx, *twy;
+    CHK(ctx->twiddles<C>(nx, &twx));
+    CHK(ctx->twiddles<C>(ny, &twy));
+    const double lam = msx_wavelength(st->kev);
+    const float kap_hi = (float)((lens->aobj / lam) * (lens->aobj / lam));
+    // columns with ky2 > kap_hi are zero after the lens and those with
+    // ky2 >= k2max were not written by the slice loop: neither is touched
+    const float k2p = std::min(tb.k2max, std::nextafter(kap_hi, INFINITY));
+    // fractions of live columns (for the profiler's byte counts)
+    const double band_slice = ctx->band, band_lens = band_fraction(k2_table(ny, st->by).data(), ny, k2p);
+    const double scale = 1.0 / ((double)nang * wsum);
+    for (int a0 = 0; a0 < nang; a0 += batch) {
+        const int nb = std::min(batch, nang - a0);
+        HIPCHK(launch_tilt_waves<C>(w, nb, nx, ny, iq_dev + a0, ctx->stream));
+        // exit waves as nx * FFT_y(psi) (the slice loop without its last row sweep)
+        ctx->band = band_slice;
+        CHK(slice_loop<C>(ctx, st, tb, t_dev, w, nb, nullptr, 0, 0, nullptr, true));
+        ctx->band = band_lens;
+        for (size_t d = 0; d < off.size(); d++) {
+            // FFT_x, lens at this defocus, IFFT_x: the spectrum Psi = FFT_x FFT_y psi
+            // is this data over nx
+            ColArgs<C> c = col_args<C>(w, ny, nb, twx);
+            c.out = w2;
+            c.ops = kColCtf | kColStore;
+            c.prune = 1;
+            c.kx2 = tb.kx2;
+            c.ky2 = tb.ky2;
+            c.k2max = k2p;
+            c.ax = st->ax;
+            c.by = st->by;
+            c.chi1 = M_PI * lam;
+            c.chi2 = 0.5 * lens->cs * lam * lam;
+            c.df = lens->df + off[d];
+            c.kap_hi = kap_hi;
+            if (lens->dfa2 != 0.0 || lens->dfa3 != 0.0) {
+                c.astig = 1;
+                c.a2 = lens->dfa2;
+                c.phi2 = lens->dfa2phi;
+                c.a3 = 2.0 * lens->dfa3 * lam / 3.0;
+                c.phi3 = lens->dfa3phi;
+            }
+            LAUNCH_COL(nx, kColSlice, c);
+            // IFFT_y with the 1/(nx ny) of ifft and the 1/nx above; weighted
+            // intensity summed over the steps per wave
+            RowArgs<C> r = row_args<C>(w2, nx, nb, twy);
+            r.scale = (Re)1 / ((Re)nx * (Re)nx * (Re)ny);
+            r.ky2 = tb.ky2;
+            r.k2max = k2p;
+            r.prune = 1;
+            r.inten = ib;
+            r.iacc = d == 0 ? 1 : 2;
+            r.iw = (Re)wt[d];
+            r.nostore = 1;
+            LAUNCH_ROW(ny, kRowInv, r);
+        }
+        HIPCHK(launch_wave_sum<Re>(ib, nb, img, scale, pix, ctx->stream));
+    }
+    if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(image, pix, ibytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MSX_OK;
 }
@@ -1656,6 +1808,28 @@ MSX_API int msx_stem(msx_ctx ctx, const msx_stack* stack, const void* trans, con
                                   batch, where);
     return stem_impl<float2>(ctx, stack, trans, probe, pos, nprobe, det, ndet, ck, nck, weight, accumulate, out, batch,
                              where);
+}
+
+MSX_API int msx_partial_coherence(msx_ctx ctx, const msx_stack* stack, const void* trans, const msx_lens* lens,
+                                  double acmin, double acmax, double sigmaf, double dfdelt, int batch, double* image,
+                                  int32_t* nangles, int32_t* nsteps, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_stack(stack));
+    if (!trans || !lens || !image) return fail(MSX_EINVAL, "trans, lens and image must be set");
+    if (stack->nx < 2 || stack->ny < 2) return fail(MSX_ESIZE, "partial coherence needs nx, ny >= 2");
+    if (!(acmin >= 0) || !(acmax >= acmin)) return fail(MSX_EINVAL, "need 0 <= acmin <= acmax");
+    if (!(lens->aobj >= 0)) return fail(MSX_EINVAL, "aobj must be >= 0");
+    if (!(sigmaf >= 0)) return fail(MSX_EINVAL, "sigmaf must be >= 0");
+    if (sigmaf > 0 && !(dfdelt > 0)) return fail(MSX_EINVAL, "dfdelt must be > 0 when sigmaf > 0");
+    if (sigmaf > 0 && (2.5 * sigmaf / dfdelt) > 4096) return fail(MSX_EINVAL, "more than 8193 defocus steps");
+    if (batch < 0) return fail(MSX_EINVAL, "batch must be >= 0");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (prec == MSX_F64)
+        return partial_impl<double2>(ctx, stack, trans, lens, acmin, acmax, sigmaf, dfdelt, batch, image, nangles,
+                                     nsteps, where);
+    return partial_impl<float2>(ctx, stack, trans, lens, acmin, acmax, sigmaf, dfdelt, batch, image, nangles, nsteps,
+                                where);
 }
 
 MSX_API int msx_ctf_image(msx_ctx ctx, const void* psi, int nwave, int nx, int ny, double ax, double by, double kev,

@@ -145,6 +145,13 @@ hipError_t launch_rs_dt(const RsArgs& a, hipStream_t s);
 hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups, double w, double* out,
                              long long out_stride_d, long long probe0, hipStream_t s);
 
+// partial coherence: tilted incident waves (iq: device [nb] (iqx, iqy)) and
+// the FP64 sum of per-wave intensities into one image
+template <typename C>
+hipError_t launch_tilt_waves(C* psi, int nb, int nx, int ny, const int2* iq, hipStream_t s);
+template <typename Re>
+hipError_t launch_wave_sum(const Re* inten, int nb, long long img, double scale, double* image, hipStream_t s);
+
 // error reporting shared by the C ABI units: records the message for
 // msx_last_error() and returns `code`
 int fail(int code, const char* fmt, ...);

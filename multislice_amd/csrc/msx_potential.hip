@@ -359,4 +359,57 @@ hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups
     return hipGetLastError();
 }
 
+// Tilted incident waves of the partial-coherence integral (mulslice.cpp:
+// 585-589): psi_b(ix, iy) = exp(2 pi i (qx xpos[ix] + qy ypos[iy])) with
+// qx = iqx/ax and freqn's xpos[ix] = ix ax/(nx-1), i.e. the phase
+// 2 pi (iqx ix/(nx-1) + iqy iy/(ny-1)); the products are reduced modulo
+// n-1 in integers, so the FP64 phase is below 4 pi.
+template <typename C>
+__global__ void tilt_wave_kernel(C* psi, int nb, int nx, int ny, const int2* iq) {
+    using Re = typename real_of<C>::type;
+    const long long img = (long long)nx * ny;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= img * nb) return;
+    const int b = (int)(i / img);
+    const long long p = i % img;
+    const int ix = (int)(p / ny), iy = (int)(p % ny);
+    const int2 q = iq[b];
+    const long long mx = (((long long)q.x * ix) % (nx - 1) + (nx - 1)) % (nx - 1);
+    const long long my = (((long long)q.y * iy) % (ny - 1) + (ny - 1)) % (ny - 1);
+    const double twopi = 6.283185307179586476925287;
+    double ph = twopi * ((double)mx / (double)(nx - 1) + (double)my / (double)(ny - 1));
+    ph -= twopi * rint(ph * (1.0 / twopi));
+    double s, c;
+    sincos(ph, &s, &c);
+    psi[i] = mk<C>((Re)c, (Re)s);
+}
+
+template <typename C>
+hipError_t launch_tilt_waves(C* psi, int nb, int nx, int ny, const int2* iq, hipStream_t s) {
+    const long long n = (long long)nx * ny * nb;
+    hipLaunchKernelGGL(tilt_wave_kernel<C>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, psi, nb, nx, ny, iq);
+    return hipGetLastError();
+}
+template hipError_t launch_tilt_waves<float2>(float2*, int, int, int, const int2*, hipStream_t);
+template hipError_t launch_tilt_waves<double2>(double2*, int, int, int, const int2*, hipStream_t);
+
+// image[p] += scale * sum_b inten[b][p], summed in FP64 in wave order
+template <typename Re>
+__global__ void wave_sum_kernel(const Re* inten, int nb, long long img, double scale, double* image) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= img) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += (double)inten[(long long)b * img + p];
+    image[p] += scale * s;
+}
+
+template <typename Re>
+hipError_t launch_wave_sum(const Re* inten, int nb, long long img, double scale, double* image, hipStream_t s) {
+    hipLaunchKernelGGL(wave_sum_kernel<Re>, dim3((unsigned)((img + 255) / 256)), dim3(256), 0, s, inten, nb, img,
+                       scale, image);
+    return hipGetLastError();
+}
+template hipError_t launch_wave_sum<float>(const float*, int, long long, double, double*, hipStream_t);
+template hipError_t launch_wave_sum<double>(const double*, int, long long, double, double*, hipStream_t);
+
 }  // namespace msx

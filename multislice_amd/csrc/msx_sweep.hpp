@@ -66,6 +66,11 @@ template <typename C> struct RowArgs {
     const float* ky2;
     float k2max;
     int prune;
+    // with inten set: iacc 0 writes |psi|^2, 1 writes iw*|psi|^2, 2 adds
+    // iw*|psi|^2 to what inten holds (weighted sums over several launches)
+    int iacc;
+    Re iw;
+    int nostore;  // 1: only the intensity is written, not the wave
 };
 
 // IN/OUT: 0 none, +1 forward (exp +i), -1 inverse (exp -i, unscaled)
@@ -179,11 +184,19 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
         }
         if (p.inten && live) {
             Re* o = p.inten + ((long long)b * p.nx + ix) * N;
+            if (p.iacc == 0) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) o[t + e * T] = a[e].x * a[e].x + a[e].y * a[e].y;
+                for (int e = 0; e < E; ++e) o[t + e * T] = a[e].x * a[e].x + a[e].y * a[e].y;
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const Re v = p.iw * (a[e].x * a[e].x + a[e].y * a[e].y);
+                    o[t + e * T] = p.iacc == 2 ? o[t + e * T] + v : v;
+                }
+            }
         }
         if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(a, t, lds, tw);
-        if (live) {
+        if (live && !p.nostore) {
             C* row = p.psi + ((long long)b * p.nx + ix) * N;
 #pragma unroll
             for (int e = 0; e < E; ++e)
@@ -231,6 +244,11 @@ template <typename C> struct ColArgs {
     double df;
     float kap_lo, kap_hi;  // aperture: kap_lo <= kx2+ky2 <= kap_hi
     Re amp;                // probe amplitude
+    // lens astigmatism (astig != 0): chi gains chi1*k2*(a2 sin(2(phi-phi2)) +
+    // a3 |k| sin(3(phi-phi3))), phi = atan2(ky, kx)
+    double a2, phi2, a3, phi3;
+    int astig;
+    C* out;                // store target (nullptr: in place)
     int ny, nb, ops;
     int wpb;               // waves handled by one work-group (loop with prefetch)
     // prune != 0: columns with ky2[iy] >= k2max are skipped entirely (their
@@ -402,7 +420,13 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
                     if (k2 <= p.kap_hi) {
                         const double kx = (double)freq_index(t + e * T, N) / p.ax;
                         const double kk = kx * kx + ky * ky;
-                        a[e] = cmul(a[e], expi_reduced<C>(-p.chi1 * kk * (p.chi2 * kk - p.df)));
+                        double w = p.chi2 * kk - p.df;
+                        if (p.astig) {
+                            // mulslice.cpp:620-628
+                            const double phi = atan2(ky, kx);
+                            w += p.a2 * sin(2.0 * (phi - p.phi2)) + p.a3 * sqrt(kk) * sin(3.0 * (phi - p.phi3));
+                        }
+                        a[e] = cmul(a[e], expi_reduced<C>(-p.chi1 * kk * w));
                     } else {
                         a[e] = mk<C>((Re)0, (Re)0);
                     }
@@ -446,7 +470,7 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         }
         if constexpr (OUT != 0) fft_line<PL, OUT, WAVE, DB, PL::P - 1>(a, t, lds, tw);
         if ((p.ops & kColStore) && live) {
-            C* col = p.psi + (long long)b * N * p.ny + g * TW + lw;
+            C* col = (p.out ? p.out : p.psi) + (long long)b * N * p.ny + g * TW + lw;
             const unsigned so = (unsigned)t * (unsigned)p.ny;
 #pragma unroll
             for (int e = 0; e < E; ++e) col[so + e * ostep] = a[e];

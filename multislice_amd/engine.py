@@ -74,6 +74,11 @@ class Detector(C.Structure):
     _fields_ = [("k_inner", C.c_double), ("k_outer", C.c_double)]
 
 
+class Lens(C.Structure):
+    _fields_ = [("cs", C.c_double), ("df", C.c_double), ("aobj", C.c_double), ("dfa2", C.c_double),
+                ("dfa2phi", C.c_double), ("dfa3", C.c_double), ("dfa3phi", C.c_double)]
+
+
 PROF_ROW, PROF_COL, PROF_SPLAT, PROF_DET = 0, 1, 2, 3
 
 EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx_graph_stats", "msx_autotune",
@@ -84,7 +89,8 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx
            "msx_unet_create", "msx_unet_destroy", "msx_unet_param_count", "msx_unet_params",
            "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
-           "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve"]
+           "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
+           "msx_partial_coherence", "msx_amorphous"]
 
 _lib = None
 _lock = threading.Lock()
@@ -138,6 +144,8 @@ def load():
         "msx_stem": (i, [vp, C.POINTER(Stack), vp, C.POINTER(Probe), vp, i, C.POINTER(Detector), i, vp, i, d, i,
                          vp, i, i, i]),
         "msx_ctf_image": (i, [vp, vp, i, i, i, d, d, d, d, d, d, vp, vp, i, i]),
+        "msx_partial_coherence": (i, [vp, vp, vp, vp, d, d, d, d, i, vp, vp, vp, i, i]),
+        "msx_amorphous": (i, [d, d, d, C.c_int32, C.c_int64, d, C.c_uint64, C.c_int64, vp, vp]),
         "msx_unet_create": (i, [vp, i, i, C.c_uint64, C.POINTER(vp)]), "msx_unet_destroy": (i, [vp]),
         "msx_unet_param_count": (C.c_int64, [vp]), "msx_unet_params": (i, [vp, vp, i]),
         "msx_unet_set_params": (i, [vp, vp, i]), "msx_unet_forward": (i, [vp, vp, vp, i, i, i, i]),
@@ -437,6 +445,33 @@ class Context:
                                C.c_void_p(out.ctypes.data), int(batch), prec, pt[1]))
         return out
 
+    def partial_coherence(self, trans, cz, layer, ax, by, kev, lens, acmin, acmax, sigmaf, dfdelt,
+                          tilt=(0.0, 0.0), bw=2.0 / 3.0, batch=0):
+        """Partial-coherence image (mulslice.cpp:539-676) of the stack `trans`:
+        illumination angles acmin..acmax (rad), Gaussian defocus spread sigmaf
+        sampled every dfdelt (Angstrom).  lens: dict(cs, df, aobj, dfa2=0,
+        dfa2phi=0, dfa3=0, dfa3phi=0).  Returns (image float64 (nx, ny), number
+        of angles, number of defocus steps); the image is a CUDA tensor when
+        trans is one, else a numpy array."""
+        prec = _prec_of(trans)
+        nl, nx, ny = trans.shape
+        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
+        pt = _buf(trans, prec)
+        g = lambda k: float(lens.get(k, 0.0))  # noqa: E731
+        ln = Lens(g("cs"), g("df"), g("aobj"), g("dfa2"), g("dfa2phi"), g("dfa3"), g("dfa3phi"))
+        if pt[1] == DEVICE:
+            import torch
+            img = torch.empty((nx, ny), dtype=torch.float64, device=trans.device)
+            pimg = C.c_void_p(img.data_ptr())
+        else:
+            img = np.zeros((nx, ny), np.float64)
+            pimg = C.c_void_p(img.ctypes.data)
+        na, nd = C.c_int32(0), C.c_int32(0)
+        _check(load().msx_partial_coherence(self.h, C.byref(st), pt[0], C.byref(ln), float(acmin), float(acmax),
+                                            float(sigmaf), float(dfdelt), int(batch), pimg, C.byref(na),
+                                            C.byref(nd), prec, pt[1]))
+        return img, na.value, nd.value
+
     def ctf_image(self, psi, ax, by, kev, cs, df, aobj, want_wave=False):
         """Objective-lens image |ifft(fft(psi) exp(-i chi) A)|^2 of (nwave, nx, ny) waves."""
         prec = _prec_of(psi)
@@ -567,6 +602,24 @@ def context():
         if _default is None:
             _default = Context(0)
         return _default
+
+
+def amorphous(cell, Z=6, density_g_cm3=2.0, min_dist=1.4, seed=0, mass=None, max_tries=200):
+    """Amorphous specimen by random sequential addition (msx_amorphous) in the
+    periodic box cell = (a, b, c) Angstrom: atoms of element Z at the given
+    mass density (mass in u; carbon 12.011 by default), at least min_dist
+    apart.  Returns an ATOM_DTYPE array (fewer atoms than the density asks
+    only if max_tries draws per atom did not place them all)."""
+    a, b, c = (float(v) for v in cell)
+    m = mass if mass is not None else {6: 12.011}.get(int(Z))
+    if m is None:
+        raise ValueError("give the atomic mass for Z != 6")
+    n = int(round(density_g_cm3 / m * 6.02214076e23 * 1e-24 * a * b * c))
+    out = np.zeros(max(n, 1), ATOM_DTYPE)
+    placed = C.c_int64(0)
+    _check(load().msx_amorphous(a, b, c, int(Z), n, float(min_dist), int(seed) & (2 ** 64 - 1), n * int(max_tries),
+                                C.c_void_p(out.ctypes.data), C.byref(placed)))
+    return out[:placed.value]
 
 
 def displace(atoms, u_rms, seed, config, ctx=None, out=None):

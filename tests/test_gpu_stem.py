@@ -62,6 +62,28 @@ def test_stem_matches_oracle(ctx, small, prec):
     assert err < (1e-11 if prec == msx.F64 else 1e-5)
 
 
+@pytest.mark.parametrize("nx,ny,cells", [(384, 384, (3, 3, 2)), (256, 512, (3, 6, 2))])
+def test_stem_radix3_and_rectangular_grids(ctx, nx, ny, cells):
+    """The radix-3 length (384) and a rectangular grid take other FFT plans and
+    column-group counts in the detector epilogue; FP32 against the FP64 oracle."""
+    at, cell = specimen.srtio3_001(cells)
+    nsl = 2 * cells[2]
+    V = O.potential(at["Z"], np.stack([at["x"], at["y"], at["z"]], 1), cell, nx, ny, nsl, 3.0)
+    t64 = O.transmission(V, cell[0], cell[1], KEV)
+    lam = msx.wavelength(KEV)
+    pos = np.random.default_rng(11).random((5, 2)) * np.array(cell[:2])
+    det = [(0.0, 0.012 / lam), (0.025 / lam, 0.080 / lam)]
+    cz, layer = np.full(nsl, cell[2] / nsl), np.arange(nsl)
+    ref = O.stem(t64, cz, layer, cell[0], cell[1], KEV, 0.021, pos, det, [nsl], cs=1e4, df=10.0)
+    got = ctx.stem(t64.astype(np.complex64), cz, layer, cell[0], cell[1], KEV,
+                   dict(alpha_max=0.021, cs=1e4, df=10.0), pos, det, [nsl], batch=2)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    per_det = (np.abs(got - ref).max(axis=2) / np.abs(ref).max(axis=2)).max()  # the weak ADF ring on its own scale
+    print(f"STEM {nx}x{ny} F32: max rel err vs oracle FP64 {err:.2e} (per detector {per_det:.2e})")
+    assert err < 1e-5
+    assert per_det < 1e-4
+
+
 def test_stem_accumulate_and_weight(ctx, small):
     cell, V, t64, nsl = small
     pos = [(1.0, 2.0), (3.0, 1.5)]
