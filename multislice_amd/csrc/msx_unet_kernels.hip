@@ -20,95 +20,139 @@ namespace unet {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 16;  // block tile; 4 wavefronts of 32x32
+// Block tile: BM output rows (32 or 64) x BN = 4096/BM columns, K step 32;
+// 4 wavefronts, each owning one 32x32 output tile (two MFMAs per K step).
+constexpr int BK = 32, KP = BK + 8;  // +8 bf16: 16-byte rows, spread banks
 
-// activation gather X[n, c, y+dy, x+dx] with zero padding
-__device__ __forceinline__ float gather(const float* X, int C, int H, int W, int n, int c, int y, int x) {
-    if ((unsigned)y >= (unsigned)H || (unsigned)x >= (unsigned)W) return 0.f;
-    return X[(((long long)n * C + c) * H + y) * W + x];
-}
-
-template <int MODE, int KS>
+// Operand staging.  A rows are the GEMM's M rows, B rows its N columns; each
+// thread stages a run of consecutive k of one row:
+//   A: row tid / (BK/KPA), k run (tid % (BK/KPA)) * KPA, KPA = BM*BK/256
+//   B: row tid % BN,       k run (tid / BN) * KPB,        KPB = BN*BK/256
+// so pixel rows are gathered by consecutive threads (coalesced) and each
+// thread writes whole runs of its LDS row.  Indices are decoded once per run
+// and stepped, not divided per element.
+template <int MODE, int KS, int BM>
 __global__ __launch_bounds__(256) void conv_gemm(ConvArgs p) {
-    __shared__ __bf16 As[BM][BK + 8];  // [m][k]  (+8 keeps 16-byte rows and spreads banks)
-    __shared__ __bf16 Bs[BN][BK + 8];  // [n][k]
+    constexpr int BN = 4096 / BM, KPA = BM * BK / 256, KPB = BN * BK / 256, TAPS = KS * KS, R = KS / 2;
+    __shared__ __align__(16) __bf16 As[2][BM][KP];
+    __shared__ __align__(16) __bf16 Bs[2][BN][KP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = BM == 64 ? (wave & 1) : 0, wn = BM == 64 ? (wave >> 1) : wave;
     const int HW = p.H * p.W;
-    constexpr int TAPS = KS * KS, R = KS / 2;
-    // GEMM sizes per mode
     long long M, N, K;
     if (MODE == kFwd) { M = p.Cout; N = (long long)p.N * HW; K = (long long)p.Cin * TAPS; }
     else if (MODE == kBwdData) { M = p.Cin; N = (long long)p.N * HW; K = (long long)p.Cout * TAPS; }
     else { M = p.Cout; N = (long long)p.Cin * TAPS; K = (long long)p.N * HW; }
     const long long m0 = (long long)blockIdx.y * BM, n0 = (long long)blockIdx.x * BN;
-    // split-K range (weight gradient only)
     long long k_beg = 0, k_end = K;
-    if (MODE == kBwdWeight) {
+    if (MODE == kBwdWeight) {  // split over pixels
         const long long chunk = ((K + gridDim.z - 1) / gridDim.z + BK - 1) / BK * BK;
         k_beg = (long long)blockIdx.z * chunk;
         k_end = k_beg + chunk < K ? k_beg + chunk : K;
     }
+    const int a_row = tid / (BK / KPA), a_k = (tid % (BK / KPA)) * KPA;
+    const int b_row = tid % BN, b_k = (tid / BN) * KPB;
+    const long long am = m0 + a_row, bn = n0 + b_row;
+    // B row: a pixel (forward, input gradient) or a (channel, tap) (weight gradient)
+    int b_img = 0, b_y = 0, b_x = 0, b_ci = 0, b_dy = 0, b_dx = 0;
+    if (bn < N) {
+        if (MODE != kBwdWeight) {
+            b_img = (int)(bn / HW);
+            const int pix = (int)(bn % HW);
+            b_y = pix / p.W;
+            b_x = pix % p.W;
+        } else {
+            b_ci = (int)(bn / TAPS);
+            const int tap = (int)(bn % TAPS);
+            b_dy = tap / KS - R;
+            b_dx = tap % KS - R;
+        }
+    }
+
+    float ra[KPA], rb[KPB];
+    auto fetch = [&](long long k0) {
+        const long long ka = k0 + a_k;
+        if (MODE == kFwd) {
+#pragma unroll
+            for (int j = 0; j < KPA; ++j) ra[j] = (am < M && ka + j < k_end) ? p.w[am * K + ka + j] : 0.f;
+        } else if (MODE == kBwdData) {
+            int co = (int)(ka / TAPS), tap = (int)(ka % TAPS);
+#pragma unroll
+            for (int j = 0; j < KPA; ++j) {
+                ra[j] = (am < M && ka + j < k_end) ? p.w[((long long)co * p.Cin + am) * TAPS + (TAPS - 1 - tap)] : 0.f;
+                if (++tap == TAPS) { tap = 0; ++co; }
+            }
+        } else {
+            int img = (int)(ka / HW), pix = (int)(ka % HW);
+#pragma unroll
+            for (int j = 0; j < KPA; ++j) {
+                float v = 0.f;
+                if (am < M && ka + j < k_end) {
+                    const long long o = ((long long)img * p.Cout + am) * HW + pix;
+                    v = p.g[o];
+                    if (p.relu && p.y[o] <= 0.f) v = 0.f;
+                }
+                ra[j] = v;
+                if (++pix == HW) { pix = 0; ++img; }
+            }
+        }
+        const long long kb = k0 + b_k;
+        if (MODE == kFwd || MODE == kBwdData) {
+            int c = (int)(kb / TAPS), tap = (int)(kb % TAPS);
+            const int C = MODE == kFwd ? p.Cin : p.Cout;
+            const float* src = MODE == kFwd ? p.x : p.g;
+#pragma unroll
+            for (int j = 0; j < KPB; ++j) {
+                float v = 0.f;
+                if (bn < N && kb + j < k_end) {
+                    const int yy = b_y + tap / KS - R, xx = b_x + tap % KS - R;
+                    if ((unsigned)yy < (unsigned)p.H && (unsigned)xx < (unsigned)p.W) {
+                        const long long o = (((long long)b_img * C + c) * p.H + yy) * p.W + xx;
+                        v = src[o];
+                        if (MODE == kBwdData && p.relu && p.y[o] <= 0.f) v = 0.f;
+                    }
+                }
+                rb[j] = v;
+                if (++tap == TAPS) { tap = 0; ++c; }
+            }
+        } else {
+            int img = (int)(kb / HW);
+            const int pix = (int)(kb % HW);
+            int y = pix / p.W, x = pix % p.W;
+#pragma unroll
+            for (int j = 0; j < KPB; ++j) {
+                float v = 0.f;
+                if (bn < N && kb + j < k_end) {
+                    const int yy = y + b_dy, xx = x + b_dx;
+                    if ((unsigned)yy < (unsigned)p.H && (unsigned)xx < (unsigned)p.W)
+                        v = p.x[(((long long)img * p.Cin + b_ci) * p.H + yy) * p.W + xx];
+                }
+                rb[j] = v;
+                if (++x == p.W) { x = 0; if (++y == p.H) { y = 0; ++img; } }
+            }
+        }
+    };
+
     f32x16 acc;
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-
-    // element (row r, col c) of the staging pattern: 256 threads x 4 values
+    const int r = lane & 31, kh = (lane >> 5) * 8;
+    int buf = 0;
+    if (k_beg < k_end) fetch(k_beg);
     for (long long k0 = k_beg; k0 < k_end; k0 += BK) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e = tid + q * 256;      // 0..1023
-            const int mm = e / BK, kk = e % BK;  // A: [64][16]
-            const long long m = m0 + mm, k = k0 + kk;
-            float va = 0.f;
-            if (m < M && k < k_end) {
-                if (MODE == kFwd) {
-                    va = p.w[m * K + k];  // W[co][ci][tap] with k = ci*TAPS + tap
-                } else if (MODE == kBwdData) {
-                    const int co = (int)(k / TAPS), tap = (int)(k % TAPS);
-                    va = p.w[((long long)co * p.Cin + m) * TAPS + (TAPS - 1 - tap)];
-                } else {
-                    const long long n = k / HW, pix = k % HW;
-                    const long long o = ((n * p.Cout + m) * HW + pix);
-                    va = p.g[o];
-                    if (p.relu && p.y[o] <= 0.f) va = 0.f;
-                }
-            }
-            As[mm][kk] = (__bf16)va;
-            const int nn = e / BK, kb = e % BK;  // B: [64 n][16 k]
-            const long long n = n0 + nn, kB = k0 + kb;
-            float vb = 0.f;
-            if (n < N && kB < k_end) {
-                if (MODE == kFwd || MODE == kBwdData) {
-                    const int img = (int)(n / HW), pix = (int)(n % HW), y = pix / p.W, x = pix % p.W;
-                    const int c = (int)(kB / TAPS), tap = (int)(kB % TAPS);
-                    const int dy = tap / KS - R, dx = tap % KS - R;
-                    if (MODE == kFwd) {
-                        vb = gather(p.x, p.Cin, p.H, p.W, img, c, y + dy, x + dx);
-                    } else {
-                        const int yy = y + dy, xx = x + dx;
-                        if ((unsigned)yy < (unsigned)p.H && (unsigned)xx < (unsigned)p.W) {
-                            const long long o = (((long long)img * p.Cout + c) * p.H + yy) * p.W + xx;
-                            vb = p.g[o];
-                            if (p.relu && p.y[o] <= 0.f) vb = 0.f;
-                        }
-                    }
-                } else {
-                    const int ci = (int)(n / TAPS), tap = (int)(n % TAPS);
-                    const int dy = tap / KS - R, dx = tap % KS - R;
-                    const int img = (int)(kB / HW), pix = (int)(kB % HW), y = pix / p.W, x = pix % p.W;
-                    vb = gather(p.x, p.Cin, p.H, p.W, img, ci, y + dy, x + dx);
-                }
-            }
-            Bs[nn][kb] = (__bf16)vb;
+        for (int j = 0; j < KPA; ++j) As[buf][a_row][a_k + j] = (__bf16)ra[j];
+#pragma unroll
+        for (int j = 0; j < KPB; ++j) Bs[buf][b_row][b_k + j] = (__bf16)rb[j];
+        __syncthreads();
+        if (k0 + BK < k_end) fetch(k0 + BK);  // the next step's loads overlap this step's MFMAs
+#pragma unroll
+        for (int s = 0; s < BK; s += 16) {
+            const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[buf][wm * 32 + r][s + kh]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[buf][wn * 32 + r][s + kh]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
-        __syncthreads();
-        // fragments: lane l holds row/col (l % 32), k = 8*(l/32) .. +7
-        const int r = lane & 31, kh = (lane >> 5) * 8;
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + r][kh]);
-        const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + r][kh]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-        __syncthreads();
+        buf ^= 1;  // double buffer: the next step writes the other buffer
     }
     // epilogue: acc[v] is C[row][col], col = lane % 32,
     // row = 8*(v/4) + 4*(lane/32) + v%4 (32x32 MFMA output layout)
@@ -122,10 +166,8 @@ __global__ __launch_bounds__(256) void conv_gemm(ConvArgs p) {
         if (MODE == kFwd) {
             if (p.bias) c += p.bias[m];
             if (p.relu && c < 0.f) c = 0.f;
-            const int img = (int)(n / HW), pix = (int)(n % HW);
-            float* dst = p.out + ((long long)img * p.out_c + p.out_c0 + m) * HW + pix;
-            if (p.accumulate) *dst += c; else *dst = c;
-        } else if (MODE == kBwdData) {
+        }
+        if (MODE == kFwd || MODE == kBwdData) {
             const int img = (int)(n / HW), pix = (int)(n % HW);
             float* dst = p.out + ((long long)img * p.out_c + p.out_c0 + m) * HW + pix;
             if (p.accumulate) *dst += c; else *dst = c;
@@ -141,27 +183,34 @@ hipError_t launch_conv(int mode, int ks, const ConvArgs& a, hipStream_t s) {
     if (mode == kFwd) { M = a.Cout; N = (long long)a.N * HW; K = (long long)a.Cin * ks * ks; }
     else if (mode == kBwdData) { M = a.Cin; N = (long long)a.N * HW; K = (long long)a.Cout * ks * ks; }
     else { M = a.Cout; N = (long long)a.Cin * ks * ks; K = (long long)a.N * HW; }
+    const int bm = M <= 32 ? 32 : 64, bn = 4096 / bm;
     int splits = 1;
     if (mode == kBwdWeight) {
-        const long long tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+        const long long tiles = ((M + bm - 1) / bm) * ((N + bn - 1) / bn);
         long long want = 2048 / (tiles > 0 ? tiles : 1);
         long long maxs = (K + 4 * BK - 1) / (4 * BK);
         splits = (int)(want < 1 ? 1 : (want > maxs ? maxs : want));
         if (splits < 1) splits = 1;
     }
-    dim3 grid((unsigned)((N + BN - 1) / BN), (unsigned)((M + BM - 1) / BM), (unsigned)splits);
-#define MSX_CONV(MODE_, KS_) hipLaunchKernelGGL((conv_gemm<MODE_, KS_>), grid, dim3(256), 0, s, a)
+    dim3 grid((unsigned)((N + bn - 1) / bn), (unsigned)((M + bm - 1) / bm), (unsigned)splits);
+#define MSX_CONV(MODE_, KS_, BM_) hipLaunchKernelGGL((conv_gemm<MODE_, KS_, BM_>), grid, dim3(256), 0, s, a)
+#define MSX_CONV_BM(MODE_, KS_)                    \
+    do {                                           \
+        if (bm == 32) MSX_CONV(MODE_, KS_, 32);    \
+        else MSX_CONV(MODE_, KS_, 64);             \
+    } while (0)
     if (ks == 3) {
-        if (mode == kFwd) MSX_CONV(kFwd, 3);
-        else if (mode == kBwdData) MSX_CONV(kBwdData, 3);
-        else MSX_CONV(kBwdWeight, 3);
+        if (mode == kFwd) MSX_CONV_BM(kFwd, 3);
+        else if (mode == kBwdData) MSX_CONV_BM(kBwdData, 3);
+        else MSX_CONV_BM(kBwdWeight, 3);
     } else if (ks == 1) {
-        if (mode == kFwd) MSX_CONV(kFwd, 1);
-        else if (mode == kBwdData) MSX_CONV(kBwdData, 1);
-        else MSX_CONV(kBwdWeight, 1);
+        if (mode == kFwd) MSX_CONV_BM(kFwd, 1);
+        else if (mode == kBwdData) MSX_CONV_BM(kBwdData, 1);
+        else MSX_CONV_BM(kBwdWeight, 1);
     } else {
         return hipErrorInvalidValue;
     }
+#undef MSX_CONV_BM
 #undef MSX_CONV
     return hipGetLastError();
 }
