@@ -76,30 +76,64 @@ def c2(ctx):
             "ms_per_wave_slice": round(1e3 * ms / (16 * 100), 4), "image_mean": float(img.mean())}
 
 
-def uneven_jobs(rng, njobs, sizes=(256, 384, 512, 768, 1024)):
-    """Synthetic uneven jobs: grid n from `sizes`, lateral size 0.1 A * n,
-    20-200 slices of 2 A, 200 or 300 kV, four distinct smooth random phase
-    layers per job (device tensors)."""
+def uneven_specimen(rng, n, nslices, seed, max_au=50):
+    """Specimen of one uneven job: amorphous carbon (2.0 g/cm^3, random
+    sequential addition with 1.4 A minimum spacing, msx_amorphous) filling a
+    periodic 0.1n x 0.1n x 2*nslices A box, plus 0..max_au Au atoms of an fcc
+    cluster (a = 4.078 A) at a random centre; carbon atoms within 1.4 A of a
+    gold atom are removed.  Returns (atoms, cell)."""
+    L, c = 0.1 * n, 2.0 * nslices
+    box = np.array([L, L, c])
+    at = engine.amorphous((L, L, c), seed=seed)
+    nau = int(rng.integers(0, max_au + 1))
+    if nau:
+        au = specimen.au_cluster(nau, rng.random(3) * box)
+        pa = np.stack([au["x"], au["y"], au["z"]], 1) % box
+        au["x"], au["y"], au["z"] = pa[:, 0], pa[:, 1], pa[:, 2]
+        pc = np.stack([at["x"], at["y"], at["z"]], 1)
+        keep = np.ones(len(at), bool)
+        for q in pa:
+            d = pc - q
+            d -= box * np.round(d / box)
+            keep &= (d * d).sum(1) >= 1.4 * 1.4
+        at = np.concatenate([at[keep], au])
+    return at, (L, L, c)
+
+
+def uneven_jobs(ctx, rng, njobs, sizes=(256, 384, 512, 768, 1024), seed=5, prec=engine.F32, max_au=50,
+                slices=(20, 200)):
+    """The uneven-job set: grid n from `sizes`, `slices` (20-200) slices of 2 A, 200 or
+    300 kV, each job its own amorphous-carbon + Au specimen (uneven_specimen);
+    potentials (radius 3 A) and transmission stacks are built on the GPU, one
+    layer per slice.  Returns job dicts (trans on the GPU) and their atoms."""
     import torch
-    jobs = []
-    for _ in range(njobs):
+    rdt = torch.float64 if prec == engine.F64 else torch.float32
+    jobs, specs = [], []
+    for k in range(njobs):
         n = int(rng.choice(sizes))
-        ns = int(rng.integers(20, 201))
+        ns = int(rng.integers(slices[0], slices[1] + 1))
         kev = float(rng.choice([200.0, 300.0]))
-        V = torch.rand((4, n, n), device="cuda") * 20.0
-        jobs.append(dict(n=n, V=V, ax=0.1 * n, by=0.1 * n, kev=kev, cz=np.full(4, 2.0), layer=np.arange(ns) % 4))
-    return jobs
+        at, cell = uneven_specimen(rng, n, ns, seed * 1000003 + k, max_au)
+        V = torch.empty((ns, n, n), dtype=rdt, device="cuda")
+        ctx.potential(at, cell, n, n, ns, 3.0, prec=prec, out=V)
+        trans = ctx.transmission(V, cell[0], cell[1], kev)
+        del V
+        jobs.append(dict(n=n, trans=trans, ax=cell[0], by=cell[1], kev=kev, cz=np.full(ns, 2.0), layer=np.arange(ns)))
+        specs.append((at, cell))
+    return jobs, specs
 
 
-def c4(ctx, njobs=256, seed=5):
+def c4(ctx, njobs=256, seed=5, f64_every=16):
     """Uneven jobs: one msx_multislice_batch call against one msx_multislice
-    call per job (same jobs, same results), after a warm-up of each."""
+    call per job (same jobs, same results), after a warm-up of each; then the
+    FP32 exit waves of every f64_every-th job against an FP64 run of the same
+    job (FP64 potential, transmission and slice loop)."""
     import torch
-    jobs = uneven_jobs(np.random.default_rng(seed), njobs)
-    for j in jobs:
-        j["trans"] = ctx.transmission(j["V"], j["ax"], j["by"], j["kev"])
-        del j["V"]
+    t0 = _sync()
+    jobs, specs = uneven_jobs(ctx, np.random.default_rng(seed), njobs, seed=seed)
+    t_build = _sync() - t0
     wave_slices = sum(len(j["layer"]) for j in jobs)
+    natoms = sum(len(a) for a, _ in specs)
 
     def batched():
         for j in jobs:
@@ -117,9 +151,25 @@ def c4(ctx, njobs=256, seed=5):
 
     batched(), one_by_one()
     tb, ts = batched(), one_by_one()
-    return {"workload": f"C4 {njobs} uneven HRTEM jobs, n in 256..1024, 20-200 slices, 200/300 kV",
-            "wave_slices": wave_slices, "batched_s": round(tb, 3), "one_by_one_s": round(ts, 3),
-            "speedup": round(ts / tb, 2), "wave_slices_per_s_batched": round(wave_slices / tb)}
+    errs = []
+    for k in range(0, njobs, f64_every):
+        j = jobs[k]
+        at, cell = specs[k]
+        n, ns = j["n"], len(j["layer"])
+        V = torch.empty((ns, n, n), dtype=torch.float64, device="cuda")
+        ctx.potential(at, cell, n, n, ns, 3.0, prec=engine.F64, out=V)
+        t64 = ctx.transmission(V, cell[0], cell[1], j["kev"])
+        del V
+        psi = torch.ones((n, n), dtype=torch.complex128, device="cuda")
+        ctx.multislice(t64, j["cz"], j["layer"], psi, cell[0], cell[1], j["kev"])
+        del t64
+        errs.append(float(torch.linalg.vector_norm(j["psi"].to(torch.complex128) - psi) / torch.linalg.vector_norm(psi)))
+    return {"workload": f"C4 {njobs} uneven HRTEM jobs: amorphous C (2.0 g/cm^3, 1.4 A spacing) + 0-50 Au, "
+                        f"n in 256..1024 at 0.1 A/px, 20-200 slices of 2 A, 200/300 kV",
+            "atoms": natoms, "wave_slices": wave_slices, "build_s": round(t_build, 2), "batched_s": round(tb, 3),
+            "one_by_one_s": round(ts, 3), "speedup": round(ts / tb, 2),
+            "wave_slices_per_s_batched": round(wave_slices / tb),
+            "fp32_vs_fp64_rel_l2": {"jobs": len(errs), "max": max(errs), "median": float(np.median(errs))}}
 
 
 def main(argv=None):
