@@ -10,6 +10,7 @@ transmission), and the objective-lens images are averaged.
 
     python -m multislice_amd.hrtem c2      # SrTiO3 52x52x50, 2048^2, 100 slices, 16 configurations
     python -m multislice_amd.hrtem c4      # 256 uneven jobs, batched vs one by one
+    python -m multislice_amd.hrtem pc      # partial-coherence images, Si 256^2 and 512^2, 80 slices
 
 print one JSON line of timings each.
 """
@@ -172,12 +173,53 @@ def c4(ctx, njobs=256, seed=5, f64_every=16):
             "fp32_vs_fp64_rel_l2": {"jobs": len(errs), "max": max(errs), "median": float(np.median(errs))}}
 
 
+def _pc_case(ctx, cells, n, reps):
+    import torch
+    at, cell = specimen.si_001(cells)
+    ns, kev = 4 * cells[2], 200.0
+    V = torch.empty((ns, n, n), dtype=torch.float32, device="cuda")
+    ctx.potential(at, cell, n, n, ns, 3.0, out=V)
+    t = ctx.transmission(V, cell[0], cell[1], kev)
+    del V
+    lens = dict(cs=1.3e7, df=700.0, aobj=0.024, dfa2=30.0, dfa2phi=0.5)
+    args = (t, np.full(ns, cell[2] / ns), np.arange(ns), cell[0], cell[1], kev, lens, 0.0, 2.5e-3, 40.0, 10.0)
+    img, na, nd = ctx.partial_coherence(*args)
+    t0 = _sync()
+    for _ in range(reps):
+        img, na, nd = ctx.partial_coherence(*args)
+    dt = (_sync() - t0) / reps
+    ctx.profile(True)
+    ctx.partial_coherence(*args)
+    prof = {}
+    for name, kind in (("row_sweeps", engine.PROF_ROW), ("column_sweeps", engine.PROF_COL)):
+        nl, ms, by = ctx.profile_read(kind)
+        prof[name] = {"launches": nl, "ms": round(ms, 3), "GB_per_s": round(by / ms / 1e6, 1) if ms else None}
+    ctx.profile(False)
+    return {"workload": f"Si [001] {cells[0]}x{cells[1]}x{cells[2]}, {n}^2, {ns} slices, 200 kV, "
+                        f"{na} illumination angles x {nd} defocus steps",
+            "ms": round(1e3 * dt, 3), "ms_per_angle": round(1e3 * dt / na, 4),
+            "wave_slices_per_s": round(na * ns / dt), "lens_images_per_s": round(na * nd / dt),
+            "image_mean": float(img.mean()), "kernels": prof}
+
+
+def pc(ctx, reps=5):
+    """Partial-coherence images (msx_partial_coherence) of Si [001] slabs
+    (slices of a/4, 200 kV): illumination cone 0-2.5 mrad, defocus spread
+    40 A sampled every 10 A (21 steps), Cs 1.3 mm, defocus 700 A, 2-fold
+    astigmatism 30 A, 24 mrad aperture.  The 4x4x20 slab on 256^2 (13 grid
+    angles) and the 8x8x20 slab on 512^2 (more angles on the finer angular
+    grid); one warm-up, then the mean of `reps` timed calls, and the sweep
+    kernel times from the profiler on a separate call."""
+    return {"cases": [_pc_case(ctx, (4, 4, 20), 256, reps), _pc_case(ctx, (8, 8, 20), 512, reps)]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m multislice_amd.hrtem")
-    ap.add_argument("workload", choices=["c2", "c4"])
+    ap.add_argument("workload", choices=["c2", "c4", "pc"])
     a = ap.parse_args(argv)
     ctx = engine.Context(0)
-    print(json.dumps(c2(ctx) if a.workload == "c2" else c4(ctx)), flush=True)
+    run = {"c2": c2, "c4": c4, "pc": pc}[a.workload]
+    print(json.dumps(run(ctx)), flush=True)
 
 
 if __name__ == "__main__":

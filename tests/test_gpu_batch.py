@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import multislice_amd as msx
+from conftest import rel_l2
 from oracle import Oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -67,6 +68,35 @@ def test_batch_fp32_vs_fp64_per_job(ctx):
     errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(g32, g64)]
     print("per-job FP32 vs FP64 rel-L2:", ", ".join(f"{e:.1e}" for e in errs))
     assert max(errs) < 1e-5
+
+
+def test_amorphous_au_jobs_match_oracle(ctx):
+    """Uneven jobs with their real specimens (amorphous carbon + an Au
+    cluster, built by msx_amorphous, potentials and transmissions on the
+    GPU): batched FP32 and FP64 exit waves against the oracle's FP64 pipeline
+    (CPU potential, transmission and slice loop) on the same atoms."""
+    import torch
+    from multislice_amd import hrtem
+    kw = dict(sizes=(128, 256), seed=21, max_au=12, slices=(8, 24))
+    jobs32, specs = hrtem.uneven_jobs(ctx, np.random.default_rng(21), 3, **kw)
+    jobs64, specs64 = hrtem.uneven_jobs(ctx, np.random.default_rng(21), 3, prec=msx.F64, **kw)
+    outs = []
+    for jobs, cdt in ((jobs32, torch.complex64), (jobs64, torch.complex128)):
+        for j in jobs:
+            j["psi"] = torch.ones((j["n"], j["n"]), dtype=cdt, device="cuda")
+        ctx.multislice_batch(jobs)
+        outs.append([j["psi"].cpu().numpy() for j in jobs])
+    for j, (at, cell), (at2, _), a, b in zip(jobs32, specs, specs64, *outs):
+        assert np.array_equal(at, at2) and np.any(at["Z"] == 6)
+        n, ns = j["n"], len(j["layer"])
+        V = O.potential(at["Z"], np.stack([at["x"], at["y"], at["z"]], 1), cell, n, n, ns, 3.0)
+        t = O.transmission(V, cell[0], cell[1], j["kev"])
+        ref = O.multislice(t, j["cz"], j["layer"], np.ones((n, n), np.complex128), cell[0], cell[1], j["kev"])
+        e32, e64 = rel_l2(a, ref), rel_l2(b, ref)
+        print(f"a-C + {int(np.sum(at['Z'] == 79))} Au, {n}^2, {ns} slices, {j['kev']:.0f} kV: "
+              f"FP32 {e32:.1e}, FP64 {e64:.1e}")
+        assert e32 < 1e-5
+        assert e64 < 1e-10
 
 
 def test_batch_rejects_host_buffers(ctx):
