@@ -7,13 +7,15 @@ Cs = 1 um, 128x128 scan over the central 2x2 cells, BF 0-10 mrad and ADF
 70-200 mrad detectors (the ADF band is clipped at the 2/3 band limit,
 71.7 mrad, on this grid), 8 frozen-phonon configurations resident in HBM.
 
-One step = one launch batch of B probes (default 256) advanced through all
-100 slices with the detector epilogue, for one frozen-phonon configuration
-(steps cycle over the 8 configurations and over the scan).  Inputs
-(transmission functions of all configurations) are resident in HBM before
-the timed region.  Prints one JSON line; value = probe-slices/s.
+One step = one msx_stem call on P probes (default 1024) advanced through
+all 100 slices with the detector epilogue, for one frozen-phonon
+configuration (steps cycle over the 8 configurations and over the scan); the
+call runs them in launch batches of B probes (default: the autotuner's
+choice), back to back on the GPU.  Inputs (transmission functions of all
+configurations) are resident in HBM before the timed region.  Prints one
+JSON line; value = probe-slices/s.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--probes P] [--batch B]
 """
 import argparse
 import json
@@ -105,15 +107,18 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": 5}))
 
 
-def traffic_from_profile():
+def traffic_from_profile(batch, tuning):
     """HBM bytes per launch of the column sweep on this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
-    probe); None when it is absent."""
-    path = os.path.join(ROOT, "profiles", "col_sweep_stem512_b256_traffic.json")
+    probe) of the same launch shape; None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"col_sweep_stem512_b{batch}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)["hbm_bytes_per_launch"]
+        d = json.load(f)
+    if tuning and d.get("col_wpb") not in (None, tuning.get("col_wpb")):
+        return None
+    return d["hbm_bytes_per_launch"]
 
 
 def surrogate_rate(msx, torch, ctx, value):
@@ -157,13 +162,15 @@ def surrogate_rate(msx, torch, ctx, value):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--probes", type=int, default=1024, help="probes per step (one msx_stem call)")
+    ap.add_argument("--batch", type=int, default=0, help="probes per launch (0: the autotuner's choice)")
     ap.add_argument("--prof-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch slice loops directly (no HIP graph replay)")
     ap.add_argument("--no-tune", action="store_true", help="built-in launch shapes instead of the autotuner's")
+    ap.add_argument("--shapes", default="", help="ROW_WPB,COL_WPB,BATCH: fixed launch shapes instead of the autotuner's")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -180,8 +187,12 @@ def main():
     ctx = msx.Context(0)
     ctx.set_graphs(not a.no_graphs)
     tuning = None
-    if not a.no_tune:
-        tuning = ctx.autotune(N, prec=msx.F32, max_batch=a.batch)
+    if a.shapes:
+        rw, cw, bt = (int(v) for v in a.shapes.split(","))
+        tuning = dict(row_wpb=rw, col_wpb=cw, batch=bt, ms_per_slice=0.0)
+        ctx.set_tuning(N, prec=msx.F32, tuning=tuning)
+    elif not a.no_tune:
+        tuning = ctx.autotune(N, prec=msx.F32, max_batch=a.batch or 512)
         log(f"autotuner (512^2 FP32): {tuning}")
     log("building potentials + transmissions for %d configs..." % NCONF)
     trans, cell, t_build, natoms = build_specimen(ctx, torch)
@@ -191,26 +202,27 @@ def main():
     layer = np.arange(NSLICE)
     det = detectors()
     probe = dict(alpha_max=ALPHA, cs=CS, df=0.0)
-    B = a.batch
-    nblk = len(pos) // B
-    out = np.zeros((1, len(det), B))
+    P = a.probes
+    B = a.batch or (tuning["batch"] if tuning else 256)
+    nblk = len(pos) // P
+    out = np.zeros((1, len(det), P))
 
     def step(i):
         c = i % NCONF
         blk = (i // NCONF) % nblk
-        ctx.stem(trans[c], cz, layer, cell[0], cell[1], KEV, probe, pos[blk * B:(blk + 1) * B], det, [NSLICE],
+        ctx.stem(trans[c], cz, layer, cell[0], cell[1], KEV, probe, pos[blk * P:(blk + 1) * P], det, [NSLICE],
                  batch=B, out=out, weight=1.0 / NCONF)
 
     for i in range(a.warmup):
         step(i)
     torch.cuda.synchronize()
-    log("timing %d steps of %d probes x %d slices" % (a.steps, B, NSLICE))
+    log("timing %d steps of %d probes (launch batches of %d) x %d slices" % (a.steps, P, B, NSLICE))
     t0 = time.perf_counter()
     for i in range(a.steps):
         step(a.warmup + i)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    value = a.steps * B * NSLICE / dt
+    value = a.steps * P * NSLICE / dt
 
     # roofline pass: per-kernel HIP-event timing on the launch stream
     ctx.profile(True)
@@ -224,7 +236,7 @@ def main():
     per_kernel = {k: {"launches": v[0], "avg_us": 1e3 * v[1] / max(1, v[0]),
                       "algorithmic_MB_per_launch": v[2] / max(1, v[0]) / 1e6,
                       "GBps": v[2] / max(1e-12, v[1] * 1e-3) / 1e9} for k, v in kinds.items()}
-    traffic = traffic_from_profile() if dom == "col_sweep" else None
+    traffic = traffic_from_profile(B, tuning) if dom == "col_sweep" else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "per_kernel": per_kernel}
@@ -245,7 +257,8 @@ def main():
         "config": {"workload": "batched STEM, SrTiO3 [001] 12x12x50 cells, 512^2 grid, 100 slices of 1.9525 A, "
                                "300 kV, 21 mrad, Cs 1 um, 128x128 scan, BF 0-10 / ADF 70-200 mrad, "
                                "8 frozen-phonon configs resident",
-                   "grid": N, "slices": NSLICE, "probes_per_launch": B, "frozen_phonon_configs": NCONF,
+                   "grid": N, "slices": NSLICE, "probes_per_step": P, "probes_per_launch": B,
+                   "frozen_phonon_configs": NCONF,
                    "scan": [SCAN, SCAN], "launch_shapes": tuning, "hip_graphs": not a.no_graphs},
         "ms_per_slice": round(1e3 * dt / (a.steps * NSLICE), 4),
         "plain_64config_images_per_hr": round(value * 3600 / (SCAN * SCAN * NSLICE * 64), 3),

@@ -527,15 +527,17 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
 // Times the slice loop on synthetic waves (zero data: the kernels do the same
 // work whatever the values) for each candidate launch shape, with the
 // per-launch event timing of the profiling mode, and keeps the fastest:
-// waves per work-group of the row and of the column sweep (1, 2, 4, 8; each
-// sweep on its own), then the STEM batch: the smallest candidate within 3% of
-// the best probe-slice rate.
+// For every STEM batch size (16 to 512 waves per launch, ten sizes): the waves per
+// work-group of the row and of the column sweep (1, 2, 4, 8; each sweep on
+// its own).  The batch chosen is the smallest within 3% of the best
+// probe-slice rate, with its own work-group shapes.
 template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_batch, msx_tuning* res) {
     using Re = typename real_of<C>::type;
     const int prec = sizeof(Re) == 8 ? MSX_F64 : MSX_F32;
     const long long img = (long long)nx * ny;
     const long long cap = std::max(1LL, (2LL << 30) / (long long)(sizeof(C) * img));
-    const int batches[] = {16, 32, 64, 128, 256, 512};
+    const int batches[] = {16, 32, 64, 96, 128, 160, 192, 256, 384, 512};
+    constexpr int kNB = (int)(sizeof(batches) / sizeof(batches[0]));
     int bmax = (int)std::min<long long>(max_batch > 0 ? max_batch : 512, cap);
     const int nslice = 6;
     std::vector<double> cz(1, 2.0);
@@ -582,28 +584,27 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     int rc = MSX_OK;
     msx_tuning best{1, 1, 0, 0.0};
     do {
-        const int nb = std::min(bmax, 256);
-        double brow = 1e30, bcol = 1e30;
-        for (int w : {1, 2, 4, 8}) {
-            double rm, cm;
-            if ((rc = trial(w, w, nb, &rm, &cm)) != MSX_OK) break;
-            if (rm < brow) brow = rm, best.row_wpb = w;
-            if (cm < bcol) bcol = cm, best.col_wpb = w;
-        }
-        if (rc != MSX_OK) break;
-        double rate[6] = {0}, top = 0.0;
-        for (int i = 0; i < 6; i++) {
+        // per batch size: the fastest waves-per-work-group of each sweep, and
+        // the resulting probe-slice rate
+        double rate[kNB] = {0}, top = 0.0;
+        int rw_of[kNB], cw_of[kNB];
+        for (int i = 0; i < kNB; i++) rw_of[i] = cw_of[i] = 1;
+        for (int i = 0; i < kNB && rc == MSX_OK; i++) {
             if (batches[i] > bmax) break;
-            double rm, cm;
-            if ((rc = trial(best.row_wpb, best.col_wpb, batches[i], &rm, &cm)) != MSX_OK) break;
-            rate[i] = batches[i] / (rm + cm);
+            double brow = 1e30, bcol = 1e30;
+            for (int w : {1, 2, 4, 8}) {
+                double rm, cm;
+                if ((rc = trial(w, w, batches[i], &rm, &cm)) != MSX_OK) break;
+                if (rm < brow) brow = rm, rw_of[i] = w;
+                if (cm < bcol) bcol = cm, cw_of[i] = w;
+            }
+            rate[i] = batches[i] / (brow + bcol);
             top = std::max(top, rate[i]);
         }
         if (rc != MSX_OK) break;
-        for (int i = 0; i < 6 && batches[i] <= bmax; i++)
+        for (int i = 0; i < kNB && batches[i] <= bmax; i++)
             if (rate[i] >= 0.97 * top) {
-                best.batch = batches[i];
-                best.ms_per_slice = 1.0 / rate[i];
+                best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i]};
                 break;
             }
     } while (0);
@@ -816,10 +817,16 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
     C* w_dev = static_cast<C*>(ctx->wave.p);
     const int groups = col_groups<C>(nx, ny);
     CHK(ctx->partial.ensure(sizeof(double) * (size_t)ndet * batch * groups));
+    // device: results [nck][ndet][nprobe], the current batch's results
+    // [nck][ndet][batch] (one address for every batch, so the slice-loop graph
+    // of every full batch is the same) and all probe positions
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t obytes = sizeof(double) * (size_t)nck * ndet * nprobe;
-    CHK(ctx->outbuf.ensure(obytes + sizeof(double2) * (size_t)batch + 256));
+    const size_t sbytes = sizeof(double) * (size_t)nck * ndet * batch;
+    CHK(ctx->outbuf.ensure(al(obytes) + al(sbytes) + sizeof(double2) * (size_t)nprobe));
     double* o_dev = static_cast<double*>(ctx->outbuf.p);
-    double2* pos_dev = reinterpret_cast<double2*>(static_cast<char*>(ctx->outbuf.p) + ((obytes + 255) & ~(size_t)255));
+    double* s_dev = reinterpret_cast<double*>(static_cast<char*>(ctx->outbuf.p) + al(obytes));
+    double2* pos_dev = reinterpret_cast<double2*>(static_cast<char*>(ctx->outbuf.p) + al(obytes) + al(sbytes));
 
     SliceTables<C> tb;
     CHK(build_tables<C>(ctx, st, true, &tb));
@@ -846,21 +853,23 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
     }
     dp.ck = ck;
     dp.nck = nck;
-    dp.out = o_dev;
-    dp.nprobe = nprobe;
-    std::vector<double2> hpos(batch);
+    dp.out = s_dev;
+    dp.nprobe = batch;
+    dp.probe0 = 0;
+    // all positions in one copy: batches follow each other without a host wait
+    std::vector<double2> hpos(nprobe);
+    for (int p = 0; p < nprobe; p++) {
+        hpos[p].x = pos[2 * p];
+        hpos[p].y = pos[2 * p + 1];
+    }
+    HIPCHK(hipMemcpyAsync(pos_dev, hpos.data(), sizeof(double2) * nprobe, hipMemcpyHostToDevice, ctx->stream));
     for (int p0 = 0; p0 < nprobe; p0 += batch) {
         const int nb = std::min(batch, nprobe - p0);
-        for (int b = 0; b < nb; b++) {
-            hpos[b].x = pos[2 * (p0 + b)];
-            hpos[b].y = pos[2 * (p0 + b) + 1];
-        }
-        HIPCHK(hipMemcpyAsync(pos_dev, hpos.data(), sizeof(double2) * nb, hipMemcpyHostToDevice, ctx->stream));
         ColArgs<C> g = col_args<C>(w_dev, ny, nb, twx);
         g.ops = kColGen | kColStore;
         g.kx2 = tb.kx2;
         g.ky2 = tb.ky2;
-        g.pos = pos_dev;
+        g.pos = pos_dev + p0;
         g.ax = st->ax;
         g.by = st->by;
         g.chi1 = M_PI * lam;
@@ -870,10 +879,10 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
         g.kap_hi = kap_hi;
         g.amp = (Re)std::sqrt((double)img / (double)nap);
         LAUNCH_COL(nx, kColGenInv, g);
-        dp.probe0 = p0;
         CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nb, nullptr, 0, 1, &dp));
-        // hpos is reused by the next batch: wait for its copy
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        // this batch's [nck][ndet][nb] block into columns p0.. of the results
+        HIPCHK(hipMemcpy2DAsync(o_dev + p0, sizeof(double) * nprobe, s_dev, sizeof(double) * batch,
+                                sizeof(double) * nb, (size_t)nck * ndet, hipMemcpyDeviceToDevice, ctx->stream));
     }
     std::vector<double> res((size_t)nck * ndet * nprobe);
     HIPCHK(hipMemcpyAsync(res.data(), o_dev, obytes, hipMemcpyDeviceToHost, ctx->stream));

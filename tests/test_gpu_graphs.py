@@ -74,6 +74,29 @@ def test_stem_graph_replays_match_direct(si_stack):
     g.close()
 
 
+def test_stem_batches_of_one_call_share_a_graph(si_stack):
+    """The launch batches of one call write their detector sums to one
+    scratch block (copied into place afterwards), so every full batch replays
+    the same graph; the short last batch captures its own."""
+    ctx_d, t, _, cell = si_stack
+    g = msx.Context(0)
+    g.set_graphs(True)
+    cz = np.full(20, cell[2] / 20)
+    layer = np.arange(20)
+    lam = msx.wavelength(200.0)
+    det = [(0.0, 0.01 / lam), (0.04 / lam, 0.09 / lam)]
+    probe = dict(alpha_max=0.02, cs=1e4)
+    pos = np.random.default_rng(5).uniform(0, cell[0], (4 * 32 + 5, 2))
+    a = g.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [7, 20], batch=32)
+    b = ctx_d.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [7, 20], batch=32)
+    np.testing.assert_array_equal(a, b)
+    one = ctx_d.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos[37:38], det, [7, 20], batch=1)
+    np.testing.assert_array_equal(a[:, :, 37:38], one)
+    captures, replays = g.graph_stats()
+    assert captures == 2 and replays == 3, (captures, replays)
+    g.close()
+
+
 def test_autotune_keeps_results_and_reports_shapes(si_stack):
     ctx_d, t, _, cell = si_stack
     c = msx.Context(0)
