@@ -6,8 +6,12 @@
 //   ModelSimulated    ImageSimulation/LMultiSlice/ModelSimulated.h:7-28
 //   CPotentialBuilder ImageSimulation/PotentialBuilder/PotentialBuilder.h:14-22
 //   AModel::Model-compatible atom container (amodel/include/Model.h:20-174,
-//   BaseStructures.h:50-135) and an ImageSpace::Image-compatible result
-//   (ImageSimulation/Image/Image.h:29-88) for callers without those DLLs.
+//   BaseStructures.h:50-135) with the .coo reader (ModelCoo.cpp:20-90) and an
+//   ImageSpace::Image-compatible result (ImageSimulation/Image/Image.h:29-88)
+//   for callers without those DLLs;
+//   Dispatcher        ImageSimulation/LMultiSlice/Dispatcher.cu:25-242 (the
+//                     XML command and the run: potential, _pt.mrc, image,
+//                     .mrc), driven by the `lmultislice` executable.
 //
 // The classes live in namespace msx::lms.  Defining MSX_LMS_GLOBAL_NAMES before
 // including this header also exports ModelPotential, ModelSimulated,
@@ -31,6 +35,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <initializer_list>
+#include <iostream>
+#include <iterator>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -68,8 +77,7 @@ struct Cortege {  // BaseStructures.h:129-135
     Record element;
 };
 
-// Z from the element symbol (the table of Model::getNumberByName, Model.h:62-81).
-inline int number_by_name(const char* name) {
+inline const char* const* element_names() {
     static const char* table[] = {
         "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si", "P",  "S",  "Cl", "Ar",
         "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr",
@@ -77,9 +85,28 @@ inline int number_by_name(const char* name) {
         "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb", "Lu", "Hf",
         "Ta", "W",  "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At", "Rn", "Fr", "Ra", "Ac", "Th",
         "Pa", "U",  "Np", "Pu", "Am", "Cm", "Bk", "Cf", "Es", "Fm", "Md", "No", "Lr"};
+    return table;
+}
+// Z from the element symbol (Model::getNumberByName, Model.h:62-81).
+inline int number_by_name(const char* name) {
     for (int i = 0; i < 103; i++)
-        if (std::strcmp(name, table[i]) == 0) return i + 1;
+        if (std::strcmp(name, element_names()[i]) == 0) return i + 1;
     return -1;
+}
+// Symbol from Z (Model::getNameByNumber, Model.h:82-105); "" outside 1..103.
+inline const char* name_by_number(int z) { return (z >= 1 && z <= 103) ? element_names()[z - 1] : ""; }
+
+// A number of a .coo file (Model::StringToDouble, Model.h:106-131): "p/q"
+// is a fraction, a decimal comma reads as a point.  (The reference turns
+// character 1 into the point, which is the comma only for one-digit integer
+// parts; here the comma itself is replaced.)
+inline double coo_number(const char* s) {
+    std::string t(s ? s : "");
+    const size_t sl = t.find('/');
+    if (sl != std::string::npos) return std::atof(t.substr(0, sl).c_str()) / std::atof(t.substr(sl + 1).c_str());
+    for (char& ch : t)
+        if (ch == ',') ch = '.';
+    return std::atof(t.c_str());
 }
 
 // Minimal container with the accessors ModelPotential uses on AModel::Model.
@@ -90,12 +117,18 @@ public:
     float getA() const { return a_; }
     float getB() const { return b_; }
     float getC() const { return c_; }
+    float getAlpha() const { return alpha_; }
+    float getBeta() const { return beta_; }
+    float getGamma() const { return gamma_; }
     size_t getNumberAtoms() const { return cells_.size(); }
     Cortege* getTableCell() { return cells_.data(); }
     int getNumberByName(const char* name) const { return number_by_name(name); }
     void setA(float v) { a_ = v; }
     void setB(float v) { b_ = v; }
     void setC(float v) { c_ = v; }
+    void setAlpha(float v) { alpha_ = v; }
+    void setBeta(float v) { beta_ = v; }
+    void setGamma(float v) { gamma_ = v; }
     void add(const char* symbol, float fx, float fy, float fz, float occ = 1.f) {
         Cortege c;
         std::strncpy(c.element.Atom, symbol, 3);
@@ -103,9 +136,50 @@ public:
         c.element.occupation = occ;
         cells_.push_back(c);
     }
+    // ModelCoo::read (ModelCoo.cpp:20-90): a comment line; "a b c alpha beta
+    // gamma n" (decimal commas allowed); then n lines "x y z Z absorption
+    // oxygenation occupation" with fractional coordinates.  0 / -1.
+    int read(const char* fileName) {
+        std::ifstream fin(fileName);
+        if (!fin.is_open()) {
+            std::cerr << "File with name " << fileName << " doesn't exists!" << std::endl;
+            return -1;
+        }
+        std::string line;
+        if (!std::getline(fin, line) || !std::getline(fin, line)) return -1;
+        for (char& ch : line)
+            if (ch == ',') ch = '.';
+        std::istringstream head(line);
+        double v[6];
+        long long n = -1;
+        for (double& x : v)
+            if (!(head >> x)) return -1;
+        if (!(head >> n) || n < 0) return -1;
+        a_ = (float)v[0], b_ = (float)v[1], c_ = (float)v[2];
+        alpha_ = (float)v[3], beta_ = (float)v[4], gamma_ = (float)v[5];
+        cells_.clear();
+        cells_.reserve((size_t)n);
+        for (long long i = 0; i < n; i++) {
+            if (!std::getline(fin, line)) return -1;
+            std::istringstream row(line);
+            std::string tok[7];
+            for (auto& t : tok)
+                if (!(row >> t)) return -1;
+            Cortege c;
+            c.element.xsCoordinate = XYZ{(float)coo_number(tok[0].c_str()), (float)coo_number(tok[1].c_str()),
+                                         (float)coo_number(tok[2].c_str())};
+            std::strncpy(c.element.Atom, name_by_number(std::atoi(tok[3].c_str())), 3);
+            c.element.absorbtion = (float)coo_number(tok[4].c_str());
+            c.element.oxygenation = (float)coo_number(tok[5].c_str());
+            c.element.occupation = (float)coo_number(tok[6].c_str());
+            c.element.R_e = 0.2f;
+            cells_.push_back(c);
+        }
+        return 0;
+    }
 
 private:
-    float a_ = 0, b_ = 0, c_ = 0;
+    float a_ = 0, b_ = 0, c_ = 0, alpha_ = 90, beta_ = 90, gamma_ = 90;
     std::vector<Cortege> cells_;
 };
 }  // namespace amodel
@@ -124,13 +198,16 @@ struct Image {
         imageData = reinterpret_cast<char*>(store.data());
     }
     double* data() { return store.data(); }
-    // |psi| of a 2-channel complex image (Image::getModule, Image.cu:138-163)
+    // |psi| of a 2-channel complex image (Image::getModule, Image.cu:234-257)
     Image getModule() const {
         Image m((size_t)width, (size_t)height, (size_t)thickness, 8, 1);
         for (size_t i = 0; i < m.store.size(); i++)
-            m.store[i] = std::hypot(store[2 * i], nChannels > 1 ? store[2 * i + 1] : 0.0);
+            m.store[i] = std::hypot(store[nChannels * i], nChannels > 1 ? store[nChannels * i + 1] : 0.0);
         return m;
     }
+    // <filename>.mrc of the first channel (Image::saveMRC, Image.cu:138-163 with
+    // MRC::save, MRC.cpp:206-210), cell lengths from the model; defined below.
+    template <class ModelT> int saveMRC(const char* filename, const ModelT* model, int w, int h, int thick) const;
 };
 
 // MRC2000 writer (mode 2, float data); the reference writes _pt.mrc and .mrc
@@ -166,6 +243,17 @@ inline int write_mrc(const char* path, const double* v, int nx, int ny, int nz, 
     for (size_t i = 0; i < n; i++) buf[i] = (float)v[i];
     ok = ok && std::fwrite(buf.data(), sizeof(float), n, f) == n;
     return (std::fclose(f) == 0 && ok) ? 0 : -1;
+}
+
+template <class ModelT>
+int Image::saveMRC(const char* filename, const ModelT* model, int w, int h, int thick) const {
+    const size_t n = (size_t)w * h * thick;
+    if (n > (size_t)width * height * thickness) return -1;
+    std::vector<double> v(n);
+    for (size_t i = 0; i < n; i++) v[i] = store[(size_t)nChannels * i];
+    const std::string p = std::string(filename) + ".mrc";
+    return write_mrc(p.c_str(), v.data(), w, h, thick, model ? model->getA() : 0.f, model ? model->getB() : 0.f,
+                     model ? model->getC() : 0.f);
 }
 
 // ----------------------------------------------------------------- Microscope
@@ -345,6 +433,135 @@ public:
     int fnPotentialBuilder() { return 42; }
 };
 inline int nPotentialBuilder = 0;
+
+// ----------------------------------------------------------------- Dispatcher
+// The run command of an LMultiSlice XML file (Dispatcher.h, Dispatcher.cu:
+// 25-159): <action> <io> fileNameInput, fileNameOutput </io> <image> nx, ny,
+// dpa </image> <slicing> radiuc, numberslices </slicing> <microscope>
+// aperture [mrad], cs [mm], defocus [nm], keV </microscope> </action>.
+struct Command {
+    std::string fileNameInput, fileNameOutput;
+    int nx = 0, ny = 0, numberSlices = 0;
+    float dpa = 0, radiuc = 0, aperture = 0, cs = 0, defocus = 0, keV = 0;
+};
+
+// Text of the element reached by the path of child names (the first match
+// at each level), trimmed; "" when absent.  Enough XML for the command files:
+// comments and declarations are skipped, attributes are ignored.
+inline std::string xml_child_value(const std::string& doc, std::initializer_list<const char*> path) {
+    size_t lo = 0, hi = doc.size();
+    for (const char* name : path) {
+        const std::string open = std::string("<") + name, close = std::string("</") + name + ">";
+        size_t at = lo, start = std::string::npos;
+        while ((at = doc.find(open, at)) != std::string::npos && at < hi) {
+            const char next = at + open.size() < doc.size() ? doc[at + open.size()] : '\0';
+            if (next == '>' || next == ' ' || next == '\t' || next == '\n' || next == '\r') {
+                start = doc.find('>', at);
+                break;
+            }
+            at += open.size();
+        }
+        if (start == std::string::npos || start >= hi) return "";
+        const size_t end = doc.find(close, start);
+        if (end == std::string::npos || end > hi) return "";
+        lo = start + 1;
+        hi = end;
+    }
+    std::string v = doc.substr(lo, hi - lo);
+    const size_t b = v.find_first_not_of(" \t\r\n"), e = v.find_last_not_of(" \t\r\n");
+    return b == std::string::npos ? "" : v.substr(b, e - b + 1);
+}
+
+// Dispatcher::parseCommand: 0, or -1 with the reference's message when the
+// file cannot be read or a field is empty.
+inline int parse_command(const char* fileNameXML, Command& command) {
+    std::ifstream fin(fileNameXML, std::ios::binary);
+    if (!fin.is_open()) {
+        std::cerr << "XML File with name [" << fileNameXML << "] doesn't exist." << std::endl;
+        return -1;
+    }
+    std::string doc((std::istreambuf_iterator<char>(fin)), std::istreambuf_iterator<char>());
+    // drop comments and declarations
+    for (size_t a; (a = doc.find("<!--")) != std::string::npos;) {
+        const size_t b = doc.find("-->", a + 4);
+        doc.erase(a, b == std::string::npos ? std::string::npos : b + 3 - a);
+    }
+    for (size_t a; (a = doc.find("<?")) != std::string::npos;) {
+        const size_t b = doc.find("?>", a + 2);
+        doc.erase(a, b == std::string::npos ? std::string::npos : b + 2 - a);
+    }
+    struct Field {
+        const char* group;
+        const char* name;
+        const char* label;
+    };
+    auto get = [&](const Field& f, std::string& out) -> int {
+        out = xml_child_value(doc, {"action", f.group, f.name});
+        if (out.empty()) {
+            std::cerr << "Empty " << f.label << " field!" << std::endl;
+            return -1;
+        }
+        return 0;
+    };
+    std::string v;
+    if (get({"io", "fileNameInput", "file input"}, command.fileNameInput)) return -1;
+    if (get({"io", "fileNameOutput", "file output"}, command.fileNameOutput)) return -1;
+    if (get({"image", "nx", "nx"}, v)) return -1;
+    command.nx = std::atoi(v.c_str());
+    if (get({"image", "ny", "ny"}, v)) return -1;
+    command.ny = std::atoi(v.c_str());
+    if (get({"image", "dpa", "dpa"}, v)) return -1;
+    command.dpa = (float)std::atof(v.c_str());
+    if (get({"slicing", "radiuc", "radiuc"}, v)) return -1;
+    command.radiuc = (float)std::atof(v.c_str());
+    if (get({"slicing", "numberslices", "numberslices"}, v)) return -1;
+    command.numberSlices = std::atoi(v.c_str());
+    if (get({"microscope", "aperture", "aperture"}, v)) return -1;
+    command.aperture = (float)std::atof(v.c_str());
+    if (get({"microscope", "cs", "cs"}, v)) return -1;
+    command.cs = (float)std::atof(v.c_str());
+    if (get({"microscope", "defocus", "defocus"}, v)) return -1;
+    command.defocus = (float)std::atof(v.c_str());
+    if (get({"microscope", "keV", "keV"}, v)) return -1;
+    command.keV = (float)std::atof(v.c_str());
+    return 0;
+}
+
+// Dispatcher::Run (Dispatcher.cu:161-242): read the command and the model
+// (.coo, getModelType), build the potential and write <output>_pt.mrc, run
+// the plane wave through the slices and the objective lens, write |psi| of
+// the image wave to <output>.mrc.  0 / -1.
+inline int dispatcher_run(const char* fileNameXML) {
+    Command command;
+    if (parse_command(fileNameXML, command) == -1) return -1;
+    const std::string in = command.fileNameInput;
+    if (in.size() < 4 || in.compare(in.size() - 4, 4, ".coo") != 0) {
+        std::cout << "Can not read file " << in << "!!! (only .coo models are supported)" << std::endl;
+        return -1;
+    }
+    amodel::Model model;
+    if (model.read(in.c_str()) == -1) {
+        std::cout << "Can not read file " << in << "!!!" << std::endl;
+        return -1;
+    }
+    std::cout << "Read file model [" << in << "] successful." << std::endl;
+    std::cout << "Image size = " << command.nx << "x" << command.ny << std::endl;
+    std::cout << "Number slides = " << command.numberSlices << std::endl;
+    std::cout << "dpa = " << command.dpa << std::endl;
+    ModelPotential potential(&model, (size_t)command.nx, (size_t)command.ny, (size_t)command.numberSlices,
+                             command.dpa, command.radiuc);
+    if (potential.calculatePotentialGrid() == -1) return -1;
+    if (potential.savePotential(command.fileNameOutput.c_str()) == -1) return -1;
+    ModelSimulated simulated(&potential, (size_t)command.nx, (size_t)command.ny, (size_t)command.numberSlices,
+                             command.dpa);
+    Microscope microscope(command.keV, command.cs, command.aperture, command.defocus);
+    Image result((size_t)command.nx, (size_t)command.ny, 1, sizeof(double), 2);
+    if (simulated.imageCalculation(&result, &microscope) == -1) return -1;
+    const Image module = result.getModule();
+    if (module.saveMRC(command.fileNameOutput.c_str(), &model, command.nx, command.ny, 1) == -1) return -1;
+    std::cout << "Calculation for [" << fileNameXML << "] finished successful." << std::endl << std::endl;
+    return 0;
+}
 
 }  // namespace lms
 }  // namespace msx
