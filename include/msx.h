@@ -149,6 +149,18 @@ MSX_API void msx_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uin
 MSX_API int msx_amorphous(double a, double b, double c, int32_t Z, int64_t natoms, double min_dist, uint64_t seed,
                           int64_t max_candidates, msx_atom* out, int64_t* nplaced);
 
+/* ---- layer stacking sequence -------------------------------------------- */
+/* Replaces Kirkland's parlay (slicelib.cpp:558-713, called by mulslice.cpp:
+ * 197): "2(abc)d" -> layer indices 0 1 2 0 1 2 3 (a = 0 ... z = 25, A = 26 ...
+ * Z = 51; n(...) repeats a group, nested up to 100 deep).  islice: host
+ * array of nsmax entries; lmax: largest allowed layer index; *nslice gets the
+ * count.  Returns parlay's codes: 0 ok, -1 layer out of range, -2 missing
+ * left parenthesis, -3 nested too deep, -5 unmatched right parenthesis, -6
+ * invalid character (the end of the string after white space counts as one,
+ * as in the reference), -7 too many layers, -8 incomplete sequence; the
+ * message is in msx_last_error().  NULL pointers give MSX_EINVAL (-1). */
+MSX_API int msx_parlay(const char* c, int32_t* islice, int nsmax, int lmax, int32_t* nslice);
+
 /* ---- 2D FFT ----------------------------------------------------------- */
 /* cfpix::fft() / cfpix::ifft() (cfpix.cpp:115-145), batched: sign = +1 is
  * fft() (exponent +i, unscaled), sign = -1 is ifft() (exponent -i, scaled by

@@ -90,7 +90,7 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx
            "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
-           "msx_partial_coherence", "msx_amorphous"]
+           "msx_partial_coherence", "msx_amorphous", "msx_parlay"]
 
 _lib = None
 _lock = threading.Lock()
@@ -146,6 +146,7 @@ def load():
         "msx_ctf_image": (i, [vp, vp, i, i, i, d, d, d, d, d, d, vp, vp, i, i]),
         "msx_partial_coherence": (i, [vp, vp, vp, vp, d, d, d, d, i, vp, vp, vp, i, i]),
         "msx_amorphous": (i, [d, d, d, C.c_int32, C.c_int64, d, C.c_uint64, C.c_int64, vp, vp]),
+        "msx_parlay": (i, [C.c_char_p, vp, i, i, vp]),
         "msx_unet_create": (i, [vp, i, i, C.c_uint64, C.POINTER(vp)]), "msx_unet_destroy": (i, [vp]),
         "msx_unet_param_count": (C.c_int64, [vp]), "msx_unet_params": (i, [vp, vp, i]),
         "msx_unet_set_params": (i, [vp, vp, i]), "msx_unet_forward": (i, [vp, vp, vp, i, i, i, i]),
@@ -602,6 +603,16 @@ def context():
         if _default is None:
             _default = Context(0)
         return _default
+
+
+def parlay(sequence, nlayers=52, nsmax=1 << 20):
+    """Layer indices of a Kirkland stacking sequence (msx_parlay): "2(abc)d"
+    -> [0, 1, 2, 0, 1, 2, 3]; letters past nlayers are out of range.  Raises
+    MsxError with parlay's code on a malformed sequence."""
+    out = np.zeros(nsmax, np.int32)
+    n = C.c_int32(0)
+    _check(load().msx_parlay(sequence.encode(), C.c_void_p(out.ctypes.data), nsmax, nlayers - 1, C.byref(n)))
+    return out[:n.value].copy()
 
 
 def amorphous(cell, Z=6, density_g_cm3=2.0, min_dist=1.4, seed=0, mass=None, max_tries=200):

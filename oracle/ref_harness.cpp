@@ -46,5 +46,9 @@ REF_API double ref_chi_multi(double cs, double df, double c12a, double c12b, dou
     p[pC23b] = (float)c23b;
     return chi(p, alx, aly, 1);
 }
+/* the stacking-sequence parser (slicelib.cpp:558-713), messages off */
+REF_API int ref_parlay(const char *c, int *islice, int nsmax, int lmax, int *nslice) {
+    return parlay(c, islice, nsmax, lmax, nslice, 0);
+}
 REF_API double ref_rangauss(unsigned long *seed) { return rangauss(seed); }
 REF_API double ref_ranflat(unsigned long *seed) { return ranflat(seed); }

@@ -83,6 +83,7 @@ def ref():
             ("ref_freqn", None, [_fp, _fp, _fp, i, d]),
             ("ref_chi_csdf", d, [d, d, d, d]),
             ("ref_chi_multi", d, [d, d, d, d, d, d, d, d]),
+            ("ref_parlay", i, [C.c_char_p, _ip, i, i, C.POINTER(C.c_int)]),
             ("ref_rangauss", d, [C.POINTER(C.c_ulong)]),
             ("ref_ranflat", d, [C.POINTER(C.c_ulong)]),
         ]:
