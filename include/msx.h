@@ -161,6 +161,28 @@ MSX_API int msx_amorphous(double a, double b, double c, int32_t Z, int64_t natom
  * message is in msx_last_error().  NULL pointers give MSX_EINVAL (-1). */
 MSX_API int msx_parlay(const char* c, int32_t* islice, int nsmax, int lmax, int32_t* nslice);
 
+/* ---- Kirkland floating-point TIFF files ----------------------------------- */
+/* Replaces floatTIFF::write / floatTIFF::read (floatTIFF.cpp:1263-1838,
+ * 324-346), the layer and wave files of Kirkland's programs: an 8-bit preview
+ * (square pixels, scaled by rmin..rmax and, for the imaginary half,
+ * imin..imax), the FP32 image and 512 FP32 parameters (param[0] = npix).
+ * pix: ny rows of nx floats, x fastest (floatTIFF's operator()(ix, iy)); a
+ * complex image has npix = 2 and its real and imaginary halves side by side
+ * (nx = 2 * width).  dx, dy: pixel size (<= 0: square).  The file matches the
+ * reference's byte for byte except the 20-byte date.  Returns 0, or
+ * floatTIFF::write's codes: -1 rmax == rmin, -2 npix not 1 or 2, -6 cannot
+ * open, -8 cannot write; MSX_EINVAL for arrays below 2 x 2. */
+MSX_API int msx_tiff_write(const char* path, const float* pix, int nx, int ny, int npix, const float* params,
+                           int nparam, float rmin, float rmax, float imin, float imax, float dx, float dy);
+/* Reads the first FP32 image of a TIFF file (either byte order) and the
+ * parameter image after it: *nx, *ny, *npix (rounded param[0]; 1 without
+ * parameters) and params (512 floats, may be NULL).  pix == NULL only
+ * queries the size; else capacity (floats) must hold nx * ny.  Returns 0,
+ * or -1 cannot open, -2/-3 not a TIFF file, -4 more than one sample per
+ * pixel, -5 no FP32 image, -7 short read. */
+MSX_API int msx_tiff_read(const char* path, float* pix, int64_t capacity, int32_t* nx, int32_t* ny, int32_t* npix,
+                          float* params);
+
 /* ---- 2D FFT ----------------------------------------------------------- */
 /* cfpix::fft() / cfpix::ifft() (cfpix.cpp:115-145), batched: sign = +1 is
  * fft() (exponent +i, unscaled), sign = -1 is ifft() (exponent -i, scaled by

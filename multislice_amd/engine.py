@@ -90,7 +90,7 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx
            "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
-           "msx_partial_coherence", "msx_amorphous", "msx_parlay"]
+           "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read"]
 
 _lib = None
 _lock = threading.Lock()
@@ -147,6 +147,9 @@ def load():
         "msx_partial_coherence": (i, [vp, vp, vp, vp, d, d, d, d, i, vp, vp, vp, i, i]),
         "msx_amorphous": (i, [d, d, d, C.c_int32, C.c_int64, d, C.c_uint64, C.c_int64, vp, vp]),
         "msx_parlay": (i, [C.c_char_p, vp, i, i, vp]),
+        "msx_tiff_write": (i, [C.c_char_p, vp, i, i, i, vp, i, C.c_float, C.c_float, C.c_float, C.c_float,
+                               C.c_float, C.c_float]),
+        "msx_tiff_read": (i, [C.c_char_p, vp, C.c_int64, vp, vp, vp, vp]),
         "msx_unet_create": (i, [vp, i, i, C.c_uint64, C.POINTER(vp)]), "msx_unet_destroy": (i, [vp]),
         "msx_unet_param_count": (C.c_int64, [vp]), "msx_unet_params": (i, [vp, vp, i]),
         "msx_unet_set_params": (i, [vp, vp, i]), "msx_unet_forward": (i, [vp, vp, vp, i, i, i, i]),
@@ -613,6 +616,44 @@ def parlay(sequence, nlayers=52, nsmax=1 << 20):
     n = C.c_int32(0)
     _check(load().msx_parlay(sequence.encode(), C.c_void_p(out.ctypes.data), nsmax, nlayers - 1, C.byref(n)))
     return out[:n.value].copy()
+
+
+def tiff_write(path, image, params=None, dx=0.0, dy=0.0, rmin=None, rmax=None, imin=None, imax=None):
+    """Write a Kirkland floating-point TIFF (msx_tiff_write).  image: real
+    (ny, nx) float array, or complex (ny, width) -> real and imaginary halves
+    side by side (npix = 2).  The preview range defaults to the data range."""
+    im = np.asarray(image)
+    if np.iscomplexobj(im):
+        pix = np.ascontiguousarray(np.concatenate([im.real, im.imag], axis=1), np.float32)
+        npix = 2
+    else:
+        pix = np.ascontiguousarray(im, np.float32)
+        npix = 1
+    half = pix[:, : pix.shape[1] // npix]
+    rmin = float(half.min()) if rmin is None else rmin
+    rmax = float(half.max()) if rmax is None else rmax
+    if npix == 2:
+        ih = pix[:, pix.shape[1] // 2:]
+        imin = float(ih.min()) if imin is None else imin
+        imax = float(ih.max()) if imax is None else imax
+    prm = np.zeros(512, np.float32) if params is None else np.ascontiguousarray(params, np.float32)
+    _check(load().msx_tiff_write(os.fsencode(path), C.c_void_p(pix.ctypes.data), pix.shape[1], pix.shape[0], npix,
+                                 C.c_void_p(prm.ctypes.data), len(prm), rmin, rmax, imin or 0.0, imax or 0.0, dx, dy))
+
+
+def tiff_read(path):
+    """Read a Kirkland floating-point TIFF (msx_tiff_read): returns (image,
+    params); a complex file (npix = 2) comes back as a complex (ny, nx/2) array."""
+    nx, ny, npix = C.c_int32(), C.c_int32(), C.c_int32()
+    prm = np.zeros(512, np.float32)
+    _check(load().msx_tiff_read(os.fsencode(path), None, 0, C.byref(nx), C.byref(ny), C.byref(npix), None))
+    pix = np.zeros((ny.value, nx.value), np.float32)
+    _check(load().msx_tiff_read(os.fsencode(path), C.c_void_p(pix.ctypes.data), pix.size, C.byref(nx), C.byref(ny),
+                                C.byref(npix), C.c_void_p(prm.ctypes.data)))
+    if npix.value == 2:
+        h = nx.value // 2
+        return pix[:, :h] + 1j * pix[:, h:], prm
+    return pix, prm
 
 
 def amorphous(cell, Z=6, density_g_cm3=2.0, min_dist=1.4, seed=0, mass=None, max_tries=200):

@@ -2,7 +2,8 @@
 # Build the reference's own CPU code into oracle/_ref/ (test infrastructure
 # only; outputs are git-ignored).  Needs the reference tree, so it runs in the
 # development container, never where the reference sources are absent (the prebuilt files are used there).
-#   libkirkland_ref.so : ImageSimulation/multislice/slicelib.cpp + ref_harness.cpp
+#   libkirkland_ref.so : ImageSimulation/multislice/slicelib.cpp + floatTIFF.cpp
+#                        + ref_harness.cpp
 #   libfftw2f_ref.so   : ImageSimulation/fftw-2.1.5/fftw/*.c (float mode)
 #                        + ref_fftw2_harness.c
 # The reference's mulslice.cpp/cfpix.cpp are NOT built: they need the FFTW 3
@@ -17,7 +18,7 @@ FW="$REF/ImageSimulation/fftw-2.1.5/fftw"
 mkdir -p "$OUT"
 TMP=$(mktemp -d)
 trap 'rm -rf "$TMP"' EXIT
-g++ -O2 -fPIC -shared -w -I"$MS" "$MS/slicelib.cpp" "$HERE/ref_harness.cpp" \
+g++ -O2 -fPIC -shared -w -I"$MS" "$MS/slicelib.cpp" "$MS/floatTIFF.cpp" "$HERE/ref_harness.cpp" \
     -o "$OUT/libkirkland_ref.so"
 objs=()
 for f in "$FW"/*.c; do

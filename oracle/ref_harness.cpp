@@ -5,15 +5,18 @@
  * TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle
  * (oracle/msx_oracle.cpp) against the compiled reference.
  *
- * Two reference units are compiled from their sources under the reference
- * tree, unmodified:
+ * Reference units compiled from their sources under the reference tree,
+ * unmodified:
  *   ImageSimulation/multislice/slicelib.cpp  (Kirkland slicelib: wavelength,
- *       sigma, bessi0/bessk0, vzatom, featom, freqn, chi, ranflat, rangauss)
+ *       sigma, bessi0/bessk0, vzatom, featom, freqn, chi, parlay, ranflat,
+ *       rangauss)
+ *   ImageSimulation/multislice/floatTIFF.cpp (Kirkland's float TIFF files)
  *   ImageSimulation/fftw-2.1.5/fftw/*.c     (the FFTW 2.1.5 library the
  *       reference vendors, built in float mode)
  * This file only forwards C-callable symbols to them.
  */
 #include "slicelib.hpp"
+#include "floatTIFF.hpp"
 
 #define REF_API extern "C" __attribute__((visibility("default")))
 
@@ -49,6 +52,32 @@ REF_API double ref_chi_multi(double cs, double df, double c12a, double c12b, dou
 /* the stacking-sequence parser (slicelib.cpp:558-713), messages off */
 REF_API int ref_parlay(const char *c, int *islice, int nsmax, int lmax, int *nslice) {
     return parlay(c, islice, nsmax, lmax, nslice, 0);
+}
+/* floatTIFF write / read (floatTIFF.cpp:1263-1838, 324-346) on a plain
+ * float array [ny][nx]; returns the class's own codes (1 = success). */
+REF_API int ref_tiff_write(const char *path, const float *pix, int nx, int ny, int npix, const float *params,
+                           int nparam, float rmin, float rmax, float imin, float imax, float dx, float dy) {
+    floatTIFF t;
+    t.resize(nx, ny);
+    t.setnpix(npix);
+    for (int iy = 0; iy < ny; iy++)
+        for (int ix = 0; ix < nx; ix++) t(ix, iy) = pix[ix + iy * nx];
+    t.zeroParam();
+    for (int i = 0; i < nparam && i < t.maxParam(); i++) t.setParam(i, params[i]);
+    return t.write(path, rmin, rmax, imin, imax, dx, dy);
+}
+REF_API int ref_tiff_read(const char *path, float *pix, long cap, int *nx, int *ny, int *npix, float *params) {
+    floatTIFF t;
+    const int rc = t.read(path);
+    if (rc != 1) return rc;
+    *nx = t.nx();
+    *ny = t.ny();
+    *npix = t.getnpix();
+    if ((long)t.nx() * t.ny() > cap) return -100;
+    for (int iy = 0; iy < t.ny(); iy++)
+        for (int ix = 0; ix < t.nx(); ix++) pix[ix + iy * t.nx()] = t(ix, iy);
+    for (int i = 0; i < t.maxParam(); i++) params[i] = t.getParam(i);
+    return rc;
 }
 REF_API double ref_rangauss(unsigned long *seed) { return rangauss(seed); }
 REF_API double ref_ranflat(unsigned long *seed) { return ranflat(seed); }

@@ -84,6 +84,10 @@ def ref():
             ("ref_chi_csdf", d, [d, d, d, d]),
             ("ref_chi_multi", d, [d, d, d, d, d, d, d, d]),
             ("ref_parlay", i, [C.c_char_p, _ip, i, i, C.POINTER(C.c_int)]),
+            ("ref_tiff_write", i, [C.c_char_p, _fp, i, i, i, _fp, i, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.c_float, C.c_float]),
+            ("ref_tiff_read", i, [C.c_char_p, _fp, C.c_long, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int), _fp]),
             ("ref_rangauss", d, [C.POINTER(C.c_ulong)]),
             ("ref_ranflat", d, [C.POINTER(C.c_ulong)]),
         ]:
