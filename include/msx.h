@@ -183,6 +183,19 @@ MSX_API int msx_tiff_write(const char* path, const float* pix, int nx, int ny, i
 MSX_API int msx_tiff_read(const char* path, float* pix, int64_t capacity, int32_t* nx, int32_t* ny, int32_t* npix,
                           float* params);
 
+/* ---- Musli output files (no reference C ABI: Musli's C# writes them) ------ */
+/* A STEM image as Musli's FlushSTEMfile (MSProject.cs:697-790) writes it:
+ * stem_n * stem_m doubles in scan order, then the 160-byte trailer "~~~
+ * scan_x scan_y thickness pi/2 pi/2 gamma kV stem_n stem_m thickness 0"
+ * (scan extents and thickness in Angstrom, gamma in rad, kV = E/1000). */
+MSX_API int msx_musli_stem_write(const char* path, const double* image, int stem_n, int stem_m, double scan_x,
+                                 double scan_y, double thickness, double gamma, double kev);
+/* A wave as Musli's SaveWav (MSProject.cs:522-596): n * m complex doubles
+ * (interleaved, x fastest: Musli's layout), then the trailer "___ a b
+ * thickness pi/2 pi/2 gamma kV n m thickness 0". */
+MSX_API int msx_musli_wave_write(const char* path, const double* wave, int n, int m, double a, double b,
+                                 double thickness, double gamma, double kev);
+
 /* ---- 2D FFT ----------------------------------------------------------- */
 /* cfpix::fft() / cfpix::ifft() (cfpix.cpp:115-145), batched: sign = +1 is
  * fft() (exponent +i, unscaled), sign = -1 is ifft() (exponent -i, scaled by

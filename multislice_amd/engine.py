@@ -90,7 +90,8 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx
            "msx_unet_set_params", "msx_unet_forward", "msx_unet_gradient", "msx_unet_train_step",
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
-           "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read"]
+           "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read",
+           "msx_musli_stem_write", "msx_musli_wave_write"]
 
 _lib = None
 _lock = threading.Lock()
@@ -150,6 +151,8 @@ def load():
         "msx_tiff_write": (i, [C.c_char_p, vp, i, i, i, vp, i, C.c_float, C.c_float, C.c_float, C.c_float,
                                C.c_float, C.c_float]),
         "msx_tiff_read": (i, [C.c_char_p, vp, C.c_int64, vp, vp, vp, vp]),
+        "msx_musli_stem_write": (i, [C.c_char_p, vp, i, i, d, d, d, d, d]),
+        "msx_musli_wave_write": (i, [C.c_char_p, vp, i, i, d, d, d, d, d]),
         "msx_unet_create": (i, [vp, i, i, C.c_uint64, C.POINTER(vp)]), "msx_unet_destroy": (i, [vp]),
         "msx_unet_param_count": (C.c_int64, [vp]), "msx_unet_params": (i, [vp, vp, i]),
         "msx_unet_set_params": (i, [vp, vp, i]), "msx_unet_forward": (i, [vp, vp, vp, i, i, i, i]),
@@ -654,6 +657,24 @@ def tiff_read(path):
         h = nx.value // 2
         return pix[:, :h] + 1j * pix[:, h:], prm
     return pix, prm
+
+
+def musli_stem_write(path, image, scan_x, scan_y, thickness, kev, gamma=np.pi / 2):
+    """Musli .stem file (msx_musli_stem_write): image (stem_m, stem_n) doubles
+    in scan order (x fastest), scan extents and thickness in Angstrom."""
+    im = np.ascontiguousarray(image, np.float64)
+    m, n = im.shape
+    _check(load().msx_musli_stem_write(os.fsencode(path), C.c_void_p(im.ctypes.data), n, m, float(scan_x),
+                                       float(scan_y), float(thickness), float(gamma), float(kev)))
+
+
+def musli_wave_write(path, wave, a, b, thickness, kev, gamma=np.pi / 2):
+    """Musli .wav file (msx_musli_wave_write): wave (m, n) complex, x fastest
+    (Musli's layout; an engine wave [ix][iy] is wave.T)."""
+    w = np.ascontiguousarray(wave, np.complex128)
+    m, n = w.shape
+    _check(load().msx_musli_wave_write(os.fsencode(path), C.c_void_p(w.ctypes.data), n, m, float(a), float(b),
+                                       float(thickness), float(gamma), float(kev)))
 
 
 def amorphous(cell, Z=6, density_g_cm3=2.0, min_dist=1.4, seed=0, mass=None, max_tries=200):
