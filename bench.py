@@ -116,7 +116,9 @@ def traffic_from_profile(batch, tuning):
         return None
     with open(path) as f:
         d = json.load(f)
-    if tuning and d.get("col_wpb") not in (None, tuning.get("col_wpb")):
+    # the built-in column tile at 512^2 is 8 columns (tile 0 means that width)
+    if tuning and (d.get("col_wpb") not in (None, tuning.get("col_wpb"))
+                   or d.get("col_tile", 8) != (tuning.get("col_tile") or 8)):
         return None
     return d["hbm_bytes_per_launch"]
 
@@ -170,7 +172,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch slice loops directly (no HIP graph replay)")
     ap.add_argument("--no-tune", action="store_true", help="built-in launch shapes instead of the autotuner's")
-    ap.add_argument("--shapes", default="", help="ROW_WPB,COL_WPB,BATCH: fixed launch shapes instead of the autotuner's")
+    ap.add_argument("--shapes", default="",
+                    help="ROW_WPB,COL_WPB,BATCH[,COL_TILE]: fixed launch shapes instead of the autotuner's")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -188,8 +191,8 @@ def main():
     ctx.set_graphs(not a.no_graphs)
     tuning = None
     if a.shapes:
-        rw, cw, bt = (int(v) for v in a.shapes.split(","))
-        tuning = dict(row_wpb=rw, col_wpb=cw, batch=bt, ms_per_slice=0.0)
+        v = [int(x) for x in a.shapes.split(",")]
+        tuning = dict(row_wpb=v[0], col_wpb=v[1], batch=v[2], ms_per_slice=0.0, col_tile=v[3] if len(v) > 3 else 0)
         ctx.set_tuning(N, prec=msx.F32, tuning=tuning)
     elif not a.no_tune:
         tuning = ctx.autotune(N, prec=msx.F32, max_batch=a.batch or 512)

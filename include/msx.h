@@ -79,13 +79,16 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
 /* Autotuner: times the slice loop of an nx x ny grid on synthetic waves for
  * candidate launch shapes and keeps the fastest in the context (used by every
  * later slice loop and msx_stem of that grid and precision): waves per
- * work-group of the row and column sweeps (0 = built-in heuristic) and the
- * STEM batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
+ * work-group of the row and column sweeps (0 = built-in heuristic), the
+ * column-sweep tile width (columns per work-group; 0 = built-in width, else
+ * the built-in one or its alternative for this line length) and the STEM
+ * batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
  * ms_per_slice is the time of one wave through one slice at that batch.
  * msx_set_tuning installs a saved result (NULL clears it). */
 typedef struct msx_tuning {
     int32_t row_wpb, col_wpb, batch;
     double ms_per_slice;
+    int32_t col_tile;
 } msx_tuning;
 MSX_API int msx_autotune(msx_ctx ctx, int nx, int ny, int prec, int max_batch, msx_tuning* best);
 MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuning* t);

@@ -405,8 +405,9 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
     CHK(ctx->twiddles<C>(nx, &twx));
     CHK(ctx->twiddles<C>(ny, &twy));
     const Re inv = (Re)1 / ((Re)nx * (Re)ny);
-    const int groups = col_groups<C>(nx, ny);
     const msx_tuning* tu = ctx->tuning(nx, ny, sizeof(Re) == 8 ? MSX_F64 : MSX_F32);
+    const int tile = tu ? tu->col_tile : 0;
+    const int groups = col_groups<C>(nx, ny, tile);
     int ick = 0;
     for (int s = 0; s < st->nslice; s++) {
         const int l = st->layer[s];
@@ -437,6 +438,7 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         c.py = tb.py + (long long)l * ny;
         c.ops = kColProp | kColStore;
         if (tu) c.wpb = tu->col_wpb;
+        c.tile = tile;
         bool is_ck = det && ick < det->nck && det->ck[ick] == s + 1;
         if (is_ck) {
             c.ops |= kColDetect;
@@ -528,9 +530,10 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
 // work whatever the values) for each candidate launch shape, with the
 // per-launch event timing of the profiling mode, and keeps the fastest:
 // For every STEM batch size (16 to 512 waves per launch, ten sizes): the waves per
-// work-group of the row and of the column sweep (1, 2, 4, 8; each sweep on
-// its own).  The batch chosen is the smallest within 3% of the best
-// probe-slice rate, with its own work-group shapes.
+// work-group of the row sweep and, for the column sweep, the tile width (the
+// built-in one and its alternative) with its waves per work-group (1, 2, 4,
+// 8; each sweep on its own).  The batch chosen is the smallest within 3% of
+// the best probe-slice rate, with its own shapes.
 template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_batch, msx_tuning* res) {
     using Re = typename real_of<C>::type;
     const int prec = sizeof(Re) == 8 ? MSX_F64 : MSX_F32;
@@ -569,8 +572,8 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     const bool had = ctx->tune.count(key) > 0;
     const msx_tuning saved = had ? ctx->tune[key] : msx_tuning{};
     // times (ms per launch) of the row and column sweeps for one shape
-    auto trial = [&](int rw, int cw, int nb, double* row_ms, double* col_ms) -> int {
-        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0};
+    auto trial = [&](int rw, int cw, int tile, int nb, double* row_ms, double* col_ms) -> int {
+        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0, tile};
         ctx->prof = true;
         for (int rep = 0; rep < 2; rep++) {  // the first pass warms up
             for (int k = 0; k < MSX_PROF_KINDS; k++) ctx->prof_n[k] = 0, ctx->prof_ms[k] = 0, ctx->prof_bytes[k] = 0;
@@ -581,30 +584,36 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
         *col_ms = ctx->prof_ms[MSX_PROF_COL] / std::max(1LL, ctx->prof_n[MSX_PROF_COL]);
         return MSX_OK;
     };
+    int tdef, talt;
+    col_tiles<C>(nx, &tdef, &talt);
+    std::vector<int> tiles{tdef};
+    if (talt != tdef) tiles.push_back(talt);
     int rc = MSX_OK;
-    msx_tuning best{1, 1, 0, 0.0};
+    msx_tuning best{1, 1, 0, 0.0, 0};
     do {
-        // per batch size: the fastest waves-per-work-group of each sweep, and
+        // per batch size: the fastest waves-per-work-group of the row sweep,
+        // the fastest (tile, waves-per-work-group) of the column sweep, and
         // the resulting probe-slice rate
         double rate[kNB] = {0}, top = 0.0;
-        int rw_of[kNB], cw_of[kNB];
-        for (int i = 0; i < kNB; i++) rw_of[i] = cw_of[i] = 1;
+        int rw_of[kNB], cw_of[kNB], tile_of[kNB];
+        for (int i = 0; i < kNB; i++) rw_of[i] = cw_of[i] = 1, tile_of[i] = tdef;
         for (int i = 0; i < kNB && rc == MSX_OK; i++) {
             if (batches[i] > bmax) break;
             double brow = 1e30, bcol = 1e30;
-            for (int w : {1, 2, 4, 8}) {
-                double rm, cm;
-                if ((rc = trial(w, w, batches[i], &rm, &cm)) != MSX_OK) break;
-                if (rm < brow) brow = rm, rw_of[i] = w;
-                if (cm < bcol) bcol = cm, cw_of[i] = w;
-            }
+            for (int tile : tiles)
+                for (int w : {1, 2, 4, 8}) {
+                    double rm, cm;
+                    if ((rc = trial(w, w, tile, batches[i], &rm, &cm)) != MSX_OK) break;
+                    if (tile == tdef && rm < brow) brow = rm, rw_of[i] = w;
+                    if (cm < bcol) bcol = cm, cw_of[i] = w, tile_of[i] = tile;
+                }
             rate[i] = batches[i] / (brow + bcol);
             top = std::max(top, rate[i]);
         }
         if (rc != MSX_OK) break;
         for (int i = 0; i < kNB && batches[i] <= bmax; i++)
             if (rate[i] >= 0.97 * top) {
-                best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i]};
+                best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i], tile_of[i]};
                 break;
             }
     } while (0);
@@ -815,7 +824,7 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
     }
     CHK(ctx->wave.ensure(sizeof(C) * img * batch));
     C* w_dev = static_cast<C*>(ctx->wave.p);
-    const int groups = col_groups<C>(nx, ny);
+    const int groups = col_groups<C>(nx, ny, tu ? tu->col_tile : 0);
     CHK(ctx->partial.ensure(sizeof(double) * (size_t)ndet * batch * groups));
     // device: results [nck][ndet][nprobe], the current batch's results
     // [nck][ndet][batch] (one address for every batch, so the slice-loop graph
@@ -1504,6 +1513,11 @@ MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuni
     }
     if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < 0 || t->col_wpb > 64 || t->batch < 0)
         return fail(MSX_EINVAL, "bad tuning values");
+    int tdef, talt;
+    if (prec == MSX_F64) col_tiles<double2>(nx, &tdef, &talt);
+    else col_tiles<float2>(nx, &tdef, &talt);
+    if (t->col_tile != 0 && t->col_tile != tdef && t->col_tile != talt)
+        return fail(MSX_EINVAL, "column tile %d: this grid takes 0, %d or %d", t->col_tile, tdef, talt);
     ctx->tune[std::make_tuple(nx, ny, prec)] = *t;
     return MSX_OK;
 }

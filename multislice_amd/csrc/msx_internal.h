@@ -30,7 +30,8 @@ enum ColMode : int {
 #define MSX_DECL_N(NN)                                                                         \
     template <typename C> hipError_t launch_row_##NN(int mode, const RowArgs<C>& a, hipStream_t s); \
     template <typename C> hipError_t launch_col_##NN(int mode, const ColArgs<C>& a, hipStream_t s); \
-    template <typename C> int col_groups_##NN(int ny);
+    template <typename C> int col_groups_##NN(int ny, int tile);                                 \
+    template <typename C> int col_tiles_##NN(int* def, int* alt);
 MSX_FOR_EACH_N(MSX_DECL_N)
 #undef MSX_DECL_N
 
@@ -58,12 +59,21 @@ template <typename C> hipError_t launch_col(int n, int mode, const ColArgs<C>& a
 }
 // number of column groups (work-groups per wave) of the column sweep for an
 // nx-long line and ny columns
-template <typename C> int col_groups(int nx, int ny) {
+template <typename C> int col_groups(int nx, int ny, int tile = 0) {
 #define MSX_CASE(NN) \
-    if (nx == NN) return col_groups_##NN<C>(ny);
+    if (nx == NN) return col_groups_##NN<C>(ny, tile);
     MSX_FOR_EACH_N(MSX_CASE)
 #undef MSX_CASE
     return 0;
+}
+// the built-in column tile width of an nx-long line and the alternative the
+// autotuner may pick (equal when there is none)
+template <typename C> void col_tiles(int nx, int* def, int* alt) {
+    *def = *alt = 0;
+#define MSX_CASE(NN) \
+    if (nx == NN) col_tiles_##NN<C>(def, alt);
+    MSX_FOR_EACH_N(MSX_CASE)
+#undef MSX_CASE
 }
 
 // potential splat (msx_potential.hip)

@@ -41,11 +41,19 @@ constexpr int row_lpw() { return kT >= 256 ? 1 : 256 / kT; }
 #ifndef MSX_COL_THREADS
 #define MSX_COL_THREADS 512
 #endif
-template <typename C> constexpr int col_tw() {
-    int tw = kN == 512 ? MSX_COL_TW512 : MSX_COL_TW;
+template <typename C> constexpr int fit_tw(int tw) {
     while (tw > 1 && (kTc * tw > MSX_COL_THREADS || (long)kN * (tw + 1) * (long)sizeof(C) > 80 * 1024)) tw /= 2;
     return tw;
 }
+template <typename C> constexpr int col_tw() { return fit_tw<C>(kN == 512 ? MSX_COL_TW512 : MSX_COL_TW); }
+// the other width the autotuner may choose (half or double the built-in one;
+// equal to it where neither fits or the line is short)
+template <typename C> constexpr int col_tw_alt() {
+    if (kN < 256) return col_tw<C>();
+    const int d = col_tw<C>(), up = fit_tw<C>(2 * d);
+    return up != d ? up : (d > 4 ? d / 2 : d);
+}
+template <typename C> constexpr int tw_of(int tile) { return tile == col_tw_alt<C>() ? col_tw_alt<C>() : col_tw<C>(); }
 
 template <typename K> hipError_t set_smem(K kernel, size_t bytes) {
     if (bytes > 64 * 1024)
@@ -77,8 +85,7 @@ template <typename C, int IN, int OUT, bool VREAL = false> hipError_t row_go(con
     return hipGetLastError();
 }
 
-template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<C>& a0, hipStream_t s) {
-    constexpr int TW = col_tw<C>();
+template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(const ColArgs<C>& a0, hipStream_t s) {
     ColArgs<C> a = a0;
     const int groups = (a.ny + TW - 1) / TW;
     a.wpb = pick_wpb(a.wpb, (long long)groups * a.nb, 2048);
@@ -95,6 +102,14 @@ template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kTc * TW), smem, s, a);
     return hipGetLastError();
+}
+// dispatch on the tile width chosen in the arguments
+template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<C>& a, hipStream_t s) {
+    constexpr int D = col_tw<C>(), A = col_tw_alt<C>();
+    if constexpr (A != D) {
+        if (a.tile == A) return col_go_tw<C, IN, OUT, OPS, A>(a, s);
+    }
+    return col_go_tw<C, IN, OUT, OPS, D>(a, s);
 }
 }  // namespace
 
@@ -143,16 +158,23 @@ template <typename C> hipError_t MSX_CAT(launch_col_, MSX_N)(int mode, const Col
     return hipErrorInvalidValue;
 }
 
-template <typename C> int MSX_CAT(col_groups_, MSX_N)(int ny) {
-    constexpr int TW = col_tw<C>();
-    return (ny + TW - 1) / TW;
+template <typename C> int MSX_CAT(col_groups_, MSX_N)(int ny, int tile) {
+    const int tw = tw_of<C>(tile);
+    return (ny + tw - 1) / tw;
+}
+template <typename C> int MSX_CAT(col_tiles_, MSX_N)(int* def, int* alt) {
+    *def = col_tw<C>();
+    *alt = col_tw_alt<C>();
+    return 0;
 }
 
 template hipError_t MSX_CAT(launch_row_, MSX_N)<float2>(int, const RowArgs<float2>&, hipStream_t);
 template hipError_t MSX_CAT(launch_row_, MSX_N)<double2>(int, const RowArgs<double2>&, hipStream_t);
 template hipError_t MSX_CAT(launch_col_, MSX_N)<float2>(int, const ColArgs<float2>&, hipStream_t);
 template hipError_t MSX_CAT(launch_col_, MSX_N)<double2>(int, const ColArgs<double2>&, hipStream_t);
-template int MSX_CAT(col_groups_, MSX_N)<float2>(int);
-template int MSX_CAT(col_groups_, MSX_N)<double2>(int);
+template int MSX_CAT(col_groups_, MSX_N)<float2>(int, int);
+template int MSX_CAT(col_groups_, MSX_N)<double2>(int, int);
+template int MSX_CAT(col_tiles_, MSX_N)<float2>(int*, int*);
+template int MSX_CAT(col_tiles_, MSX_N)<double2>(int*, int*);
 
 }  // namespace msx

@@ -251,6 +251,7 @@ template <typename C> struct ColArgs {
     C* out;                // store target (nullptr: in place)
     int ny, nb, ops;
     int wpb;               // waves handled by one work-group (loop with prefetch)
+    int tile;              // columns per work-group (0: the built-in width for this length)
     // prune != 0: columns with ky2[iy] >= k2max are skipped entirely (their
     // k-space values are all masked to zero, so nothing downstream reads them)
     int prune;

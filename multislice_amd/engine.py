@@ -63,7 +63,8 @@ class Probe(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("row_wpb", C.c_int32), ("col_wpb", C.c_int32), ("batch", C.c_int32), ("ms_per_slice", C.c_double)]
+    _fields_ = [("row_wpb", C.c_int32), ("col_wpb", C.c_int32), ("batch", C.c_int32), ("ms_per_slice", C.c_double),
+                ("col_tile", C.c_int32)]
 
 
 class Job(C.Structure):
@@ -249,12 +250,13 @@ class Context:
         """Pick launch shapes for an nx x ny grid (msx_autotune); returns them as a dict."""
         t = Tuning()
         _check(load().msx_autotune(self.h, nx, ny or nx, prec, max_batch, C.byref(t)))
-        return dict(row_wpb=t.row_wpb, col_wpb=t.col_wpb, batch=t.batch, ms_per_slice=t.ms_per_slice)
+        return dict(row_wpb=t.row_wpb, col_wpb=t.col_wpb, batch=t.batch, ms_per_slice=t.ms_per_slice,
+                    col_tile=t.col_tile)
 
     def set_tuning(self, nx, ny=None, prec=F32, tuning=None):
         """Install a saved autotuner result (dict as returned by autotune), or clear it (None)."""
         t = None if tuning is None else C.byref(Tuning(tuning["row_wpb"], tuning["col_wpb"], tuning["batch"],
-                                                       tuning.get("ms_per_slice", 0.0)))
+                                                       tuning.get("ms_per_slice", 0.0), tuning.get("col_tile", 0)))
         _check(load().msx_set_tuning(self.h, nx, ny or nx, prec, t))
 
     def set_graphs(self, enable=True):

@@ -111,6 +111,7 @@ def test_autotune_keeps_results_and_reports_shapes(si_stack):
     print("tuned 256^2 FP32:", best)
     assert best["row_wpb"] in (1, 2, 4, 8) and best["col_wpb"] in (1, 2, 4, 8)
     assert 16 <= best["batch"] <= 256 and best["ms_per_slice"] > 0
+    assert best["col_tile"] in (16, 32)  # the built-in 16-column tile at 256^2 or its alternative
     got = c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20], batch=50)
     np.testing.assert_array_equal(got, ref)
     # an explicit shape, and the tuned default batch
@@ -119,4 +120,36 @@ def test_autotune_keeps_results_and_reports_shapes(si_stack):
     np.testing.assert_allclose(c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20]), ref,
                                rtol=1e-12, atol=0)
     c.set_tuning(256, prec=msx.F32, tuning=None)
+    c.close()
+
+
+@pytest.mark.parametrize("n,tiles", [(256, (16, 32)), (512, (8, 16))])
+def test_column_tile_choice_keeps_results(n, tiles):
+    """Both column tile widths of a grid give the same exit waves (each column
+    is transformed by the same code; bit-identical in this build) and STEM
+    detector sums equal to FP32 rounding (the two instantiations may fuse
+    products into FMAs differently, and their FP64 partial sums are grouped
+    differently)."""
+    import torch
+    c = msx.Context(0)
+    rng = np.random.default_rng(n)
+    nl = 3
+    t = np.exp(1j * rng.random((nl, n, n)) * 2).astype(np.complex64)
+    td = torch.from_numpy(t).cuda()
+    cz, layer = np.full(nl, 2.0), np.arange(7) % nl
+    a = 0.1 * n
+    lam = msx.wavelength(300.0)
+    det = [(0.0, 0.01 / lam), (0.02 / lam, 0.06 / lam)]
+    pos = rng.uniform(0, a, (20, 2))
+    waves, stems = [], []
+    for tile in tiles:
+        c.set_tuning(n, prec=msx.F32, tuning=dict(row_wpb=0, col_wpb=0, batch=8, col_tile=tile))
+        psi = torch.ones((3, n, n), dtype=torch.complex64, device="cuda")
+        c.multislice(td, cz, layer, psi, a, a, 300.0)
+        waves.append(psi.cpu().numpy())
+        stems.append(c.stem(td, cz, layer, a, a, 300.0, dict(alpha_max=0.02), pos, det, [4, 7], batch=8))
+    np.testing.assert_array_equal(waves[0], waves[1])
+    np.testing.assert_allclose(stems[0], stems[1], rtol=1e-6, atol=0)
+    with pytest.raises(msx.MsxError):
+        c.set_tuning(n, prec=msx.F32, tuning=dict(row_wpb=0, col_wpb=0, batch=8, col_tile=3))
     c.close()
