@@ -148,6 +148,34 @@ struct msx_ctx_s {
     std::map<int, DevBuf> tw32, tw64;
     DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf;
 
+    // extra streams for job groups that run concurrently (msx_multislice_batch),
+    // each with its own wave, pointer-table and propagator-table buffers
+    struct Lane {
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        DevBuf wave, aux, tabs;
+    };
+    std::vector<Lane> lanes;
+    hipEvent_t lanes_go = nullptr;
+    int ensure_lanes(int n) {
+        if (!lanes_go) HIPCHK(hipEventCreateWithFlags(&lanes_go, hipEventDisableTiming));
+        while ((int)lanes.size() < n) {
+            Lane l;
+            HIPCHK(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+            lanes.push_back(std::move(l));
+        }
+        return MSX_OK;
+    }
+    // make lane i the context's stream and buffers (and back again)
+    void swap_lane(int i) {
+        Lane& l = lanes[i];
+        std::swap(stream, l.stream);
+        std::swap(wave, l.wave);
+        std::swap(aux, l.aux);
+        std::swap(tabs, l.tabs);
+    }
+
     // launch shapes picked by msx_autotune / msx_set_tuning, per (nx, ny, prec)
     std::map<std::tuple<int, int, int>, msx_tuning> tune;
     const msx_tuning* tuning(int nx, int ny, int prec) const {
@@ -190,6 +218,14 @@ struct msx_ctx_s {
         drop_graphs();
         for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf})
             b->release();
+        for (Lane& l : lanes) {
+            l.wave.release();
+            l.aux.release();
+            l.tabs.release();
+            if (l.done) hipEventDestroy(l.done);
+            if (l.stream) hipStreamDestroy(l.stream);
+        }
+        if (lanes_go) hipEventDestroy(lanes_go);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (own) hipStreamDestroy(own);
@@ -643,7 +679,8 @@ struct BatchItem {
 };
 
 template <typename C>
-int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g) {
+int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g,
+                std::vector<std::vector<const void*>>& keep, bool sync) {
     using Re = typename real_of<C>::type;
     std::stable_sort(g.begin(), g.end(),
                      [](const BatchItem& a, const BatchItem& b) { return a.st->nslice > b.st->nslice; });
@@ -670,7 +707,9 @@ int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g) {
     std::vector<int> active(S + 1, 0);
     for (int s = 0; s <= S; s++)
         for (int i = 0; i < nb; i++) active[s] += g[i].st->nslice > s;
-    std::vector<const C*> tab((size_t)S * nb, nullptr);
+    // host copy kept by the caller until the stream has consumed it
+    keep.emplace_back((size_t)S * nb, nullptr);
+    std::vector<const void*>& tab = keep.back();
     for (int s = 0; s < S; s++)
         for (int i = 0; i < active[s]; i++) {
             const msx_stack* js = g[i].st;
@@ -722,7 +761,7 @@ int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g) {
     }
     for (int i = 0; i < nb; i++)
         HIPCHK(hipMemcpyAsync(jobs[g[i].job].psi, w + i * img, sizeof(C) * img, hipMemcpyDeviceToDevice, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (sync) HIPCHK(hipStreamSynchronize(ctx->stream));
     return MSX_OK;
 }
 
@@ -741,6 +780,7 @@ template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs
             if (a->cz[a->layer[s]] != b->cz[b->layer[s]]) return false;
         return true;
     };
+    std::vector<std::vector<BatchItem>> groups;
     for (int i = 0; i < njobs; i++) {
         if (used[i]) continue;
         std::vector<BatchItem> g{{i, jobs[i].stack}};
@@ -753,9 +793,41 @@ template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs
             g.push_back({k, jobs[k].stack});
             used[k] = true;
         }
-        CHK(batch_group<C>(ctx, jobs, g));
+        groups.push_back(std::move(g));
     }
-    return MSX_OK;
+    std::vector<std::vector<const void*>> keep;
+    // Groups are independent: up to kLanes of them run at once, each on its
+    // own stream (groups of small grids alone leave most of the GPU idle).
+    // Profiling times launches one at a time, so it keeps one stream.
+    constexpr int kLanes = 4;
+    const int nl = ctx->prof ? 1 : std::min<int>(kLanes, (int)groups.size());
+    if (nl <= 1) {
+        for (auto& g : groups) CHK(batch_group<C>(ctx, jobs, g, keep, true));
+        return MSX_OK;
+    }
+    CHK(ctx->ensure_lanes(nl));
+    // twiddle tables are shared: create them before any lane runs
+    for (auto& g : groups) {
+        const C* tw;
+        CHK(ctx->twiddles<C>(g[0].st->nx, &tw));
+        CHK(ctx->twiddles<C>(g[0].st->ny, &tw));
+    }
+    HIPCHK(hipEventRecord(ctx->lanes_go, ctx->stream));
+    for (int l = 0; l < nl; l++) HIPCHK(hipStreamWaitEvent(ctx->lanes[l].stream, ctx->lanes_go, 0));
+    int rc = MSX_OK;
+    for (size_t gi = 0; gi < groups.size() && rc == MSX_OK; gi++) {
+        const int l = (int)(gi % nl);
+        ctx->swap_lane(l);
+        rc = batch_group<C>(ctx, jobs, groups[gi], keep, false);
+        ctx->swap_lane(l);
+    }
+    // the context stream waits for every lane; the call returns when all are done
+    for (int l = 0; l < nl; l++) {
+        HIPCHK(hipEventRecord(ctx->lanes[l].done, ctx->lanes[l].stream));
+        HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->lanes[l].done, 0));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return rc;
 }
 
 int check_stack(const msx_stack* st) {
