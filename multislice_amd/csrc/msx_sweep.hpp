@@ -99,11 +99,16 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
 
     TwReg<PL, C> tw;
     if constexpr (IN != 0 || OUT != 0) load_twiddles<PL>(tw, t, p.tw);
+    // The inverse-transform scale is folded into the transmission factors
+    // (a * (scale * t) instead of (a * scale) * t): once per work-group for a
+    // shared transmission, per wave for a per-wave one, so both give the same
+    // arithmetic.
+    const bool fold = (p.trans || p.trans_tab) && p.scale != (Re)1;
     C tr[E];
     if (p.trans) {
         const C* trow = p.trans + (long long)ix * N;
 #pragma unroll
-        for (int e = 0; e < E; ++e) tr[e] = trow[t + e * T];
+        for (int e = 0; e < E; ++e) tr[e] = fold ? cscale(trow[t + e * T], p.scale) : trow[t + e * T];
     }
 
     unsigned inb = ~0u;  // bit e: element t + e*T lies inside the y band
@@ -170,10 +175,10 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
             // jobs batched in one launch: each wave has its own transmission
             const C* trow = p.trans_tab[b] + (long long)ix * N;
 #pragma unroll
-            for (int e = 0; e < E; ++e) tr[e] = trow[t + e * T];
+            for (int e = 0; e < E; ++e) tr[e] = fold ? cscale(trow[t + e * T], p.scale) : trow[t + e * T];
         }
         if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
-        if (p.scale != (Re)1) {
+        if (p.scale != (Re)1 && !fold) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = cscale(a[e], p.scale);
         }
