@@ -146,3 +146,39 @@ def test_displacement_host_equals_device(ctx):
     for f in ("x", "y", "z"):
         assert np.abs(dev[f] - host[f]).max() < 1e-13
     assert np.array_equal(dev["Z"], host["Z"])
+
+
+def test_stem_largest_grid(ctx):
+    """The largest supported grid (2048^2: column tiles of 2 columns, the
+    longest FFT plan) through the detector epilogue, against the oracle."""
+    n, a = 2048, 4 * 3.905 * 2
+    rng = np.random.default_rng(20)
+    t = np.exp(1j * 0.3 * rng.random((2, n, n)))  # a weak random phase object
+    cz, layer = np.full(2, 1.9525), np.arange(2)
+    lam = msx.wavelength(KEV)
+    det = [(0.0, 0.012 / lam), (0.03 / lam, 0.09 / lam)]
+    pos = [(3.1, 4.7), (12.0, 9.5)]
+    ref = O.stem(t, cz, layer, a, a, KEV, 0.021, pos, det, [2], cs=1e4, df=5.0)
+    got = ctx.stem(t.astype(np.complex64), cz, layer, a, a, KEV, dict(alpha_max=0.021, cs=1e4, df=5.0), pos, det,
+                   [2])
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    per_det = (np.abs(got - ref).max(axis=2) / np.abs(ref).max(axis=2)).max()
+    print(f"STEM 2048^2 F32: max rel err vs oracle FP64 {err:.2e} (per detector {per_det:.2e})")
+    assert err < 1e-5
+    assert per_det < 1e-4
+
+
+def test_stem_rejects(ctx, small):
+    cell, V, t64, nsl = small
+    cz, layer = np.full(nsl, 1.9525), np.arange(nsl)
+    args = (t64, cz, layer, cell[0], cell[1], KEV, dict(alpha_max=0.02))
+    with pytest.raises(msx.MsxError):  # no probe
+        ctx.stem(*args, np.zeros((0, 2)), [(0.0, 1.0)], [nsl])
+    with pytest.raises(msx.MsxError):  # no detector
+        ctx.stem(*args, [(1.0, 1.0)], np.zeros((0, 2)), [nsl])
+    with pytest.raises(msx.MsxError):  # checkpoints must increase within 1..nslice
+        ctx.stem(*args, [(1.0, 1.0)], [(0.0, 1.0)], [nsl, 1])
+    with pytest.raises(msx.MsxError):
+        ctx.stem(*args, [(1.0, 1.0)], [(0.0, 1.0)], [nsl + 1])
+    with pytest.raises(msx.MsxError):  # more detectors than the epilogue holds
+        ctx.stem(*args, [(1.0, 1.0)], [(0.0, 1.0)] * 9, [nsl])
