@@ -181,7 +181,7 @@ def main():
         copy_probe()
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; the path shards over probes/configs "
+        print(json.dumps({"error": "msx runs on one GPU only; the path shards over probes/configs "
                                    "across GPUs by design but no multi-GPU run is provided"}))
         sys.exit(2)
 

@@ -126,7 +126,8 @@ def test_autotune_keeps_results_and_reports_shapes(si_stack):
 @pytest.mark.parametrize("n,tiles", [(256, (16, 32)), (512, (8, 16))])
 def test_column_tile_choice_keeps_results(n, tiles):
     """Both column tile widths of a grid give the same exit waves (each column
-    is transformed by the same code; bit-identical in this build) and STEM
+    is transformed by the same code, which compiles to the same arithmetic in
+    both tile instantiations, so bit for bit) and STEM
     detector sums equal to FP32 rounding (the two instantiations may fuse
     products into FMAs differently, and their FP64 partial sums are grouped
     differently)."""
