@@ -15,6 +15,8 @@
  *       reference vendors, built in float mode)
  * This file only forwards C-callable symbols to them.
  */
+#include <cstdio>
+
 #include "slicelib.hpp"
 #include "floatTIFF.hpp"
 
@@ -49,9 +51,12 @@ REF_API double ref_chi_multi(double cs, double df, double c12a, double c12b, dou
     p[pC23b] = (float)c23b;
     return chi(p, alx, aly, 1);
 }
-/* the stacking-sequence parser (slicelib.cpp:558-713), messages off */
+/* the stacking-sequence parser (slicelib.cpp:558-713); exitflag 0 returns the
+ * error code instead of exiting (its messages still print) */
 REF_API int ref_parlay(const char *c, int *islice, int nsmax, int lmax, int *nslice) {
-    return parlay(c, islice, nsmax, lmax, nslice, 0);
+    const int rc = parlay(c, islice, nsmax, lmax, nslice, 0);
+    std::fflush(stdout);  // its error messages go out while the caller still captures them
+    return rc;
 }
 /* floatTIFF write / read (floatTIFF.cpp:1263-1838, 324-346) on a plain
  * float array [ny][nx]; returns the class's own codes (1 = success). */
