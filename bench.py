@@ -151,6 +151,11 @@ def surrogate_rate(msx, torch, ctx, value):
     if reps_:
         with open(reps_[-1]) as f:
             ev = json.load(f)
+        passes = ev["network"].get("forward_passes_per_image", 1)
+        if passes != 1:  # the evaluated surrogate averages several network passes per image
+            t_all = 4 * per_config + passes * t_unet
+            out.update({"surrogate_4config": round(3600.0 / t_all, 3),
+                        "gpu_time_ratio": round(64 * per_config / t_all, 2), "unet_passes_per_image": passes})
         out["accuracy"] = {"source": os.path.relpath(reps_[-1], ROOT),
                            "held_out_specimens": ev["eval_specimens"],
                            "surrogate_rel_rms_mean": round(ev["eval"]["surrogate_rel_rms_mean"], 4),
