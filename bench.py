@@ -125,10 +125,11 @@ def traffic_from_profile(batch, tuning):
 
 def surrogate_rate(msx, torch, ctx, value):
     """Converged ADF images/hr of this workload (128x128 scan, 100 slices):
-    plain 64-config multislice vs the surrogate (4 configs + one U-Net pass,
-    4 levels / 32 channels, timed here on a 128x128 image).  The surrogate's
-    accuracy comes from the evaluation report committed under profiles/ (its
-    own scaled-down specimen set, named there)."""
+    plain 64-config multislice vs the surrogate (4 configs + the U-Net passes
+    per image that the evaluated surrogate makes, 4 levels / 32 channels, one
+    pass timed here on a 128x128 image).  The surrogate's accuracy and pass
+    count come from the evaluation report committed under profiles/ (its own
+    scaled-down specimen set, named there)."""
     import glob
     net = msx.UNet(ctx, 4, 32, seed=0)
     x = torch.rand((1, SCAN, SCAN), device="cuda") + 0.5
