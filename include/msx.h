@@ -79,7 +79,9 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
 /* Autotuner: times the slice loop of an nx x ny grid on synthetic waves for
  * candidate launch shapes and keeps the fastest in the context (used by every
  * later slice loop and msx_stem of that grid and precision): waves per
- * work-group of the row and column sweeps (0 = built-in heuristic), the
+ * work-group of the row and column sweeps (0 = built-in heuristic; a column
+ * value of -1 to -8 selects the balanced schedule, in which -col_wpb
+ * work-groups per resident slot share the (column group, wave) units), the
  * column-sweep tile width (columns per work-group; 0 = built-in width, else
  * the built-in one or its alternative for this line length) and the STEM
  * batch (smallest within 3% of the best rate, at most max_batch; 0: 512).

@@ -567,8 +567,9 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
 // per-launch event timing of the profiling mode, and keeps the fastest:
 // For every STEM batch size (16 to 512 waves per launch, ten sizes): the waves per
 // work-group of the row sweep and, for the column sweep, the tile width (the
-// built-in one and its alternative) with its waves per work-group (1, 2, 4,
-// 8; each sweep on its own).  The batch chosen is the smallest within 3% of
+// built-in one and its alternative) with its schedule: chunked with 1, 2, 4
+// or 8 waves per work-group, or balanced with 1 or 2 work-groups per
+// resident slot (each sweep on its own).  The batch chosen is the smallest within 3% of
 // the best probe-slice rate, with its own shapes.
 template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_batch, msx_tuning* res) {
     using Re = typename real_of<C>::type;
@@ -637,10 +638,12 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
             if (batches[i] > bmax) break;
             double brow = 1e30, bcol = 1e30;
             for (int tile : tiles)
-                for (int w : {1, 2, 4, 8}) {
+                for (int w : {1, 2, 4, 8, -1, -2}) {
+                    // w < 0: the column sweep's balanced schedule (the row
+                    // sweep is timed at positive w only)
                     double rm, cm;
-                    if ((rc = trial(w, w, tile, batches[i], &rm, &cm)) != MSX_OK) break;
-                    if (tile == tdef && rm < brow) brow = rm, rw_of[i] = w;
+                    if ((rc = trial(w > 0 ? w : 8, w, tile, batches[i], &rm, &cm)) != MSX_OK) break;
+                    if (tile == tdef && w > 0 && rm < brow) brow = rm, rw_of[i] = w;
                     if (cm < bcol) bcol = cm, cw_of[i] = w, tile_of[i] = tile;
                 }
             rate[i] = batches[i] / (brow + bcol);
@@ -1583,7 +1586,7 @@ MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuni
         ctx->tune.erase(std::make_tuple(nx, ny, prec));
         return MSX_OK;
     }
-    if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < 0 || t->col_wpb > 64 || t->batch < 0)
+    if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < -8 || t->col_wpb > 64 || t->batch < 0)
         return fail(MSX_EINVAL, "bad tuning values");
     int tdef, talt;
     if (prec == MSX_F64) col_tiles<double2>(nx, &tdef, &talt);

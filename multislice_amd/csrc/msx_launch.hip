@@ -2,6 +2,9 @@
 // length.  Compiled once per supported length with -DMSX_N=<N> (the build
 // runs these compilations in parallel); each object defines
 // launch_row_<N>, launch_col_<N> and col_groups_<N> for float2 and double2.
+#include <algorithm>
+#include <atomic>
+
 #include "msx_internal.h"
 
 #ifndef MSX_N
@@ -85,21 +88,61 @@ template <typename C, int IN, int OUT, bool VREAL = false> hipError_t row_go(con
     return hipGetLastError();
 }
 
+// work-groups of kernel k (block threads, smem bytes) resident on the whole
+// device at once, cached per device and LDS size in the caller's
+// per-kernel cache ((smem << 24) | (groups + 1); 0: empty)
+constexpr int kDev = 16;
+template <typename K>
+hipError_t resident_groups(K k, int threads, size_t smem, std::atomic<long long> (&cache)[kDev], long long* out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < kDev) {
+        const long long c = cache[dev].load(std::memory_order_relaxed);
+        if (c != 0 && (size_t)(c >> 24) == smem) {
+            *out = (c & 0xffffff) - 1;
+            return hipSuccess;
+        }
+    }
+    int per_cu = 0, cus = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k), threads, smem);
+    if (e != hipSuccess) return e;
+    e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    *out = std::max(1LL, (long long)per_cu * cus);
+    if (dev < kDev && *out < 0xffffff) cache[dev].store(((long long)smem << 24) | (*out + 1), std::memory_order_relaxed);
+    return hipSuccess;
+}
+
 template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(const ColArgs<C>& a0, hipStream_t s) {
     ColArgs<C> a = a0;
     const int groups = (a.ny + TW - 1) / TW;
-    a.wpb = pick_wpb(a.wpb, (long long)groups * a.nb, 2048);
-    const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
-    const unsigned grid = (unsigned)(groups * nchunk);
     constexpr size_t tile = (size_t)kN * (TW + 1) * sizeof(C);
     // double-buffer the exchanges when two tiles fit in the work-group LDS
     // limit (a launch asking for the full 160 KiB is refused)
     constexpr bool DB = MSX_COL_DB && 2 * tile <= 150 * 1024;
-    size_t smem = DB ? 2 * tile : tile;
-    if (smem < 2048) smem = 2048;  // detector reduction scratch
     auto k = col_sweep<C, kN, IN, OUT, TW, OPS, DB>;
-    hipError_t e = set_smem(k, smem);
-    if (e != hipSuccess) return e;
+    size_t smem = col_list_offset<C, kN, TW, DB>();  // exchange tiles, detector reduction scratch
+    unsigned grid;
+    if (a.wpb < 0) {
+        // balanced schedule: -wpb work-groups per resident slot, each with
+        // its list of (group, live mask) pairs after the exchange tiles
+        smem += (size_t)groups * sizeof(int2);
+        hipError_t e = set_smem(k, smem);
+        if (e != hipSuccess) return e;
+        static std::atomic<long long> cache[kDev];
+        long long slots = 0;
+        e = resident_groups(k, kTc * TW, smem, cache, &slots);
+        if (e != hipSuccess) return e;
+        const long long g = std::min(slots * std::min(-a.wpb, 8), (long long)groups * a.nb);
+        grid = (unsigned)std::max(1LL, g);
+    } else {
+        a.wpb = pick_wpb(a.wpb, (long long)groups * a.nb, 2048);
+        const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
+        grid = (unsigned)(groups * nchunk);
+        hipError_t e = set_smem(k, smem);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k, dim3(grid), dim3(kTc * TW), smem, s, a);
     return hipGetLastError();
 }

@@ -302,12 +302,48 @@ __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
     const int xcd = bid % kXcd, idx = bid / kXcd;
     return xcd * per + min(xcd, rem) + idx;
 }
+// Byte offset of the balanced schedule's group list in the column sweep's
+// dynamic LDS: after the exchange buffers (which also hold the detector
+// reduction scratch, at least 2 KiB).
+template <typename C, int N, int TW, bool DB> constexpr size_t col_list_offset() {
+    constexpr size_t tile = (size_t)N * (TW + 1) * sizeof(C);
+    constexpr size_t x = DB ? 2 * tile : tile;
+    return x < 2048 ? 2048 : x;
+}
+
+// Raw buffer stores (gfx9 descriptor word 3: 32-bit data, no swizzle).  A
+// store whose byte offset is at or beyond the descriptor's size is dropped
+// by the hardware; kBufDrop plus any in-line offset used here stays far
+// beyond every image size (at most 2048 x 2048 x 16 B = 64 MiB) and below
+// 4 GiB.
+constexpr int kBufWord3 = 0x00020000;
+constexpr unsigned kBufDrop = 0x80000000u;
+__device__ __forceinline__ void buf_store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    using u2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    using u4 = __attribute__((ext_vector_type(4))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
+}
+
+// Schedules (p.wpb):
+//   > 0  chunked: work-group = (column group g, waves [b0, b0 + wpb)), the
+//        groups of one chunk numbered consecutively;
+//   < 0  balanced: the grid is sized to the work-groups the chip holds at
+//        once, and work-group w takes the units (live group, wave) u = w,
+//        w + G, w + 2G, ... in wave-major order, so every work-group gets
+//        the same number of units (to one) and no partly filled last round
+//        of work-groups idles most of the chip; at each step the work-groups
+//        of one XCD hold consecutive groups of one wave, as in the chunked
+//        order.
 template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB>
 __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(ColArgs<C> p) {
     using Re = typename real_of<C>::type;
     using PL = ColPlan<N>;
     constexpr int T = PL::T, E = N / T;
     constexpr bool WAVE = false;  // a column is spread over several wavefronts
+    static_assert(TW <= 32, "live-column masks are 32 bits");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
     const int lw = threadIdx.x % TW, t = threadIdx.x / TW;
@@ -316,42 +352,102 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
     // renumber them so that consecutive logical groups (neighbouring columns,
     // which share L2 lines when TW*sizeof(C) < 128 B) run on the same XCD.
     const int bid = MSX_XCD_REMAP ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-    const int g = bid % groups, chunk = bid / groups;
-    const int b0 = chunk * p.wpb, b1 = min(p.nb, b0 + p.wpb);
-    const int iy_raw = g * TW + lw;
-    const int iy = iy_raw < p.ny ? iy_raw : p.ny - 1;
-    const bool live = iy_raw < p.ny && (!p.prune || p.ky2[iy] < p.k2max);
-    // dead lanes of a partly live group load from the group's first live
+    // bit c set: column gg*TW + c exists and is in band (or pruning is off).
+    // Dead lanes of a partly live group load from the group's first live
     // column (same cache lines as a live lane); their column is transformed
     // on its own, never stored and left out of the detector sums, so the
-    // values need no zeroing
-    int first_live = -1;
-    for (int c = TW - 1; c >= 0; --c) {
-        const int yc = g * TW + c;
-        if (yc < p.ny && (!p.prune || p.ky2[yc] < p.k2max)) first_live = c;
-    }
-    const int ld_col = live ? lw : (first_live >= 0 ? first_live : 0);
-    if (!__syncthreads_or(live)) {
-        // whole group outside the band: its detector partials are zero
-        if constexpr ((OPS & kColDetect) != 0) {
-            if ((int)threadIdx.x < p.ndet)
-                for (int b = b0; b < b1; ++b) p.partial[((long long)threadIdx.x * p.nb + b) * groups + g] = 0.0;
+    // values need no zeroing.
+    auto live_mask = [&](int gg) -> unsigned {
+        unsigned m = 0;
+        for (int c = 0; c < TW; ++c) {
+            const int y = gg * TW + c;
+            if (y < p.ny && (!p.prune || p.ky2[y] < p.k2max)) m |= 1u << c;
         }
-        return;
+        return m;
+    };
+    const bool bal = p.wpb < 0;
+    int2* glist = reinterpret_cast<int2*>(smem_raw + col_list_offset<C, N, TW, DB>());
+    int ngl = 0, gfix = 0, b0 = 0, b1 = 0;
+    unsigned mfix = 0;
+    if (bal) {
+        // stable partition of the groups: the live ones (group, mask) in
+        // ascending order at the front, the dead ones after them
+        __shared__ int cnt[2][16];
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (blockDim.x + 63) >> 6;
+        int nd = 0;
+        for (int base = 0; base < groups; base += blockDim.x) {
+            const int gg = base + threadIdx.x;
+            const unsigned m = gg < groups ? live_mask(gg) : 0u;
+            const bool lv = m != 0, dd = gg < groups && m == 0;
+            const unsigned long long bl = __ballot(lv), bd = __ballot(dd);
+            if (lane == 0) {
+                cnt[0][wv] = __popcll(bl);
+                cnt[1][wv] = __popcll(bd);
+            }
+            __syncthreads();
+            int pl = ngl, pd = nd, tl = ngl, td = nd;
+            for (int w = 0; w < nwv; ++w) {
+                if (w < wv) pl += cnt[0][w], pd += cnt[1][w];
+                tl += cnt[0][w], td += cnt[1][w];
+            }
+            const unsigned long long below = (1ull << lane) - 1ull;
+            if (lv) glist[pl + __popcll(bl & below)] = make_int2(gg, (int)m);
+            if (dd) glist[groups - 1 - (pd + __popcll(bd & below))] = make_int2(gg, 0);
+            __syncthreads();
+            ngl = __builtin_amdgcn_readfirstlane(tl);  // uniform (LDS sums of ballots)
+            nd = __builtin_amdgcn_readfirstlane(td);
+        }
+        if constexpr ((OPS & kColDetect) != 0) {
+            // the dead groups' detector partials are zero
+            const long long V = (long long)nd * p.nb;
+            if ((int)threadIdx.x < p.ndet)
+                for (long long v = bid; v < V; v += gridDim.x) {
+                    const int gd = glist[ngl + (int)(v % nd)].x;
+                    p.partial[((long long)threadIdx.x * p.nb + v / nd) * groups + gd] = 0.0;
+                }
+        }
+        if (ngl == 0 || (long long)bid >= (long long)ngl * p.nb) return;
+    } else {
+        gfix = bid % groups;
+        const int chunk = bid / groups;
+        b0 = chunk * p.wpb;
+        b1 = min(p.nb, b0 + p.wpb);
+        mfix = live_mask(gfix);
+        if (mfix == 0) {
+            // whole group outside the band: its detector partials are zero
+            if constexpr ((OPS & kColDetect) != 0) {
+                if ((int)threadIdx.x < p.ndet)
+                    for (int b = b0; b < b1; ++b) p.partial[((long long)threadIdx.x * p.nb + b) * groups + gfix] = 0.0;
+            }
+            return;
+        }
     }
+    // unit i of this work-group: (group, live mask, wave); false past the end
+    const long long units = (long long)ngl * p.nb;
+    auto unit = [&](int i, int& g, unsigned& m, int& b) -> bool {
+        if (bal) {
+            const long long u = bid + (long long)i * gridDim.x;
+            if (u >= units) return false;
+            const int2 e = glist[(int)(u % ngl)];
+            g = __builtin_amdgcn_readfirstlane(e.x);
+            m = (unsigned)__builtin_amdgcn_readfirstlane(e.y);
+            b = (int)(u / ngl);
+            return true;
+        }
+        g = gfix;
+        m = mfix;
+        b = b0 + i;
+        return b < b1;
+    };
     const LdsView<C> lds[2] = {{smem + lw, TW + 1, false}, {smem + N * (TW + 1) + lw, TW + 1, false}};
     constexpr bool gen = (OPS & kColGen) != 0;
     constexpr bool point = (OPS & (kColProp | kColMask | kColDetect | kColCtf)) != 0;
 
     TwReg<PL, C> tw;
     if constexpr (IN != 0 || OUT != 0) load_twiddles<PL>(tw, t, p.tw);
-    // per-element constants along x (registers), per-column constants along y
+    // per-element constants along x (registers)
     float kx2c[MSX_COL_CONST ? E : 1];
     C pxc[MSX_COL_CONST ? E : 1];
-    float ky2 = 0.f;
-    C py = mk<C>((Re)1, (Re)0);
-    if constexpr (point || gen) ky2 = p.ky2[iy];
-    if constexpr ((OPS & kColProp) != 0) py = p.py[iy];
     if constexpr (MSX_COL_CONST) {
         if constexpr (point || gen) {
 #pragma unroll
@@ -368,26 +464,55 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
     auto PX = [&](int e) -> C {
         if constexpr (MSX_COL_CONST) return pxc[e]; else return p.px[t + e * T];
     };
-    const double ky = (gen || (OPS & kColCtf)) ? (double)freq_index(iy, p.ny) / p.by : 0.0;
+    // per-column constants along y of a unit's group, loaded before the
+    // prefetch of the next unit (so that waiting for them never waits for it)
+    auto col_y = [&](int g) { return min(g * TW + lw, p.ny - 1); };
+    auto load_ky2 = [&](int g) -> float {
+        if constexpr (point || gen) return p.ky2[col_y(g)]; else return 0.f;
+    };
+    auto load_py = [&](int g) -> C {
+        if constexpr ((OPS & kColProp) != 0) return p.py[col_y(g)]; else return mk<C>((Re)1, (Re)0);
+    };
 
     // strided accesses: uniform per-(work-group, wave) base + 32-bit lane
     // offset (one image is far below 4 GiB)
-    const unsigned off0 = (unsigned)t * (unsigned)p.ny + (unsigned)ld_col;
     const unsigned ostep = (unsigned)T * (unsigned)p.ny;
-    auto load = [&](int b, C (&v)[E]) {
+    auto load = [&](int g, unsigned m, int b, C (&v)[E]) {
+        const int ld_col = ((m >> lw) & 1u) ? lw : __ffs(m) - 1;
         const C* col = p.psi + (long long)b * N * p.ny + g * TW;
+        const unsigned off0 = (unsigned)t * (unsigned)p.ny + (unsigned)ld_col;
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = col[off0 + e * ostep];
     };
 
+    int g, b;
+    unsigned m;
+    unit(0, g, m, b);  // every work-group that gets here has a unit
     C a[E], nxt[MSX_COL_PREFETCH ? E : 1];
-    if (MSX_COL_PREFETCH && !gen && b0 < b1) load(b0, a);
-    for (int b = b0; b < b1; ++b) {
+    if (MSX_COL_PREFETCH && !gen) load(g, m, b, a);
+    // Drain the constant loads (and the first tile) once: the wait counters
+    // count in issue order, and a load still pending at the loop entry makes
+    // the compiler wait for everything issued after it (the prefetch) in
+    // every iteration.
+    __builtin_amdgcn_s_waitcnt(0);
+    for (int i = 0;; ++i) {
+        int g2 = g, b2 = b;
+        unsigned m2 = m;
+        const bool more = unit(i + 1, g2, m2, b2);
+        if (!more) g2 = g, m2 = m, b2 = b;
+        const float ky2 = load_ky2(g);
+        const C py = load_py(g);
         if constexpr (MSX_COL_PREFETCH) {
-            if (!gen && b + 1 < b1) load(b + 1, nxt);
+            // unconditional: past the last unit it reloads the current tile
+            // (cache hits); a branch around the loads would make the
+            // compiler wait for them at the join
+            if (!gen) load(g2, m2, b2, nxt);
         } else {
-            if (!gen) load(b, a);
+            if (!gen) load(g, m, b, a);
         }
+        const int iy = col_y(g);
+        const bool live = (m >> lw) & 1u;
+        const double ky = (gen || (OPS & kColCtf)) ? (double)freq_index(iy, p.ny) / p.by : 0.0;
         if constexpr (gen) {
             const double2 pos = p.pos[b];
 #pragma unroll
@@ -475,16 +600,29 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
             }
         }
         if constexpr (OUT != 0) fft_line<PL, OUT, WAVE, DB, PL::P - 1>(a, t, lds, tw);
-        if ((p.ops & kColStore) && live) {
-            C* col = (p.out ? p.out : p.psi) + (long long)b * N * p.ny + g * TW + lw;
-            const unsigned so = (unsigned)t * (unsigned)p.ny;
+        {
+            // Buffer stores bounded by this wave's image: the lanes of dead
+            // columns (and every lane when the sweep stores nothing) aim past
+            // the bound, where the hardware drops the write.  No branch: a
+            // branch around the stores would make every later wait of the
+            // loop count them (the wait counters count in issue order and
+            // the compiler merges both paths).
+            C* img = (p.out ? p.out : p.psi) + (long long)__builtin_amdgcn_readfirstlane(b) * N * p.ny;
+            const int bytes = (p.ops & kColStore) ? N * p.ny * (int)sizeof(C) : 0;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(img, 0, bytes, kBufWord3);
+            const unsigned so = live ? ((unsigned)t * (unsigned)p.ny + (unsigned)(g * TW + lw)) * (unsigned)sizeof(C)
+                                     : kBufDrop;
 #pragma unroll
-            for (int e = 0; e < E; ++e) col[so + e * ostep] = a[e];
+            for (int e = 0; e < E; ++e) buf_store(a[e], rs, so + (unsigned)e * ostep * (unsigned)sizeof(C));
         }
+        if (!more) break;
         if constexpr (MSX_COL_PREFETCH) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = nxt[e];
         }
+        g = g2;
+        m = m2;
+        b = b2;
     }
 }
 

@@ -109,7 +109,7 @@ def test_autotune_keeps_results_and_reports_shapes(si_stack):
     ref = c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20], batch=50)
     best = c.autotune(256, prec=msx.F32, max_batch=256)
     print("tuned 256^2 FP32:", best)
-    assert best["row_wpb"] in (1, 2, 4, 8) and best["col_wpb"] in (1, 2, 4, 8)
+    assert best["row_wpb"] in (1, 2, 4, 8) and best["col_wpb"] in (1, 2, 4, 8, -1, -2)
     assert 16 <= best["batch"] <= 256 and best["ms_per_slice"] > 0
     assert best["col_tile"] in (16, 32)  # the built-in 16-column tile at 256^2 or its alternative
     got = c.stem(t, cz, layer, cell[0], cell[1], 200.0, probe, pos, det, [20], batch=50)
@@ -153,4 +153,42 @@ def test_column_tile_choice_keeps_results(n, tiles):
     np.testing.assert_allclose(stems[0], stems[1], rtol=1e-6, atol=0)
     with pytest.raises(msx.MsxError):
         c.set_tuning(n, prec=msx.F32, tuning=dict(row_wpb=0, col_wpb=0, batch=8, col_tile=3))
+    c.close()
+
+
+@pytest.mark.parametrize("nx,ny,tile,prec", [(512, 512, 8, msx.F32), (256, 256, 32, msx.F32),
+                                             (256, 16, 32, msx.F32), (512, 256, 0, msx.F64)])
+def test_balanced_column_schedule_keeps_results(nx, ny, tile, prec):
+    """The column sweep's balanced schedule (col_wpb < 0: one work-group per
+    resident slot, (column group, wave) units dealt out in wave-major order)
+    runs every column through the same code as the chunked schedule: exit
+    waves, partial-coherence-style CTF sweeps and STEM detector sums (whose
+    per-group partials it writes, zero for the groups outside the band) are
+    bit-identical.  256 x 16 with 32-column tiles has lanes past the image
+    edge, whose stores must be dropped."""
+    import torch
+    c = msx.Context(0)
+    rng = np.random.default_rng(nx + ny)
+    nl = 3
+    cdt, ndt = (torch.complex64, np.complex64) if prec == msx.F32 else (torch.complex128, np.complex128)
+    t = np.exp(1j * rng.random((nl, nx, ny)) * 2).astype(ndt)
+    td = torch.from_numpy(t).cuda()
+    cz, layer = np.full(nl, 2.0), np.arange(7) % nl
+    a, b = 0.1 * nx, 0.1 * ny
+    lam = msx.wavelength(300.0)
+    det = [(0.0, 0.01 / lam), (0.02 / lam, 0.06 / lam)]
+    pos = rng.uniform(0, min(a, b), (37, 2))
+    psi0 = torch.from_numpy((rng.random((5, nx, ny)) + 1j * rng.random((5, nx, ny))).astype(ndt)).cuda()
+    out = {}
+    for cw in (4, 1, -1, -2):
+        c.set_tuning(nx, ny, prec=prec, tuning=dict(row_wpb=0, col_wpb=cw, batch=16, col_tile=tile))
+        psi = psi0.clone()
+        c.multislice(td, cz, layer, psi, a, b, 300.0)
+        stem = c.stem(td, cz, layer, a, b, 300.0, dict(alpha_max=0.02), pos, det, [3, 7], batch=16)
+        out[cw] = (psi.cpu().numpy(), stem)
+        print(f"{nx}x{ny} tile {tile} col_wpb {cw}: |psi| sum {np.abs(out[cw][0]).sum():.6e}", flush=True)
+    for cw in (1, -1, -2):
+        np.testing.assert_array_equal(out[cw][0], out[4][0])
+        np.testing.assert_array_equal(out[cw][1], out[4][1])
+    assert np.isfinite(out[4][0]).all() and np.isfinite(out[4][1]).all() and (out[4][1][:, 0] > 0).all()
     c.close()
