@@ -79,12 +79,13 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
 /* Autotuner: times the slice loop of an nx x ny grid on synthetic waves for
  * candidate launch shapes and keeps the fastest in the context (used by every
  * later slice loop and msx_stem of that grid and precision): waves per
- * work-group of the row and column sweeps (0 = built-in heuristic; a column
- * value of -1 to -8 selects the balanced schedule, in which -col_wpb
- * work-groups per resident slot share the (column group, wave) units), the
+ * work-group of the row and column sweeps (row: 0 = built-in heuristic;
+ * column: -1 to -8 select the balanced schedule, in which -col_wpb
+ * work-groups per resident slot share the (column group, wave) units, and 0
+ * is that schedule with one work-group per slot), the
  * column-sweep tile width (columns per work-group; 0 = built-in width, else
  * the built-in one or its alternative for this line length) and the STEM
- * batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
+ * batch (smallest within 1% of the best rate, at most max_batch; 0: 512).
  * ms_per_slice is the time of one wave through one slice at that batch.
  * msx_set_tuning installs a saved result (NULL clears it). */
 typedef struct msx_tuning {

@@ -124,9 +124,11 @@ template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(con
     auto k = col_sweep<C, kN, IN, OUT, TW, OPS, DB>;
     size_t smem = col_list_offset<C, kN, TW, DB>();  // exchange tiles, detector reduction scratch
     unsigned grid;
-    if (a.wpb < 0) {
-        // balanced schedule: -wpb work-groups per resident slot, each with
-        // its list of (group, live mask) pairs after the exchange tiles
+    if (a.wpb <= 0) {
+        // balanced schedule (the built-in one): -wpb work-groups per resident
+        // slot (0: one), each with its list of (group, live mask) pairs after
+        // the exchange tiles
+        a.wpb = a.wpb < 0 ? a.wpb : -1;
         smem += (size_t)groups * sizeof(int2);
         hipError_t e = set_smem(k, smem);
         if (e != hipSuccess) return e;
@@ -137,7 +139,6 @@ template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(con
         const long long g = std::min(slots * std::min(-a.wpb, 8), (long long)groups * a.nb);
         grid = (unsigned)std::max(1LL, g);
     } else {
-        a.wpb = pick_wpb(a.wpb, (long long)groups * a.nb, 2048);
         const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
         grid = (unsigned)(groups * nchunk);
         hipError_t e = set_smem(k, smem);

@@ -255,7 +255,7 @@ template <typename C> struct ColArgs {
     int astig;
     C* out;                // store target (nullptr: in place)
     int ny, nb, ops;
-    int wpb;               // waves handled by one work-group (loop with prefetch)
+    int wpb;               // > 0: waves per work-group (chunked); < 0: balanced schedule (below)
     int tile;              // columns per work-group (0: the built-in width for this length)
     // prune != 0: columns with ky2[iy] >= k2max are skipped entirely (their
     // k-space values are all masked to zero, so nothing downstream reads them)
@@ -326,8 +326,17 @@ __device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, u
     using u4 = __attribute__((ext_vector_type(4))) unsigned;
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
 }
+// Raw buffer loads: an offset at or beyond the descriptor's size reads zero
+// without touching memory.
+template <typename C> __device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
 
-// Schedules (p.wpb):
+// Schedules (p.wpb; the launcher turns 0 into -1):
 //   > 0  chunked: work-group = (column group g, waves [b0, b0 + wpb)), the
 //        groups of one chunk numbered consecutively;
 //   < 0  balanced: the grid is sized to the work-groups the chip holds at
@@ -474,22 +483,27 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         if constexpr ((OPS & kColProp) != 0) return p.py[col_y(g)]; else return mk<C>((Re)1, (Re)0);
     };
 
-    // strided accesses: uniform per-(work-group, wave) base + 32-bit lane
-    // offset (one image is far below 4 GiB)
+    // strided accesses: buffer descriptor of the wave's image (uniform) +
+    // 32-bit lane byte offset (one image is at most 64 MiB).  With valid
+    // false the descriptor is empty: the loads return zeros and read nothing
+    // (the prefetch past a work-group's last unit, issued without a branch).
     const unsigned ostep = (unsigned)T * (unsigned)p.ny;
-    auto load = [&](int g, unsigned m, int b, C (&v)[E]) {
+    const int img_bytes = N * p.ny * (int)sizeof(C);
+    auto load = [&](int g, unsigned m, int b, bool valid, C (&v)[E]) {
         const int ld_col = ((m >> lw) & 1u) ? lw : __ffs(m) - 1;
-        const C* col = p.psi + (long long)b * N * p.ny + g * TW;
-        const unsigned off0 = (unsigned)t * (unsigned)p.ny + (unsigned)ld_col;
+        const C* img = p.psi + (long long)__builtin_amdgcn_readfirstlane(b) * N * p.ny;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<C*>(img), 0,
+                                                                             valid ? img_bytes : 0, kBufWord3);
+        const unsigned off0 = ((unsigned)t * (unsigned)p.ny + (unsigned)(g * TW + ld_col)) * (unsigned)sizeof(C);
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = col[off0 + e * ostep];
+        for (int e = 0; e < E; ++e) v[e] = buf_load<C>(rs, off0 + (unsigned)e * ostep * (unsigned)sizeof(C));
     };
 
     int g, b;
     unsigned m;
     unit(0, g, m, b);  // every work-group that gets here has a unit
     C a[E], nxt[MSX_COL_PREFETCH ? E : 1];
-    if (MSX_COL_PREFETCH && !gen) load(g, m, b, a);
+    if (MSX_COL_PREFETCH && !gen) load(g, m, b, true, a);
     // Drain the constant loads (and the first tile) once: the wait counters
     // count in issue order, and a load still pending at the loop entry makes
     // the compiler wait for everything issued after it (the prefetch) in
@@ -503,12 +517,12 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         const float ky2 = load_ky2(g);
         const C py = load_py(g);
         if constexpr (MSX_COL_PREFETCH) {
-            // unconditional: past the last unit it reloads the current tile
-            // (cache hits); a branch around the loads would make the
-            // compiler wait for them at the join
-            if (!gen) load(g2, m2, b2, nxt);
+            // unconditional (past the last unit an empty descriptor reads
+            // nothing): a branch around the loads would make the compiler
+            // wait for them at the join
+            if (!gen) load(g2, m2, b2, more, nxt);
         } else {
-            if (!gen) load(g, m, b, a);
+            if (!gen) load(g, m, b, true, a);
         }
         const int iy = col_y(g);
         const bool live = (m >> lw) & 1u;
