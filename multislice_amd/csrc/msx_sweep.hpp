@@ -42,6 +42,31 @@ namespace msx {
 
 constexpr int kMaxDet = 8;
 
+// Raw buffer stores (gfx9 descriptor word 3: 32-bit data, no swizzle).  A
+// store whose byte offset is at or beyond the descriptor's size is dropped
+// by the hardware; kBufDrop plus any in-line offset used here stays far
+// beyond every image size (at most 2048 x 2048 x 16 B = 64 MiB) and below
+// 4 GiB.
+constexpr int kBufWord3 = 0x00020000;
+constexpr unsigned kBufDrop = 0x80000000u;
+__device__ __forceinline__ void buf_store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    using u2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    using u4 = __attribute__((ext_vector_type(4))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
+}
+// Raw buffer loads: an offset at or beyond the descriptor's size reads zero
+// without touching memory.
+template <typename C> __device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+
 // ----------------------------------------------------------------------------
 // Row sweep: lines along the contiguous axis (length N = ny).
 // ----------------------------------------------------------------------------
@@ -120,7 +145,7 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
     const unsigned ld_mask = (p.prune & 1) ? inb : ~0u;
     const unsigned st_mask = (p.prune & 2) ? inb : ~0u;
 
-    auto load = [&](int b, C (&v)[E]) {
+    auto load = [&](int b, C (&v)[E], bool valid = true) {
         if constexpr (VREAL) {
             const Re* vr = p.vreal + ((long long)b * p.nx + ix) * N;
 #pragma unroll
@@ -139,13 +164,19 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
                 v[e] = mk<C>(c, s);
             }
         } else {
-            const C* in = p.src ? p.src + (long long)b * p.src_bstride + (long long)ix * N
-                                : p.psi + ((long long)b * p.nx + ix) * N;
-            // masked elements read element t (in band, fetched anyway) and are
-            // zeroed after the load: unconditional loads keep the prefetch
-            // free of branch-induced waits
+            // buffer descriptor of wave b's image (empty when !valid: the
+            // loads then read zero and touch no memory); masked elements read
+            // element t (in band, fetched anyway) and are zeroed after the
+            // load: unconditional loads keep the prefetch free of
+            // branch-induced waits
+            const C* img = p.src ? p.src + (long long)__builtin_amdgcn_readfirstlane(b) * p.src_bstride
+                                 : p.psi + (long long)__builtin_amdgcn_readfirstlane(b) * p.nx * N;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<C*>(img), 0, valid ? p.nx * N * (int)sizeof(C) : 0, kBufWord3);
+            const unsigned row = (unsigned)ix * (unsigned)N;
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = in[((ld_mask >> e) & 1u) ? t + e * T : t];
+            for (int e = 0; e < E; ++e)
+                v[e] = buf_load<C>(rs, (row + (unsigned)(((ld_mask >> e) & 1u) ? t + e * T : t)) * (unsigned)sizeof(C));
         }
     };
 
@@ -164,19 +195,43 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
         load(b0, a);
         mask_in(a);
     }
+    // jobs batched in one launch: each wave has its own transmission, whose
+    // address is read one wave ahead (after the prefetch, so that using it
+    // never waits for this wave's stores)
+    const C* tnext = (p.trans_tab && b0 < b1) ? p.trans_tab[b0] : nullptr;
+    // Drain the constant loads (and the first row) once: a load still
+    // pending at the loop entry makes the compiler wait for everything
+    // issued after it in every iteration.
+    __builtin_amdgcn_s_waitcnt(0);
     for (int b = b0; b < b1; ++b) {
+        if (p.trans_tab) {
+            // Loaded before the prefetch, through a buffer descriptor, and
+            // scaled only where it is used: the wait counters count in issue
+            // order, so consuming a load issued after the prefetch would wait
+            // for the whole prefetch.
+            const unsigned long long ta = (unsigned long long)tnext;
+            const C* tcur = reinterpret_cast<const C*>(
+                ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(ta >> 32)) << 32) |
+                (unsigned)__builtin_amdgcn_readfirstlane((unsigned)ta));
+            const __amdgpu_buffer_rsrc_t rt = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<C*>(tcur), 0, p.nx * N * (int)sizeof(C), kBufWord3);
+            const unsigned row = (unsigned)ix * (unsigned)N;
+#pragma unroll
+            for (int e = 0; e < E; ++e) tr[e] = buf_load<C>(rt, (row + (unsigned)(t + e * T)) * (unsigned)sizeof(C));
+        }
         if (MSX_ROW_PREFETCH) {
-            if (b + 1 < b1) load(b + 1, nxt);
+            // unconditional (past the last wave of the chunk the descriptor
+            // is empty) unless the input is the real potential
+            if constexpr (VREAL) {
+                if (b + 1 < b1) load(b + 1, nxt);
+            } else {
+                load(b + 1 < b1 ? b + 1 : b, nxt, b + 1 < b1);
+            }
         } else {
             load(b, a);
             mask_in(a);
         }
-        if (p.trans_tab) {
-            // jobs batched in one launch: each wave has its own transmission
-            const C* trow = p.trans_tab[b] + (long long)ix * N;
-#pragma unroll
-            for (int e = 0; e < E; ++e) tr[e] = fold ? cscale(trow[t + e * T], p.scale) : trow[t + e * T];
-        }
+        if (p.trans_tab) tnext = p.trans_tab[b + 1 < b1 ? b + 1 : b];
         if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
         if (p.scale != (Re)1 && !fold) {
 #pragma unroll
@@ -184,8 +239,9 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
         }
         if (p.trans || p.trans_tab) {
             // cfpix::operator*= (cfpix.cpp:296-318): (wr*tr - wi*ti, wr*ti + wi*tr)
+            const bool scale_here = p.trans_tab && fold;
 #pragma unroll
-            for (int e = 0; e < E; ++e) a[e] = cmul(a[e], tr[e]);
+            for (int e = 0; e < E; ++e) a[e] = cmul(a[e], scale_here ? cscale(tr[e], p.scale) : tr[e]);
         }
         if (p.inten && live) {
             Re* o = p.inten + ((long long)b * p.nx + ix) * N;
@@ -201,11 +257,18 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
             }
         }
         if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(a, t, lds, tw);
-        if (live && !p.nostore) {
-            C* row = p.psi + ((long long)b * p.nx + ix) * N;
+        {
+            // buffer stores into wave b's image; dead rows, pruned elements
+            // and a sweep that stores nothing aim past the bound (dropped):
+            // no branch, so no later wait has to count them conservatively
+            C* img = p.psi + (long long)__builtin_amdgcn_readfirstlane(b) * p.nx * N;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                img, 0, p.nostore ? 0 : p.nx * N * (int)sizeof(C), kBufWord3);
+            const unsigned row = (unsigned)ix * (unsigned)N;
 #pragma unroll
             for (int e = 0; e < E; ++e)
-                if ((st_mask >> e) & 1u) row[t + e * T] = a[e];
+                buf_store(a[e], rs, (live && ((st_mask >> e) & 1u)) ? (row + (unsigned)(t + e * T)) * (unsigned)sizeof(C)
+                                                                     : kBufDrop);
         }
         if (MSX_ROW_PREFETCH) {
 #pragma unroll
@@ -309,31 +372,6 @@ template <typename C, int N, int TW, bool DB> constexpr size_t col_list_offset()
     constexpr size_t tile = (size_t)N * (TW + 1) * sizeof(C);
     constexpr size_t x = DB ? 2 * tile : tile;
     return x < 2048 ? 2048 : x;
-}
-
-// Raw buffer stores (gfx9 descriptor word 3: 32-bit data, no swizzle).  A
-// store whose byte offset is at or beyond the descriptor's size is dropped
-// by the hardware; kBufDrop plus any in-line offset used here stays far
-// beyond every image size (at most 2048 x 2048 x 16 B = 64 MiB) and below
-// 4 GiB.
-constexpr int kBufWord3 = 0x00020000;
-constexpr unsigned kBufDrop = 0x80000000u;
-__device__ __forceinline__ void buf_store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
-    using u2 = __attribute__((ext_vector_type(2))) unsigned;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
-}
-__device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, unsigned off) {
-    using u4 = __attribute__((ext_vector_type(4))) unsigned;
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
-}
-// Raw buffer loads: an offset at or beyond the descriptor's size reads zero
-// without touching memory.
-template <typename C> __device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
-template <> __device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
-}
-template <> __device__ __forceinline__ double2 buf_load<double2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
 }
 
 // Schedules (p.wpb; the launcher turns 0 into -1):
