@@ -85,7 +85,7 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
  * is that schedule with one work-group per slot), the
  * column-sweep tile width (columns per work-group; 0 = built-in width, else
  * the built-in one or its alternative for this line length) and the STEM
- * batch (smallest within 1% of the best rate, at most max_batch; 0: 512).
+ * batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
  * ms_per_slice is the time of one wave through one slice at that batch.
  * msx_set_tuning installs a saved result (NULL clears it). */
 typedef struct msx_tuning {

@@ -569,7 +569,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
 // work-group of the row sweep and, for the column sweep, the tile width (the
 // built-in one and its alternative) with its schedule: chunked with 1, 2, 4
 // or 8 waves per work-group, or balanced with 1 or 2 work-groups per
-// resident slot (each sweep on its own).  The batch chosen is the smallest within 1% of
+// resident slot (each sweep on its own).  The batch chosen is the smallest within 3% of
 // the best probe-slice rate, with its own shapes.
 template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_batch, msx_tuning* res) {
     using Re = typename real_of<C>::type;
@@ -651,7 +651,7 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
         }
         if (rc != MSX_OK) break;
         for (int i = 0; i < kNB && batches[i] <= bmax; i++)
-            if (rate[i] >= 0.99 * top) {
+            if (rate[i] >= 0.97 * top) {
                 best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i], tile_of[i]};
                 break;
             }

@@ -73,7 +73,8 @@ inline int pick_wpb(int requested, long long units, long long target_blocks) {
     return (int)(w < 1 ? 1 : (w > 8 ? 8 : w));
 }
 
-template <typename C, int IN, int OUT, bool VREAL = false> hipError_t row_go(const RowArgs<C>& a0, hipStream_t s) {
+template <typename C, int IN, int OUT, bool VREAL = false, bool INTEN = false>
+hipError_t row_go(const RowArgs<C>& a0, hipStream_t s) {
     constexpr int LPW = row_lpw();
     RowArgs<C> a = a0;
     const int nxg = (a.nx + LPW - 1) / LPW;
@@ -81,7 +82,7 @@ template <typename C, int IN, int OUT, bool VREAL = false> hipError_t row_go(con
     const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
     const unsigned grid = (unsigned)(nxg * nchunk);
     const size_t smem = (size_t)LPW * (kN + kN / 16) * sizeof(C);
-    auto k = row_sweep<C, kN, IN, OUT, LPW, VREAL>;
+    auto k = row_sweep<C, kN, IN, OUT, LPW, VREAL, INTEN>;
     hipError_t e = set_smem(k, smem);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k, dim3(grid), dim3(kT * LPW), smem, s, a);
@@ -158,6 +159,10 @@ template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<
 }  // namespace
 
 template <typename C> hipError_t MSX_CAT(launch_row_, MSX_N)(int mode, const RowArgs<C>& a, hipStream_t s) {
+    if (a.inten) {  // |psi|^2 output: only after the inverse transform
+        if (mode == kRowInv && !a.vreal) return row_go<C, -1, 0, false, true>(a, s);
+        return hipErrorInvalidValue;
+    }
     if (a.vreal) {
         switch (mode) {
             case kRowFwd: return row_go<C, 0, 1, true>(a, s);

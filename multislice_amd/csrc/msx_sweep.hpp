@@ -37,6 +37,9 @@
 #ifndef MSX_COL_CONST
 #define MSX_COL_CONST 1
 #endif
+#ifndef MSX_ROW_DRAIN
+#define MSX_ROW_DRAIN 1
+#endif
 
 namespace msx {
 
@@ -57,9 +60,22 @@ __device__ __forceinline__ void buf_store(double2 v, __amdgpu_buffer_rsrc_t r, u
     using u4 = __attribute__((ext_vector_type(4))) unsigned;
     __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, v), r, (int)off, 0, 0);
 }
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    using u2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
+}
 // Raw buffer loads: an offset at or beyond the descriptor's size reads zero
 // without touching memory.
 template <typename C> __device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
 template <> __device__ __forceinline__ float2 buf_load<float2>(__amdgpu_buffer_rsrc_t r, unsigned off) {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
@@ -104,8 +120,9 @@ template <typename C> struct RowArgs {
 // next wave's row is prefetched while the current one is transformed.
 // VREAL: the input is a real potential turned into exp(i*vscale*V) (the
 // transmission build); a compile-time switch so the slice kernels carry none
-// of its sincos code.
-template <typename C, int N, int IN, int OUT, int LPW, bool VREAL>
+// of its sincos code.  INTEN: the |psi|^2 output (p.inten) is written; also
+// compile-time, so the slice kernels hold no registers for it.
+template <typename C, int N, int IN, int OUT, int LPW, bool VREAL, bool INTEN = false>
 __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
     using Re = typename real_of<C>::type;
     using PL = Plan<N>;
@@ -191,7 +208,10 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
         }
     };
     C a[E], nxt[E];
-    if (MSX_ROW_PREFETCH && b0 < b1) {
+    // the |psi|^2 variant (lens sweeps of many waves, one per work-group)
+    // does without the prefetch and its registers
+    constexpr bool PF = MSX_ROW_PREFETCH && !INTEN;
+    if (PF && b0 < b1) {
         load(b0, a);
         mask_in(a);
     }
@@ -202,7 +222,13 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
     // Drain the constant loads (and the first row) once: a load still
     // pending at the loop entry makes the compiler wait for everything
     // issued after it in every iteration.
-    __builtin_amdgcn_s_waitcnt(0);
+    if (MSX_ROW_DRAIN) __builtin_amdgcn_s_waitcnt(0);
+    // |psi|^2 output of wave b: [nb][nx][N] reals
+    auto inten_rsrc = [&](int b) {
+        return __builtin_amdgcn_make_buffer_rsrc(p.inten + (long long)__builtin_amdgcn_readfirstlane(b) * p.nx * N, 0,
+                                                 p.nx * N * (int)sizeof(Re), kBufWord3);
+    };
+    const unsigned row0 = (unsigned)ix * (unsigned)N;
     for (int b = b0; b < b1; ++b) {
         if (p.trans_tab) {
             // Loaded before the prefetch, through a buffer descriptor, and
@@ -219,7 +245,7 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
 #pragma unroll
             for (int e = 0; e < E; ++e) tr[e] = buf_load<C>(rt, (row + (unsigned)(t + e * T)) * (unsigned)sizeof(C));
         }
-        if (MSX_ROW_PREFETCH) {
+        if constexpr (PF) {
             // unconditional (past the last wave of the chunk the descriptor
             // is empty) unless the input is the real potential
             if constexpr (VREAL) {
@@ -230,6 +256,15 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
         } else {
             load(b, a);
             mask_in(a);
+        }
+        Re oi[INTEN ? E : 1];
+        if constexpr (INTEN) {
+            // the running sums this wave adds to, loaded with its row
+            if (p.iacc == 2) {
+#pragma unroll
+                for (int e = 0; e < E; ++e)
+                    oi[e] = buf_load<Re>(inten_rsrc(b), (row0 + (unsigned)(t + e * T)) * (unsigned)sizeof(Re));
+            }
         }
         if (p.trans_tab) tnext = p.trans_tab[b + 1 < b1 ? b + 1 : b];
         if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
@@ -243,17 +278,15 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = cmul(a[e], scale_here ? cscale(tr[e], p.scale) : tr[e]);
         }
-        if (p.inten && live) {
-            Re* o = p.inten + ((long long)b * p.nx + ix) * N;
-            if (p.iacc == 0) {
+        if constexpr (INTEN) {
+            // buffer stores (dead rows aim past the bound)
+            const __amdgpu_buffer_rsrc_t ri = inten_rsrc(b);
 #pragma unroll
-                for (int e = 0; e < E; ++e) o[t + e * T] = a[e].x * a[e].x + a[e].y * a[e].y;
-            } else {
-#pragma unroll
-                for (int e = 0; e < E; ++e) {
-                    const Re v = p.iw * (a[e].x * a[e].x + a[e].y * a[e].y);
-                    o[t + e * T] = p.iacc == 2 ? o[t + e * T] + v : v;
-                }
+            for (int e = 0; e < E; ++e) {
+                Re v = a[e].x * a[e].x + a[e].y * a[e].y;
+                if (p.iacc != 0) v = p.iw * v;
+                if (p.iacc == 2) v = oi[e] + v;
+                buf_store(v, ri, live ? (row0 + (unsigned)(t + e * T)) * (unsigned)sizeof(Re) : kBufDrop);
             }
         }
         if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(a, t, lds, tw);
@@ -270,7 +303,7 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
                 buf_store(a[e], rs, (live && ((st_mask >> e) & 1u)) ? (row + (unsigned)(t + e * T)) * (unsigned)sizeof(C)
                                                                      : kBufDrop);
         }
-        if (MSX_ROW_PREFETCH) {
+        if constexpr (PF) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = nxt[e];
             mask_in(a);
