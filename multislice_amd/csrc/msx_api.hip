@@ -146,7 +146,7 @@ struct msx_ctx_s {
         return MSX_OK;
     }
     std::map<int, DevBuf> tw32, tw64;
-    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf;
+    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab;
 
     // extra streams for job groups that run concurrently (msx_multislice_batch),
     // each with its own wave, pointer-table and propagator-table buffers
@@ -216,7 +216,7 @@ struct msx_ctx_s {
         for (auto& kv : tw32) kv.second.release();
         for (auto& kv : tw64) kv.second.release();
         drop_graphs();
-        for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf})
+        for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf, &radtab})
             b->release();
         for (Lane& l : lanes) {
             l.wave.release();
@@ -1194,9 +1194,17 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     std::vector<int> cnt(ncell + 1, 0);
     std::vector<long long> cell_of(natoms, -1);
     std::vector<AtomRec> rec(natoms);
+    int slot_of[104];
+    std::fill(slot_of, slot_of + 104, -1);
+    RadArgs ra;
+    std::memset(&ra, 0, sizeof ra);
     for (int64_t i = 0; i < natoms; i++) {
         const msx_atom& at = atoms[i];
         if (at.Z < 1 || at.Z > 103) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..103", (long long)i, at.Z);
+        if (slot_of[at.Z] < 0) {
+            slot_of[at.Z] = ra.nsp;
+            ra.zs[ra.nsp++] = at.Z;
+        }
         // ModelPotential.cu:63-66: kz*dz <= z < (kz+1)*dz
         long long kz = (long long)std::floor(at.z / dz);
         if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
@@ -1207,7 +1215,7 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
         int bx = std::min(nbx - 1, (int)(x / (a / nbx))), by = std::min(nby - 1, (int)(y / (b / nby)));
         const long long cell = (kz * nbx + bx) * nby + by;
         cell_of[i] = cell;
-        rec[i] = AtomRec{x, y, at.occ, at.Z};
+        rec[i] = AtomRec{x, y, at.occ, (int16_t)at.Z, (int16_t)slot_of[at.Z]};
         cnt[cell + 1]++;
     }
     for (long long k = 0; k < ncell; k++) cnt[k + 1] += cnt[k];
@@ -1241,6 +1249,21 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     sa.nby = nby;
     sa.table = table;
     sa.formula = formula;
+    sa.rtab = nullptr;
+    sa.rt_noct = 0;
+    if (prec == MSX_F32 && ra.nsp > 0) {
+        // FP32 output: V_Z(r^2) from radial tables (below FP32 resolution of
+        // the exact sum, which the FP64 output keeps)
+        // octaves up to the one holding radius^2 (at least one)
+        ra.noct = std::max(1, (int)std::ceil(std::log2(radius * radius)) - kRadOmin + 1);
+        ra.table = table;
+        ra.formula = formula;
+        CHK(ctx->radtab.ensure(sizeof(double) * (size_t)ra.nsp * ra.noct * (kRadPts + 3)));
+        ra.tab = static_cast<double*>(ctx->radtab.p);
+        HIPCHK(launch_radial_tables(ra, ctx->stream));
+        sa.rtab = ra.tab;
+        sa.rt_noct = ra.noct;
+    }
     CHK(ctx->timed(MSX_PROF_SPLAT, (double)vbytes, [&] { return launch_splat(sa, ctx->stream); }));
     if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(V, v_dev, vbytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));

@@ -78,10 +78,17 @@ template <typename C> void col_tiles(int nx, int* def, int* alt) {
 
 // potential splat (msx_potential.hip)
 struct AtomRec {
-    double x, y;  // Angstrom, wrapped into [0,a) x [0,b)
+    double x, y;   // Angstrom, wrapped into [0,a) x [0,b)
     float occ;
-    int Z;
+    int16_t Z;     // 1..103
+    int16_t slot;  // index of Z among the call's species (radial tables)
 };
+// Radial tables of the FP32 splat: per species, V_Z(s) at s = r^2 =
+// 2^(o + omin) * (1 + (j - 1) / kRadPts), j = 0 .. kRadPts + 2, for octaves
+// o = 0 .. noct - 1 (a guard point either side of each octave's kRadPts
+// intervals), evaluated by the same FP64 function as the exact splat.
+constexpr int kRadPts = 256;
+constexpr int kRadOmin = -24;  // s below 2^-24 A^2 (r < 2.4e-4 A) is evaluated exactly
 struct SplatArgs {
     const int* cell_start;   // [nslices*nbx*nby + 1] start of each (slice, bin) in atoms
     const AtomRec* atoms;    // sorted by (slice, bin x, bin y)
@@ -89,8 +96,16 @@ struct SplatArgs {
     float* v32;              // FP32 output [nslices][nx][ny], or nullptr
     double a, b, radius;
     int nx, ny, nslices, nbx, nby, table, formula;
+    const double* rtab;      // radial tables [nsp][rt_noct][kRadPts + 3] (FP32 output), or nullptr: exact
+    int rt_noct;
+};
+struct RadArgs {
+    double* tab;  // [nsp][noct][kRadPts + 3]
+    int zs[104];  // Z of each species slot
+    int nsp, noct, table, formula;
 };
 hipError_t launch_splat(const SplatArgs& a, hipStream_t s);
+hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s);
 hipError_t upload_fparams(hipStream_t s);
 
 // frozen-phonon displacement (msx_potential.hip); u_rms in device memory [104]

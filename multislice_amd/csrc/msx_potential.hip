@@ -83,6 +83,43 @@ __device__ __forceinline__ double d_vz(int Z, double r, int table, int formula) 
     return 300.73079394295 * suml + 150.36539697148 * sumg;
 }
 
+// V_Z(s = r^2) from the radial table of species `slot` (s in [2^kRadOmin,
+// 2^(kRadOmin + noct))): the octave from the exponent of s, the position in
+// it from the mantissa (uniform spacing within an octave), cubic Lagrange
+// interpolation through the four surrounding points.  Relative error below
+// 3e-9 (99.9th percentile), up to 3e-8 next to the branch switches of the
+// reference's Bessel polynomials: below FP32 resolution.
+__device__ __forceinline__ double radial_interp(const SplatArgs& p, int slot, double s) {
+    int e;
+    const double m = frexp(s, &e);  // s = m 2^e, m in [0.5, 1)
+    const int o = e - 1 - kRadOmin;
+    const double u = (2.0 * m - 1.0) * kRadPts;
+    const int k = (int)u;
+    const double t = u - k;
+    const double* T = p.rtab + ((long long)slot * p.rt_noct + o) * (kRadPts + 3) + k;
+    const double p0 = T[0], p1 = T[1], p2 = T[2], p3 = T[3];
+    const double tm1 = t - 1.0, tm2 = t - 2.0, tp1 = t + 1.0;
+    return (-t * tm1 * tm2 * (1.0 / 6.0)) * p0 + (tp1 * tm1 * tm2 * 0.5) * p1 - (tp1 * t * tm2 * 0.5) * p2 +
+           (tp1 * t * tm1 * (1.0 / 6.0)) * p3;
+}
+
+__global__ __launch_bounds__(256) void radial_table_kernel(RadArgs p) {
+    const long long per = (long long)p.noct * (kRadPts + 3);
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= per * p.nsp) return;
+    const int sp = (int)(i / per);
+    const int rem = (int)(i % per);
+    const int o = rem / (kRadPts + 3), j = rem % (kRadPts + 3);
+    const double s = ldexp(1.0 + (double)(j - 1) / kRadPts, o + kRadOmin);
+    p.tab[i] = d_vz(p.zs[sp], sqrt(s), p.table, p.formula);
+}
+
+hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
+    const long long n = (long long)a.nsp * a.noct * (kRadPts + 3);
+    hipLaunchKernelGGL(radial_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     const int iy = blockIdx.x * 16 + (threadIdx.x & 15);
     const int ix = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -90,6 +127,7 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     const double dx = p.a / p.nx, dy = p.b / p.ny;
     const double xp = ix * dx, yp = iy * dy;
     const double r2max = p.radius * p.radius;
+    const double smin = ldexp(1.0, kRadOmin);
     // bins reached by this tile
     const double bwx = p.a / p.nbx, bwy = p.b / p.nby;
     const double xlo = (double)(blockIdx.y * 16) * dx - p.radius, xhi = (double)(blockIdx.y * 16 + 15) * dx + p.radius;
@@ -112,7 +150,12 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
                 double dY = fabs(at.y - yp);
                 if (dY >= p.b / 2.0) dY -= p.b;
                 const double r2 = dX * dX + dY * dY;
-                if (r2 < r2max) v += (double)at.occ * d_vz(at.Z, sqrt(r2), p.table, p.formula);
+                if (r2 < r2max) {
+                    // FP32 output: the radial table (exact below its range)
+                    const double vz = (p.rtab && r2 >= smin) ? radial_interp(p, at.slot, r2)
+                                                             : d_vz(at.Z, sqrt(r2), p.table, p.formula);
+                    v += (double)at.occ * vz;
+                }
             }
         }
     }
