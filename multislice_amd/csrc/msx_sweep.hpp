@@ -209,8 +209,9 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
     };
     C a[E], nxt[E];
     // the |psi|^2 variant (lens sweeps of many waves, one per work-group)
-    // does without the prefetch and its registers
-    constexpr bool PF = MSX_ROW_PREFETCH && !INTEN;
+    // and the transmission build (whose loads compute exp(i vscale V)) do
+    // without the prefetch and its registers
+    constexpr bool PF = MSX_ROW_PREFETCH && !INTEN && !VREAL;
     if (PF && b0 < b1) {
         load(b0, a);
         mask_in(a);
