@@ -64,7 +64,7 @@ def build(jobs=None, verbose=True, defines=(), out=None, bdir=None):
         for f in futs:
             f.result()
     tmp = out + ".tmp"
-    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [u[1] for u in units])
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + [u[1] for u in units])
     os.replace(tmp, out)
     if verbose:
         print("built", out)

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -1185,6 +1186,21 @@ int partial_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_
 }
 
 // host binning of atoms for the splat kernel
+// fn(lo, hi) over [0, n) split across up to 16 host threads (one below 32768
+// items per thread); the ranges are disjoint, so fn may write per-item output
+template <typename F> void parallel_ranges(int64_t n, F&& fn) {
+    const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hw, 16), n / 32768));
+    if (T == 1) {
+        fn(int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve(T);
+    for (int t = 0; t < T; t++) th.emplace_back([&fn, n, t, T] { fn(n * t / T, n * (t + 1) / T); });
+    for (auto& x : th) x.join();
+}
+
 int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c, int nx, int ny,
                    int nslices, double radius, int table, int formula, void* V, int prec, int where) {
     const double dz = c / nslices;
@@ -1194,30 +1210,41 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     std::vector<int> cnt(ncell + 1, 0);
     std::vector<long long> cell_of(natoms, -1);
     std::vector<AtomRec> rec(natoms);
+    // species present (radial tables: slots in ascending Z)
     int slot_of[104];
     std::fill(slot_of, slot_of + 104, -1);
     RadArgs ra;
     std::memset(&ra, 0, sizeof ra);
+    bool present[104] = {};
     for (int64_t i = 0; i < natoms; i++) {
-        const msx_atom& at = atoms[i];
-        if (at.Z < 1 || at.Z > 103) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..103", (long long)i, at.Z);
-        if (slot_of[at.Z] < 0) {
-            slot_of[at.Z] = ra.nsp;
-            ra.zs[ra.nsp++] = at.Z;
-        }
-        // ModelPotential.cu:63-66: kz*dz <= z < (kz+1)*dz
-        long long kz = (long long)std::floor(at.z / dz);
-        if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
-        if (kz < 0 || kz >= nslices) continue;
-        double x = at.x - a * std::floor(at.x / a), y = at.y - b * std::floor(at.y / b);
-        if (x >= a) x -= a;
-        if (y >= b) y -= b;
-        int bx = std::min(nbx - 1, (int)(x / (a / nbx))), by = std::min(nby - 1, (int)(y / (b / nby)));
-        const long long cell = (kz * nbx + bx) * nby + by;
-        cell_of[i] = cell;
-        rec[i] = AtomRec{x, y, at.occ, (int16_t)at.Z, (int16_t)slot_of[at.Z]};
-        cnt[cell + 1]++;
+        const int Z = atoms[i].Z;
+        if (Z < 1 || Z > 103) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..103", (long long)i, Z);
+        present[Z] = true;
     }
+    for (int Z = 1; Z <= 103; Z++)
+        if (present[Z]) {
+            slot_of[Z] = ra.nsp;
+            ra.zs[ra.nsp++] = Z;
+        }
+    // slice and lateral bin of every atom, in parallel over atom ranges
+    parallel_ranges(natoms, [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) {
+            const msx_atom& at = atoms[i];
+            // ModelPotential.cu:63-66: kz*dz <= z < (kz+1)*dz
+            long long kz = (long long)std::floor(at.z / dz);
+            if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
+            if (kz < 0 || kz >= nslices) continue;
+            double x = at.x - a * std::floor(at.x / a), y = at.y - b * std::floor(at.y / b);
+            if (x >= a) x -= a;
+            if (y >= b) y -= b;
+            int bx = std::min(nbx - 1, (int)(x / (a / nbx))), by = std::min(nby - 1, (int)(y / (b / nby)));
+            cell_of[i] = (kz * nbx + bx) * nby + by;
+            rec[i] = AtomRec{x, y, at.occ, (int16_t)at.Z, (int16_t)slot_of[at.Z]};
+        }
+    });
+    // counting sort by cell, atoms of a cell in input order
+    for (int64_t i = 0; i < natoms; i++)
+        if (cell_of[i] >= 0) cnt[cell_of[i] + 1]++;
     for (long long k = 0; k < ncell; k++) cnt[k + 1] += cnt[k];
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
     std::vector<AtomRec> sorted(std::max<long long>(1, cnt[ncell]));
