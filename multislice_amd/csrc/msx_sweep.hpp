@@ -247,13 +247,9 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
             for (int e = 0; e < E; ++e) tr[e] = buf_load<C>(rt, (row + (unsigned)(t + e * T)) * (unsigned)sizeof(C));
         }
         if constexpr (PF) {
-            // unconditional (past the last wave of the chunk the descriptor
-            // is empty) unless the input is the real potential
-            if constexpr (VREAL) {
-                if (b + 1 < b1) load(b + 1, nxt);
-            } else {
-                load(b + 1 < b1 ? b + 1 : b, nxt, b + 1 < b1);
-            }
+            // unconditional: past the last wave of the chunk the descriptor
+            // is empty
+            load(b + 1 < b1 ? b + 1 : b, nxt, b + 1 < b1);
         } else {
             load(b, a);
             mask_in(a);
