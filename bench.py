@@ -107,6 +107,29 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes_read": n, "copy_probe_bytes_written": n, "launches": 5}))
 
 
+def tile_probe():
+    """Bandwidth ceilings of the access patterns (msx_copy_probe): a contiguous
+    copy and copies with the column sweep's pattern (8- and 16-column tiles)
+    of 128 images of 512 x 512 complex64 (256 MiB read, 256 MiB written)."""
+    import torch
+    import multislice_amd as msx
+    ctx = msx.Context(0)
+    nimg, n = 128, 512
+    src = torch.ones((nimg, n, n), dtype=torch.complex64, device="cuda")
+    dst = torch.empty_like(src)
+    bytes_ = 2 * src.numel() * 8
+    out = {}
+    for mode in (0, 8, 16):
+        ctx.copy_probe(mode, src, dst, nimg, n, n)
+        ms = sorted(ctx.copy_probe(mode, src, dst, nimg, n, n) for _ in range(10))
+        med = ms[len(ms) // 2]
+        out[{0: "contiguous", 8: "tile8", 16: "tile16"}[mode]] = {"ms_median": round(med, 4),
+                                                                  "GBps": round(bytes_ / med / 1e6, 1)}
+    assert bool((dst == src).all())
+    ctx.close()
+    print(json.dumps({"copy_bytes_per_launch": bytes_, "images": nimg, "grid": n, **out}))
+
+
 def traffic_from_profile(batch, tuning):
     """HBM bytes per launch of the column sweep on this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
@@ -180,9 +203,14 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="built-in launch shapes instead of the autotuner's")
     ap.add_argument("--shapes", default="",
                     help="ROW_WPB,COL_WPB,BATCH[,COL_TILE]: fixed launch shapes instead of the autotuner's")
+    ap.add_argument("--tile-probe", action="store_true",
+                    help="bandwidth of a contiguous copy and of the column sweep's access pattern, then exit")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 512 MiB device to device 5 times (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.tile_probe:
+        tile_probe()
+        return
     if a.copy_probe:
         copy_probe()
         return

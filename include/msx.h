@@ -104,6 +104,15 @@ MSX_API int msx_graph_stats(msx_ctx ctx, int64_t* captures, int64_t* replays);
  * HBM bytes are added to per-kind totals (enabling resets them).  This
  * serialises launches: use it for measurement passes, not production runs. */
 enum { MSX_PROF_ROW = 0, MSX_PROF_COL = 1, MSX_PROF_SPLAT = 2, MSX_PROF_DET = 3, MSX_PROF_KINDS = 4 };
+/* Bandwidth probe (no counterpart in the reference; measurement only): one
+ * launch copying device buffers src -> dst of nimg nx x ny complex64 images,
+ * timed with HIP events on the context stream (*ms).  mode 0: a contiguous
+ * copy; mode 8 or 16 (nx = 512, ny a multiple of it): the column sweep's
+ * access pattern (that many adjacent columns per work-group, 16 strided rows
+ * per thread, XCD order) without the transforms -- the ceiling of that
+ * pattern for the roofline of the column sweep. */
+MSX_API int msx_copy_probe(msx_ctx ctx, int mode, const void* src, void* dst, int nimg, int nx, int ny,
+                           double* ms);
 MSX_API int msx_profile(msx_ctx ctx, int enable);
 MSX_API int msx_profile_read(msx_ctx ctx, int kind, int64_t* launches, double* total_ms, double* total_bytes);
 

@@ -1618,6 +1618,32 @@ MSX_API int msx_profile_read(msx_ctx ctx, int kind, int64_t* launches, double* t
     return MSX_OK;
 }
 
+MSX_API int msx_copy_probe(msx_ctx ctx, int mode, const void* src, void* dst, int nimg, int nx, int ny,
+                           double* ms) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    if (!src || !dst || !ms || nimg < 1 || nx < 1 || ny < 1) return fail(MSX_EINVAL, "copy probe: bad arguments");
+    if (mode != 0 && !(nx == 512 && (mode == 8 || mode == 16) && ny % mode == 0))
+        return fail(MSX_EINVAL, "copy probe: mode 0, or 8 / 16 with nx = 512 and ny a multiple of the mode");
+    const long long n = (long long)nimg * nx * ny;
+    if (mode == 0 && (n % 2 || (uintptr_t)src % 16 || (uintptr_t)dst % 16))
+        return fail(MSX_EINVAL, "copy probe: mode 0 needs an even number of values and 16-byte aligned buffers");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->ev0) {
+        HIPCHK(hipEventCreate(&ctx->ev0));
+        HIPCHK(hipEventCreate(&ctx->ev1));
+    }
+    const float2* s = static_cast<const float2*>(src);
+    float2* d = static_cast<float2*>(dst);
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    HIPCHK(mode == 0 ? launch_stream_copy(s, d, n, ctx->stream) : launch_tile_copy(s, d, nimg, ny, mode, ctx->stream));
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    HIPCHK(hipEventSynchronize(ctx->ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t;
+    return MSX_OK;
+}
+
 MSX_API int msx_autotune(msx_ctx ctx, int nx, int ny, int prec, int max_batch, msx_tuning* best) {
     if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
     CHK(check_prec_where(prec, MSX_DEVICE));

@@ -167,6 +167,12 @@ struct RsArgs {
 hipError_t launch_rs_dt(const RsArgs& a, hipStream_t s);
 
 // detector finish: out[d][p] += w * sum_g partial[d][b][g] (p = probe0 + b)
+// bandwidth probes (msx_copy_probe): contiguous copy of n complex64 values,
+// and a copy of nimg 512 x ny images with the column sweep's access pattern
+// (tw adjacent columns per work-group, 32 threads x 16 rows per column, XCD
+// order)
+hipError_t launch_stream_copy(const float2* src, float2* dst, long long n, hipStream_t s);
+hipError_t launch_tile_copy(const float2* src, float2* dst, int nimg, int ny, int tw, hipStream_t s);
 hipError_t launch_det_finish(const double* partial, int ndet, int nb, int groups, double w, double* out,
                              long long out_stride_d, long long probe0, hipStream_t s);
 

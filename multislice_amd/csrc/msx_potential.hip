@@ -383,6 +383,40 @@ hipError_t launch_rs_dt(const RsArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                          long long n4) {
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) dst[i] = src[i];
+}
+hipError_t launch_stream_copy(const float2* src, float2* dst, long long n, hipStream_t s) {
+    const long long n4 = n / 2;  // two complex64 values per float4 (n even, checked by the caller)
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, s, reinterpret_cast<const float4*>(src),
+                       reinterpret_cast<float4*>(dst), n4);
+    return hipGetLastError();
+}
+// the column sweep's loads and stores without the transforms: each thread
+// loads its 16 rows (stride 32) of one column, then stores them
+template <int TW>
+__global__ __launch_bounds__(32 * TW) void tile_copy_kernel(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                            int ny, int groups) {
+    const int bid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int g = bid % groups, b = bid / groups;
+    const int lw = threadIdx.x % TW, t = threadIdx.x / TW;
+    const long long base = (long long)b * 512 * ny + (long long)g * TW + lw;
+    float2 v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = src[base + (long long)(t + 32 * e) * ny];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) dst[base + (long long)(t + 32 * e) * ny] = v[e];
+}
+hipError_t launch_tile_copy(const float2* src, float2* dst, int nimg, int ny, int tw, hipStream_t s) {
+    const int groups = ny / tw;  // ny % tw == 0, checked by the caller
+    const dim3 grid((unsigned)(groups * nimg));
+    if (tw == 8) hipLaunchKernelGGL(tile_copy_kernel<8>, grid, dim3(256), 0, s, src, dst, ny, groups);
+    else if (tw == 16) hipLaunchKernelGGL(tile_copy_kernel<16>, grid, dim3(512), 0, s, src, dst, ny, groups);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 __global__ void det_finish_kernel(const double* partial, int ndet, int nb, int groups, double w, double* out,
                                   long long out_stride_d, long long probe0) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;

@@ -84,7 +84,7 @@ PROF_ROW, PROF_COL, PROF_SPLAT, PROF_DET = 0, 1, 2, 3
 
 EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx_graph_stats", "msx_autotune",
            "msx_set_tuning", "msx_last_error", "msx_version",
-           "msx_profile", "msx_profile_read", "msx_displace", "msx_philox4x32_10",
+           "msx_profile", "msx_profile_read", "msx_copy_probe", "msx_displace", "msx_philox4x32_10",
            "msx_wavelength", "msx_sigma", "msx_vzatom", "msx_featom", "msx_fft2d", "msx_potential",
            "msx_transmission", "msx_multislice", "msx_stem", "msx_ctf_image",
            "msx_unet_create", "msx_unet_destroy", "msx_unet_param_count", "msx_unet_params",
@@ -164,6 +164,7 @@ def load():
         "msx_transmission_abs": (i, [vp, vp, vp, i, i, i, d, d, d, d, vp, i, i]),
         "msx_potential_rs": (i, [vp, vp, C.c_int64, vp, vp, d, d, d, d, d, i, i, i, i, vp, vp, i, i]),
         "msx_multislice_batch": (i, [vp, vp, i, i, i]),
+        "msx_copy_probe": (i, [vp, i, vp, vp, i, i, i, C.POINTER(d)]),
         "msx_potential_rec_curve": (i, [vp, vp, C.c_int64, vp, vp, i, i, d, vp, d, d, d, d, i, i, i, i, i, vp, vp, i,
                                         i]),
         "msx_potential_rs_curve": (i, [vp, vp, C.c_int64, vp, vp, i, i, d, vp, d, d, d, d, d, i, i, i, i, vp, vp, i,
@@ -278,6 +279,17 @@ class Context:
         n, ms, by = C.c_int64(), C.c_double(), C.c_double()
         _check(load().msx_profile_read(self.h, int(kind), C.byref(n), C.byref(ms), C.byref(by)))
         return n.value, ms.value, by.value
+
+    def copy_probe(self, mode, src, dst, nimg, nx, ny):
+        """msx_copy_probe: ms of one copy of nimg nx x ny complex64 images
+        (torch CUDA tensors src -> dst); mode 0 contiguous, 8 / 16 the column
+        sweep's access pattern with that tile width (nx = 512)."""
+        if src.numel() < nimg * nx * ny or dst.numel() < nimg * nx * ny:
+            raise ValueError("copy_probe: buffers smaller than nimg * nx * ny")
+        ms = C.c_double()
+        _check(load().msx_copy_probe(self.h, int(mode), C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()),
+                                     int(nimg), int(nx), int(ny), C.byref(ms)))
+        return ms.value
 
     # -------------------------------------------------------------- calls
     def fft2d(self, data, sign):
