@@ -107,7 +107,7 @@ enum { MSX_PROF_ROW = 0, MSX_PROF_COL = 1, MSX_PROF_SPLAT = 2, MSX_PROF_DET = 3,
 /* Bandwidth probe (no counterpart in the reference; measurement only): one
  * launch copying device buffers src -> dst of nimg nx x ny complex64 images,
  * timed with HIP events on the context stream (*ms).  mode 0: a contiguous
- * copy; mode 8 or 16 (nx = 512, ny a multiple of it): the column sweep's
+ * copy (nimg*nx*ny a multiple of 4096, 16-byte aligned buffers); mode 8 or 16 (nx = 512, ny a multiple of it): the column sweep's
  * access pattern (that many adjacent columns per work-group, 16 strided rows
  * per thread, XCD order) without the transforms -- the ceiling of that
  * pattern for the roofline of the column sweep. */

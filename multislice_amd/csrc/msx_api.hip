@@ -1625,8 +1625,8 @@ MSX_API int msx_copy_probe(msx_ctx ctx, int mode, const void* src, void* dst, in
     if (mode != 0 && !(nx == 512 && (mode == 8 || mode == 16) && ny % mode == 0))
         return fail(MSX_EINVAL, "copy probe: mode 0, or 8 / 16 with nx = 512 and ny a multiple of the mode");
     const long long n = (long long)nimg * nx * ny;
-    if (mode == 0 && (n % 2 || (uintptr_t)src % 16 || (uintptr_t)dst % 16))
-        return fail(MSX_EINVAL, "copy probe: mode 0 needs an even number of values and 16-byte aligned buffers");
+    if (mode == 0 && (n % 4096 || (uintptr_t)src % 16 || (uintptr_t)dst % 16))
+        return fail(MSX_EINVAL, "copy probe: mode 0 needs a multiple of 4096 values and 16-byte aligned buffers");
     HIPCHK(hipSetDevice(ctx->device));
     if (!ctx->ev0) {
         HIPCHK(hipEventCreate(&ctx->ev0));

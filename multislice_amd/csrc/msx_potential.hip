@@ -383,14 +383,22 @@ hipError_t launch_rs_dt(const RsArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst,
-                                                          long long n4) {
-    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) dst[i] = src[i];
+// each work-group copies 2048 consecutive float4; each thread loads its 8
+// (coalesced across the work-group at each step) before storing them, so 8
+// loads per thread are in flight.  No bounds test (a branch around the loads
+// would serialise them): the caller passes a multiple of 2048 float4.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float4* __restrict__ src, float4* __restrict__ dst) {
+    const long long i0 = (long long)blockIdx.x * 2048 + threadIdx.x;
+    float4 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = src[i0 + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) dst[i0 + 256 * k] = v[k];
 }
 hipError_t launch_stream_copy(const float2* src, float2* dst, long long n, hipStream_t s) {
-    const long long n4 = n / 2;  // two complex64 values per float4 (n even, checked by the caller)
-    hipLaunchKernelGGL(stream_copy_kernel, dim3(4096), dim3(256), 0, s, reinterpret_cast<const float4*>(src),
-                       reinterpret_cast<float4*>(dst), n4);
+    const long long n4 = n / 2;  // two complex64 values per float4; a multiple of 2048 (checked by the caller)
+    hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)(n4 / 2048)), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst));
     return hipGetLastError();
 }
 // the column sweep's loads and stores without the transforms: each thread
