@@ -1280,12 +1280,12 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     sa.rt_noct = 0;
     if (prec == MSX_F32 && ra.nsp > 0) {
         // FP32 output: V_Z(r^2) from radial tables (below FP32 resolution of
-        // the exact sum, which the FP64 output keeps)
-        // octaves up to the one holding radius^2 (at least one)
+        // the exact sum, which the FP64 output keeps); octaves from 2^kRadOmin
+        // up to the one holding radius^2
         ra.noct = std::max(1, (int)std::ceil(std::log2(radius * radius)) - kRadOmin + 1);
         ra.table = table;
         ra.formula = formula;
-        CHK(ctx->radtab.ensure(sizeof(double) * (size_t)ra.nsp * ra.noct * (kRadPts + 3)));
+        CHK(ctx->radtab.ensure(sizeof(double) * (size_t)ra.nsp * ((size_t)ra.noct * (kRadPts + 3) + 1)));
         ra.tab = static_cast<double*>(ctx->radtab.p);
         HIPCHK(launch_radial_tables(ra, ctx->stream));
         sa.rtab = ra.tab;

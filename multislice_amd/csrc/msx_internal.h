@@ -86,9 +86,12 @@ struct AtomRec {
 // Radial tables of the FP32 splat: per species, V_Z(s) at s = r^2 =
 // 2^(o + omin) * (1 + (j - 1) / kRadPts), j = 0 .. kRadPts + 2, for octaves
 // o = 0 .. noct - 1 (a guard point either side of each octave's kRadPts
-// intervals), evaluated by the same FP64 function as the exact splat.
+// intervals), evaluated by the same FP64 formula as the exact splat without
+// its clamp r >= 1e-10 (so the tabulated function stays smooth), followed by
+// one entry holding the clamped value V_Z(1e-10) for s < 1e-20.
 constexpr int kRadPts = 256;
-constexpr int kRadOmin = -24;  // s below 2^-24 A^2 (r < 2.4e-4 A) is evaluated exactly
+// lowest octave: s = r^2 from 2^-67 A^2, below 1e-20 = 2^-66.4
+constexpr int kRadOmin = -67;
 struct SplatArgs {
     const int* cell_start;   // [nslices*nbx*nby + 1] start of each (slice, bin) in atoms
     const AtomRec* atoms;    // sorted by (slice, bin x, bin y)
@@ -96,11 +99,11 @@ struct SplatArgs {
     float* v32;              // FP32 output [nslices][nx][ny], or nullptr
     double a, b, radius;
     int nx, ny, nslices, nbx, nby, table, formula;
-    const double* rtab;      // radial tables [nsp][rt_noct][kRadPts + 3] (FP32 output), or nullptr: exact
+    const double* rtab;      // radial tables [nsp][rt_noct * (kRadPts + 3) + 1] (FP32 output), or nullptr: exact
     int rt_noct;
 };
 struct RadArgs {
-    double* tab;  // [nsp][noct][kRadPts + 3]
+    double* tab;  // [nsp][noct * (kRadPts + 3) + 1]
     int zs[104];  // Z of each species slot
     int nsp, noct, table, formula;
 };

@@ -65,9 +65,12 @@ __device__ __forceinline__ double d_bessk0(double x) {
 
 // vzatom (slicelib.cpp:2666-2699) / calculateProjectedPotential
 // (ModelPotential.cu:232-254)
+// CLAMP: r < 1e-10 is evaluated at 1e-10, as the reference does; the radial
+// tables are filled without it (smooth below the clamp; r > 0 there)
+template <bool CLAMP = true>
 __device__ __forceinline__ double d_vz(int Z, double r, int table, int formula) {
     const double* p = c_fparams[table][Z - 1];
-    if (r < 1.0e-10) r = 1.0e-10;
+    if (CLAMP && r < 1.0e-10) r = 1.0e-10;
     double suml = 0.0, sumg = 0.0;
     if (formula == 0) {
         const double pi = 3.141592654;
@@ -88,38 +91,53 @@ __device__ __forceinline__ double d_vz(int Z, double r, int table, int formula) 
 // it from the mantissa (uniform spacing within an octave), cubic Lagrange
 // interpolation through the four surrounding points.  Relative error below
 // 3e-9 (99.9th percentile), up to 3e-8 next to the branch switches of the
-// reference's Bessel polynomials: below FP32 resolution.
+// reference's Bessel polynomials: below FP32 resolution.  Below s = 1e-20
+// (r < 1e-10, where the formula clamps r) the table's last entry holds the
+// clamped value.
 __device__ __forceinline__ double radial_interp(const SplatArgs& p, int slot, double s) {
+    const long long per = (long long)p.rt_noct * (kRadPts + 3);
+    const double* base = p.rtab + (long long)slot * (per + 1);
+    if (s < 1.0e-20) return base[per];
     int e;
     const double m = frexp(s, &e);  // s = m 2^e, m in [0.5, 1)
     const int o = e - 1 - kRadOmin;
     const double u = (2.0 * m - 1.0) * kRadPts;
     const int k = (int)u;
     const double t = u - k;
-    const double* T = p.rtab + ((long long)slot * p.rt_noct + o) * (kRadPts + 3) + k;
+    const double* T = base + (long long)o * (kRadPts + 3) + k;
     const double p0 = T[0], p1 = T[1], p2 = T[2], p3 = T[3];
     const double tm1 = t - 1.0, tm2 = t - 2.0, tp1 = t + 1.0;
     return (-t * tm1 * tm2 * (1.0 / 6.0)) * p0 + (tp1 * tm1 * tm2 * 0.5) * p1 - (tp1 * t * tm2 * 0.5) * p2 +
            (tp1 * t * tm1 * (1.0 / 6.0)) * p3;
 }
 
+// per species: noct * (kRadPts + 3) grid values of the unclamped formula,
+// then the value at the clamp r = 1e-10
 __global__ __launch_bounds__(256) void radial_table_kernel(RadArgs p) {
     const long long per = (long long)p.noct * (kRadPts + 3);
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= per * p.nsp) return;
-    const int sp = (int)(i / per);
-    const int rem = (int)(i % per);
-    const int o = rem / (kRadPts + 3), j = rem % (kRadPts + 3);
+    if (i >= (per + 1) * p.nsp) return;
+    const int sp = (int)(i / (per + 1));
+    const long long rem = i % (per + 1);
+    if (rem == per) {
+        p.tab[i] = d_vz(p.zs[sp], 1.0e-10, p.table, p.formula);
+        return;
+    }
+    const int o = (int)(rem / (kRadPts + 3)), j = (int)(rem % (kRadPts + 3));
     const double s = ldexp(1.0 + (double)(j - 1) / kRadPts, o + kRadOmin);
-    p.tab[i] = d_vz(p.zs[sp], sqrt(s), p.table, p.formula);
+    p.tab[i] = d_vz<false>(p.zs[sp], sqrt(s), p.table, p.formula);
 }
 
 hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
-    const long long n = (long long)a.nsp * a.noct * (kRadPts + 3);
+    const long long n = (long long)a.nsp * (a.noct * (kRadPts + 3) + 1);
     hipLaunchKernelGGL(radial_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
+// TAB: V_Z from the radial tables (FP32 output); otherwise the exact FP64
+// formula per pair (separate instantiations: the table kernel carries none of
+// the Bessel code)
+template <bool TAB>
 __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     const int iy = blockIdx.x * 16 + (threadIdx.x & 15);
     const int ix = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -127,7 +145,6 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     const double dx = p.a / p.nx, dy = p.b / p.ny;
     const double xp = ix * dx, yp = iy * dy;
     const double r2max = p.radius * p.radius;
-    const double smin = ldexp(1.0, kRadOmin);
     // bins reached by this tile
     const double bwx = p.a / p.nbx, bwy = p.b / p.nby;
     const double xlo = (double)(blockIdx.y * 16) * dx - p.radius, xhi = (double)(blockIdx.y * 16 + 15) * dx + p.radius;
@@ -151,9 +168,9 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
                 if (dY >= p.b / 2.0) dY -= p.b;
                 const double r2 = dX * dX + dY * dY;
                 if (r2 < r2max) {
-                    // FP32 output: the radial table (exact below its range)
-                    const double vz = (p.rtab && r2 >= smin) ? radial_interp(p, at.slot, r2)
-                                                             : d_vz(at.Z, sqrt(r2), p.table, p.formula);
+                    double vz;
+                    if constexpr (TAB) vz = radial_interp(p, at.slot, r2);
+                    else vz = d_vz(at.Z, sqrt(r2), p.table, p.formula);
                     v += (double)at.occ * vz;
                 }
             }
@@ -168,7 +185,8 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
 
 hipError_t launch_splat(const SplatArgs& a, hipStream_t s) {
     dim3 grid((a.ny + 15) / 16, (a.nx + 15) / 16, a.nslices);
-    hipLaunchKernelGGL(splat_kernel, grid, dim3(256), 0, s, a);
+    if (a.rtab) hipLaunchKernelGGL(splat_kernel<true>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(splat_kernel<false>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
