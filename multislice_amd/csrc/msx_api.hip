@@ -148,6 +148,22 @@ struct msx_ctx_s {
     }
     std::map<int, DevBuf> tw32, tw64;
     DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab;
+    // host side of the potential's atom binning, kept across calls (no page
+    // faults on reuse); the sorted records go to pinned memory for the upload
+    std::vector<long long> h_cell;
+    std::vector<AtomRec> h_rec;
+    std::vector<int> h_cnt, h_fill;
+    AtomRec* h_pin = nullptr;
+    size_t h_pin_cap = 0;
+    int ensure_pinned(size_t n) {
+        if (n <= h_pin_cap) return MSX_OK;
+        if (h_pin) HIPCHK(hipHostFree(h_pin));
+        h_pin = nullptr;
+        h_pin_cap = 0;
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_pin), sizeof(AtomRec) * n, hipHostMallocDefault));
+        h_pin_cap = n;
+        return MSX_OK;
+    }
 
     // extra streams for job groups that run concurrently (msx_multislice_batch),
     // each with its own wave, pointer-table and propagator-table buffers
@@ -227,6 +243,7 @@ struct msx_ctx_s {
             if (l.stream) hipStreamDestroy(l.stream);
         }
         if (lanes_go) hipEventDestroy(lanes_go);
+        if (h_pin) hipHostFree(h_pin);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
         if (own) hipStreamDestroy(own);
@@ -1207,9 +1224,12 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     const int nbx = std::max(1, std::min(1024, (int)std::floor(a / radius)));
     const int nby = std::max(1, std::min(1024, (int)std::floor(b / radius)));
     const long long ncell = (long long)nslices * nbx * nby;
-    std::vector<int> cnt(ncell + 1, 0);
-    std::vector<long long> cell_of(natoms, -1);
-    std::vector<AtomRec> rec(natoms);
+    std::vector<int>& cnt = ctx->h_cnt;
+    std::vector<long long>& cell_of = ctx->h_cell;
+    std::vector<AtomRec>& rec = ctx->h_rec;
+    cnt.assign(ncell + 1, 0);
+    cell_of.resize(natoms);
+    rec.resize(natoms);
     // species present (radial tables: slots in ascending Z)
     int slot_of[104];
     std::fill(slot_of, slot_of + 104, -1);
@@ -1230,6 +1250,7 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     parallel_ranges(natoms, [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; i++) {
             const msx_atom& at = atoms[i];
+            cell_of[i] = -1;
             // ModelPotential.cu:63-66: kz*dz <= z < (kz+1)*dz
             long long kz = (long long)std::floor(at.z / dz);
             if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
@@ -1246,14 +1267,21 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     for (int64_t i = 0; i < natoms; i++)
         if (cell_of[i] >= 0) cnt[cell_of[i] + 1]++;
     for (long long k = 0; k < ncell; k++) cnt[k + 1] += cnt[k];
-    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    std::vector<AtomRec> sorted(std::max<long long>(1, cnt[ncell]));
+    std::vector<int>& fill = ctx->h_fill;
+    fill.assign(cnt.begin(), cnt.end() - 1);
+    const size_t nsorted = (size_t)std::max<long long>(1, cnt[ncell]);
+    // an earlier call that failed after its upload was queued may still be
+    // reading the pinned records
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    CHK(ctx->ensure_pinned(nsorted));
+    AtomRec* sorted = ctx->h_pin;
     for (int64_t i = 0; i < natoms; i++)
         if (cell_of[i] >= 0) sorted[fill[cell_of[i]]++] = rec[i];
-    CHK(ctx->atoms.ensure(sizeof(AtomRec) * sorted.size()));
+    CHK(ctx->atoms.ensure(sizeof(AtomRec) * nsorted));
     CHK(ctx->cells.ensure(sizeof(int) * cnt.size()));
-    HIPCHK(hipMemcpyAsync(ctx->atoms.p, sorted.data(), sizeof(AtomRec) * sorted.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
+    // the pinned records are reused by the next call only after this call's
+    // final stream synchronisation
+    HIPCHK(hipMemcpyAsync(ctx->atoms.p, sorted, sizeof(AtomRec) * nsorted, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->cells.p, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, ctx->stream));
     const size_t vbytes = (prec == MSX_F64 ? sizeof(double) : sizeof(float)) * (size_t)nslices * nx * ny;
     void* v_dev = V;
