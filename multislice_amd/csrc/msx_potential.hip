@@ -139,8 +139,12 @@ hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
 // the Bessel code)
 template <bool TAB>
 __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
-    const int iy = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int ix = blockIdx.y * 16 + (threadIdx.x >> 4);
+    // each wavefront covers an 8 x 8 quarter of the 16 x 16 tile: a compact
+    // footprint leaves fewer lanes idle outside the radius of an atom the
+    // wavefront evaluates
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int iy = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
+    const int ix = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
     const int kz = blockIdx.z;
     const double dx = p.a / p.nx, dy = p.b / p.ny;
     const double xp = ix * dx, yp = iy * dy;
