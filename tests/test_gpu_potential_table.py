@@ -49,3 +49,34 @@ def test_fp32_table_splat_within_fp32_resolution(radius, table, formula):
     Vo = O.potential(at["Z"], xyz, cell, 64, 64, 2, radius, table=table, formula=formula, occ=at["occ"])
     assert rel_l2(V64, Vo) < 1e-13
     ctx.close()
+
+
+def test_potential_edge_cases():
+    """No atoms, or only atoms outside the slab: a zero potential in both
+    precisions (no species, so no radial tables); an element outside 1..103
+    is refused with the atom's index.  Calls in a row reuse the context's
+    binning buffers (a large call, then a small one, then the large one
+    again gives the same potential)."""
+    ctx = msx.Context(0)
+    cell = (10.0, 10.0, 4.0)
+    for prec in (msx.F32, msx.F64):
+        V = ctx.potential(np.zeros(0, dtype=msx.ATOM_DTYPE), cell, 32, 32, 2, 3.0, prec=prec)
+        assert V.shape == (2, 32, 32) and not V.any()
+        out = np.zeros(3, dtype=msx.ATOM_DTYPE)
+        out["Z"], out["occ"] = 14, 1.0
+        out["z"] = [-0.5, 4.0, 7.0]  # below the slab, at its top face, above it
+        V = ctx.potential(out, cell, 32, 32, 2, 3.0, prec=prec)
+        assert not V.any()
+    bad = np.zeros(2, dtype=msx.ATOM_DTYPE)
+    bad["Z"] = [14, 104]
+    with pytest.raises(msx.MsxError) as e:
+        ctx.potential(bad, cell, 32, 32, 2, 3.0)
+    assert "atom 1" in str(e.value)
+    rng = np.random.default_rng(3)
+    big = specimen(rng, 400, cell, [8, 22, 38])
+    small = specimen(rng, 5, cell, [79])
+    V1 = ctx.potential(big, cell, 64, 64, 2, 3.0)
+    ctx.potential(small, cell, 64, 64, 2, 3.0)
+    V2 = ctx.potential(big, cell, 64, 64, 2, 3.0)
+    np.testing.assert_array_equal(V1, V2)
+    ctx.close()
