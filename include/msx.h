@@ -225,10 +225,22 @@ MSX_API int msx_fft2d(msx_ctx ctx, void* data, int nbatch, int nx, int ny, int s
  *   atoms with r < radius contribute occ*vz(r) (r clamped >= 1e-10), summed
  *   in FP64.
  * V: nslices images of nx*ny reals [kz][ix][iy] (float for MSX_F32, double for
- * MSX_F64), host or device per `where`. */
+ * MSX_F64), host or device per `where`.  An atom with Z outside 1..103 or a
+ * coordinate that is not finite is refused (MSX_EINVAL, naming the first such
+ * atom). */
 MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c,
                           int nx, int ny, int nslices, double radius, int table, int formula, void* V,
                           int prec, int where);
+/* msx_potential for atoms already in device memory (atoms: a device array,
+ * e.g. the output of msx_displace with MSX_DEVICE): the atoms are binned on
+ * the GPU (cell of every atom, stable radix sort by cell) instead of on the
+ * host, and the potential is identical to msx_potential's on the same atoms.
+ * No counterpart in the reference, whose atoms always start on the host
+ * (ModelPotential.cu:27-99); for frozen-phonon loops that displace on the
+ * GPU.  At most INT_MAX - 1 atoms and slice x bin cells. */
+MSX_API int msx_potential_device_atoms(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b,
+                                       double c, int nx, int ny, int nslices, double radius, int table, int formula,
+                                       void* V, int prec, int where);
 /* Reciprocal-space projected potential from structure factors: Muslib
  * Potential_rec_DT_DW (Muslib/Pot_rec_DT_DW.cpp:3-132) followed by the inverse
  * transform of the phase-grating build (MSProjectReuse.cs:425-431).  Per slice

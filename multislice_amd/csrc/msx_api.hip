@@ -1,6 +1,7 @@
 // msx_api.hip -- the C ABI of include/msx.h: context, device tables, and the
 // host-side orchestration of the sweep kernels for each reference entry point.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -147,7 +148,7 @@ struct msx_ctx_s {
         return MSX_OK;
     }
     std::map<int, DevBuf> tw32, tw64;
-    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab;
+    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab, binbuf;
     // host side of the potential's atom binning, kept across calls (no page
     // faults on reuse); the sorted records go to pinned memory for the upload
     std::vector<long long> h_cell;
@@ -233,7 +234,8 @@ struct msx_ctx_s {
         for (auto& kv : tw32) kv.second.release();
         for (auto& kv : tw64) kv.second.release();
         drop_graphs();
-        for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf, &radtab})
+        for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf, &radtab,
+                           &binbuf})
             b->release();
         for (Lane& l : lanes) {
             l.wave.release();
@@ -1218,12 +1220,74 @@ template <typename F> void parallel_ranges(int64_t n, F&& fn) {
     for (auto& x : th) x.join();
 }
 
+// lateral bins of the splat: cells of at least the cutoff radius, at most
+// 1024 per axis
+BinGeom splat_geom(double a, double b, double c, int nslices, double radius) {
+    BinGeom g;
+    g.a = a;
+    g.b = b;
+    g.dz = c / nslices;
+    g.nslices = nslices;
+    g.nbx = std::max(1, std::min(1024, (int)std::floor(a / radius)));
+    g.nby = std::max(1, std::min(1024, (int)std::floor(b / radius)));
+    return g;
+}
+
+int bad_atom(int64_t i, int Z) {
+    if (Z < 1 || Z > 103) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..103", (long long)i, Z);
+    return fail(MSX_EINVAL, "atom %lld: coordinate not finite", (long long)i);
+}
+
+// radial tables (FP32 output), splat of the binned atoms in ctx->atoms /
+// ctx->cells, copy to the host when asked
+int splat_binned(msx_ctx ctx, RadArgs& ra, const BinGeom& g, int nx, int ny, double radius, int table,
+                 int formula, void* V, int prec, int where) {
+    const size_t vbytes = (prec == MSX_F64 ? sizeof(double) : sizeof(float)) * (size_t)g.nslices * nx * ny;
+    void* v_dev = V;
+    if (where == MSX_HOST) {
+        CHK(ctx->vtmp.ensure(vbytes));
+        v_dev = ctx->vtmp.p;
+    }
+    SplatArgs sa;
+    sa.cell_start = static_cast<const int*>(ctx->cells.p);
+    sa.atoms = static_cast<const AtomRec*>(ctx->atoms.p);
+    sa.v64 = prec == MSX_F64 ? static_cast<double*>(v_dev) : nullptr;
+    sa.v32 = prec == MSX_F32 ? static_cast<float*>(v_dev) : nullptr;
+    sa.a = g.a;
+    sa.b = g.b;
+    sa.radius = radius;
+    sa.nx = nx;
+    sa.ny = ny;
+    sa.nslices = g.nslices;
+    sa.nbx = g.nbx;
+    sa.nby = g.nby;
+    sa.table = table;
+    sa.formula = formula;
+    sa.rtab = nullptr;
+    sa.rt_noct = 0;
+    if (prec == MSX_F32 && ra.nsp > 0) {
+        // FP32 output: V_Z(r^2) from radial tables (below FP32 resolution of
+        // the exact sum, which the FP64 output keeps); octaves from 2^kRadOmin
+        // up to the one holding radius^2
+        ra.noct = std::max(1, (int)std::ceil(std::log2(radius * radius)) - kRadOmin + 1);
+        ra.table = table;
+        ra.formula = formula;
+        CHK(ctx->radtab.ensure(sizeof(double) * (size_t)ra.nsp * ((size_t)ra.noct * (kRadPts + 3) + 1)));
+        ra.tab = static_cast<double*>(ctx->radtab.p);
+        HIPCHK(launch_radial_tables(ra, ctx->stream));
+        sa.rtab = ra.tab;
+        sa.rt_noct = ra.noct;
+    }
+    CHK(ctx->timed(MSX_PROF_SPLAT, (double)vbytes, [&] { return launch_splat(sa, ctx->stream); }));
+    if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(V, v_dev, vbytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
 int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c, int nx, int ny,
                    int nslices, double radius, int table, int formula, void* V, int prec, int where) {
-    const double dz = c / nslices;
-    const int nbx = std::max(1, std::min(1024, (int)std::floor(a / radius)));
-    const int nby = std::max(1, std::min(1024, (int)std::floor(b / radius)));
-    const long long ncell = (long long)nslices * nbx * nby;
+    const BinGeom g = splat_geom(a, b, c, nslices, radius);
+    const long long ncell = (long long)nslices * g.nbx * g.nby;
     std::vector<int>& cnt = ctx->h_cnt;
     std::vector<long long>& cell_of = ctx->h_cell;
     std::vector<AtomRec>& rec = ctx->h_rec;
@@ -1237,9 +1301,10 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     std::memset(&ra, 0, sizeof ra);
     bool present[104] = {};
     for (int64_t i = 0; i < natoms; i++) {
-        const int Z = atoms[i].Z;
-        if (Z < 1 || Z > 103) return fail(MSX_EINVAL, "atom %lld: Z = %d outside 1..103", (long long)i, Z);
-        present[Z] = true;
+        const msx_atom& at = atoms[i];
+        if (at.Z < 1 || at.Z > 103 || !std::isfinite(at.x) || !std::isfinite(at.y) || !std::isfinite(at.z))
+            return bad_atom(i, at.Z);
+        present[at.Z] = true;
     }
     for (int Z = 1; Z <= 103; Z++)
         if (present[Z]) {
@@ -1250,16 +1315,11 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     parallel_ranges(natoms, [&](int64_t lo, int64_t hi) {
         for (int64_t i = lo; i < hi; i++) {
             const msx_atom& at = atoms[i];
-            cell_of[i] = -1;
-            // ModelPotential.cu:63-66: kz*dz <= z < (kz+1)*dz
-            long long kz = (long long)std::floor(at.z / dz);
-            if (kz >= 0 && kz < nslices && !(kz * dz <= at.z && at.z < (kz + 1) * dz)) kz += (at.z < kz * dz) ? -1 : 1;
-            if (kz < 0 || kz >= nslices) continue;
-            double x = at.x - a * std::floor(at.x / a), y = at.y - b * std::floor(at.y / b);
-            if (x >= a) x -= a;
-            if (y >= b) y -= b;
-            int bx = std::min(nbx - 1, (int)(x / (a / nbx))), by = std::min(nby - 1, (int)(y / (b / nby)));
-            cell_of[i] = (kz * nbx + bx) * nby + by;
+            double x, y;
+            if (!bin_atom(g, at.x, at.y, at.z, &cell_of[i], &x, &y)) {
+                cell_of[i] = -1;
+                continue;
+            }
             rec[i] = AtomRec{x, y, at.occ, (int16_t)at.Z, (int16_t)slot_of[at.Z]};
         }
     });
@@ -1283,46 +1343,61 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     // final stream synchronisation
     HIPCHK(hipMemcpyAsync(ctx->atoms.p, sorted, sizeof(AtomRec) * nsorted, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->cells.p, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice, ctx->stream));
-    const size_t vbytes = (prec == MSX_F64 ? sizeof(double) : sizeof(float)) * (size_t)nslices * nx * ny;
-    void* v_dev = V;
-    if (where == MSX_HOST) {
-        CHK(ctx->vtmp.ensure(vbytes));
-        v_dev = ctx->vtmp.p;
-    }
-    SplatArgs sa;
-    sa.cell_start = static_cast<const int*>(ctx->cells.p);
-    sa.atoms = static_cast<const AtomRec*>(ctx->atoms.p);
-    sa.v64 = prec == MSX_F64 ? static_cast<double*>(v_dev) : nullptr;
-    sa.v32 = prec == MSX_F32 ? static_cast<float*>(v_dev) : nullptr;
-    sa.a = a;
-    sa.b = b;
-    sa.radius = radius;
-    sa.nx = nx;
-    sa.ny = ny;
-    sa.nslices = nslices;
-    sa.nbx = nbx;
-    sa.nby = nby;
-    sa.table = table;
-    sa.formula = formula;
-    sa.rtab = nullptr;
-    sa.rt_noct = 0;
-    if (prec == MSX_F32 && ra.nsp > 0) {
-        // FP32 output: V_Z(r^2) from radial tables (below FP32 resolution of
-        // the exact sum, which the FP64 output keeps); octaves from 2^kRadOmin
-        // up to the one holding radius^2
-        ra.noct = std::max(1, (int)std::ceil(std::log2(radius * radius)) - kRadOmin + 1);
-        ra.table = table;
-        ra.formula = formula;
-        CHK(ctx->radtab.ensure(sizeof(double) * (size_t)ra.nsp * ((size_t)ra.noct * (kRadPts + 3) + 1)));
-        ra.tab = static_cast<double*>(ctx->radtab.p);
-        HIPCHK(launch_radial_tables(ra, ctx->stream));
-        sa.rtab = ra.tab;
-        sa.rt_noct = ra.noct;
-    }
-    CHK(ctx->timed(MSX_PROF_SPLAT, (double)vbytes, [&] { return launch_splat(sa, ctx->stream); }));
-    if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(V, v_dev, vbytes, hipMemcpyDeviceToHost, ctx->stream));
+    return splat_binned(ctx, ra, g, nx, ny, radius, table, formula, V, prec, where);
+}
+
+// The same potential from atoms in device memory: binned on the GPU (stable
+// radix sort by cell), so the records and cell starts equal the host
+// binning's and the potential is identical.
+int potential_dev_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b, double c, int nx,
+                       int ny, int nslices, double radius, int table, int formula, void* V, int prec, int where) {
+    const BinGeom g = splat_geom(a, b, c, nslices, radius);
+    const long long ncell = (long long)nslices * g.nbx * g.nby;
+    if (natoms >= INT_MAX || ncell >= INT_MAX)
+        return fail(MSX_EINVAL, "device atoms: %lld atoms, %lld cells (at most %d each)", (long long)natoms, ncell,
+                    INT_MAX - 1);
+    const int n = (int)natoms;
+    size_t tb = 0;
+    HIPCHK(bin_sort_temp_bytes(n, (int)ncell, &tb));
+    auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t arr = up(sizeof(int) * std::max(n, 1));
+    CHK(ctx->binbuf.ensure(4 * arr + up(sizeof(int) * 104) + up(std::max<size_t>(tb, 1))));
+    char* base = static_cast<char*>(ctx->binbuf.p);
+    DevBinArgs da;
+    da.atoms = reinterpret_cast<const AtomIO*>(atoms);
+    da.n = n;
+    da.g = g;
+    da.ncell = (int)ncell;
+    da.key = reinterpret_cast<unsigned*>(base);
+    da.val = reinterpret_cast<int*>(base + arr);
+    da.key2 = reinterpret_cast<unsigned*>(base + 2 * arr);
+    da.val2 = reinterpret_cast<int*>(base + 3 * arr);
+    da.flags = reinterpret_cast<int*>(base + 4 * arr);
+    da.temp = base + 4 * arr + up(sizeof(int) * 104);
+    da.temp_bytes = tb;
+    HIPCHK(launch_bin_keys(da, ctx->stream));
+    int flags[104];
+    HIPCHK(hipMemcpyAsync(flags, da.flags, sizeof flags, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    return MSX_OK;
+    if (flags[0] != INT_MAX) {
+        msx_atom at;
+        HIPCHK(hipMemcpy(&at, atoms + flags[0], sizeof at, hipMemcpyDeviceToHost));
+        return bad_atom(flags[0], at.Z);
+    }
+    RadArgs ra;
+    std::memset(&ra, 0, sizeof ra);
+    SlotMap slots;
+    for (int Z = 0; Z < 104; Z++) slots.s[Z] = -1;
+    for (int Z = 1; Z <= 103; Z++)
+        if (flags[Z]) {
+            slots.s[Z] = (int16_t)ra.nsp;
+            ra.zs[ra.nsp++] = Z;
+        }
+    CHK(ctx->atoms.ensure(sizeof(AtomRec) * std::max(n, 1)));
+    CHK(ctx->cells.ensure(sizeof(int) * (ncell + 1)));
+    HIPCHK(launch_bin_sort(da, static_cast<int*>(ctx->cells.p), static_cast<AtomRec*>(ctx->atoms.p), slots,
+                           ctx->stream));
+    return splat_binned(ctx, ra, g, nx, ny, radius, table, formula, V, prec, where);
 }
 
 // 2 pi a0 e (V A^2): projected potential in V A of the Fourier sum of f_e/area
@@ -1858,6 +1933,19 @@ MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, do
     if (natoms < 0 || (natoms > 0 && !atoms) || !V) return fail(MSX_EINVAL, "bad atoms/V");
     HIPCHK(hipSetDevice(ctx->device));
     return potential_impl(ctx, atoms, natoms, a, b, c, nx, ny, nslices, radius, table, formula, V, prec, where);
+}
+
+MSX_API int msx_potential_device_atoms(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b,
+                                       double c, int nx, int ny, int nslices, double radius, int table, int formula,
+                                       void* V, int prec, int where) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    if (nx <= 0 || ny <= 0 || nx > 16384 || ny > 16384 || nslices <= 0) return fail(MSX_EINVAL, "bad grid %d x %d x %d", nx, ny, nslices);
+    if (!(a > 0) || !(b > 0) || !(c > 0) || !(radius > 0)) return fail(MSX_EINVAL, "a, b, c, radius must be > 0");
+    if (table < 0 || table > 1 || formula < 0 || formula > 1) return fail(MSX_EINVAL, "bad table/formula");
+    if (natoms < 0 || (natoms > 0 && !atoms) || !V) return fail(MSX_EINVAL, "bad atoms/V");
+    HIPCHK(hipSetDevice(ctx->device));
+    return potential_dev_impl(ctx, atoms, natoms, a, b, c, nx, ny, nslices, radius, table, formula, V, prec, where);
 }
 
 MSX_API int msx_potential_rec(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw,

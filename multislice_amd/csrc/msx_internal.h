@@ -107,8 +107,64 @@ struct RadArgs {
     int zs[104];  // Z of each species slot
     int nsp, noct, table, formula;
 };
+// Slab geometry of the splat's binning: slices of thickness dz, nbx x nby
+// lateral bins of the cell a x b.
+struct BinGeom {
+    double a, b, dz;
+    int nslices, nbx, nby;
+};
+// Slice and lateral bin of one atom, with x, y wrapped into the cell; false
+// for an atom outside the slab.  Slice rule of ModelPotential.cu:63-66
+// (kz*dz <= z < (kz+1)*dz, the floor corrected by one where rounding puts z
+// across a boundary).  Evaluated identically on the host and the GPU (no
+// contraction), so both bin and wrap every atom the same way; bins are
+// clamped into range for any finite input.
+__host__ __device__ inline bool bin_atom(const BinGeom& g, double x, double y, double z, long long* cell,
+                                         double* xw, double* yw) {
+#pragma clang fp contract(off)
+    const double fz = floor(z / g.dz);
+    if (!(fz >= 0.0 && fz < (double)g.nslices)) return false;
+    long long kz = (long long)fz;
+    if (!(kz * g.dz <= z && z < (kz + 1) * g.dz)) kz += (z < kz * g.dz) ? -1 : 1;
+    if (kz < 0 || kz >= g.nslices) return false;
+    x = x - g.a * floor(x / g.a);
+    y = y - g.b * floor(y / g.b);
+    if (x >= g.a) x -= g.a;
+    if (y >= g.b) y -= g.b;
+    const double fx = x / (g.a / g.nbx), fy = y / (g.b / g.nby);
+    const int bx = fx >= g.nbx - 1 ? g.nbx - 1 : (fx > 0.0 ? (int)fx : 0);
+    const int by = fy >= g.nby - 1 ? g.nby - 1 : (fy > 0.0 ? (int)fy : 0);
+    *cell = (kz * g.nbx + bx) * g.nby + by;
+    *xw = x;
+    *yw = y;
+    return true;
+}
 hipError_t launch_splat(const SplatArgs& a, hipStream_t s);
 hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s);
+
+// Binning of device-resident atoms (msx_binning.hip): the same records and
+// cell starts as the host binning (stable sort by cell, atoms of a cell in
+// input order).  flags[0]: index of the first invalid atom (Z outside 1..103
+// or a coordinate not finite), INT_MAX if none; flags[Z] = 1 for each Z
+// present.  key/val/key2/val2: n ints each.
+struct AtomIO;
+struct DevBinArgs {
+    const AtomIO* atoms;
+    int n;
+    BinGeom g;
+    int ncell;
+    unsigned *key, *key2;  // cell keys (sorted into key2)
+    int *val, *val2, *flags;
+    void* temp;
+    size_t temp_bytes;
+};
+struct SlotMap {
+    int16_t s[104];
+};
+hipError_t bin_sort_temp_bytes(int n, int ncell, size_t* bytes);
+hipError_t launch_bin_keys(const DevBinArgs& a, hipStream_t s);
+hipError_t launch_bin_sort(const DevBinArgs& a, int* cell_start, AtomRec* recs, const SlotMap& slots,
+                           hipStream_t s);
 hipError_t upload_fparams(hipStream_t s);
 
 // frozen-phonon displacement (msx_potential.hip); u_rms in device memory [104]

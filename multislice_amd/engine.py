@@ -92,7 +92,7 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
            "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read",
-           "msx_musli_stem_write", "msx_musli_wave_write"]
+           "msx_musli_stem_write", "msx_musli_wave_write", "msx_potential_device_atoms"]
 
 _lib = None
 _lock = threading.Lock()
@@ -141,6 +141,7 @@ def load():
         "msx_philox4x32_10": (None, [vp, vp, vp]),
         "msx_fft2d": (i, [vp, vp, i, i, i, i, i, i]),
         "msx_potential": (i, [vp, vp, C.c_int64, d, d, d, i, i, i, d, i, i, vp, i, i]),
+        "msx_potential_device_atoms": (i, [vp, vp, C.c_int64, d, d, d, i, i, i, d, i, i, vp, i, i]),
         "msx_transmission": (i, [vp, vp, i, i, i, d, d, d, d, vp, i, i]),
         "msx_multislice": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i, i]),
         "msx_stem": (i, [vp, C.POINTER(Stack), vp, C.POINTER(Probe), vp, i, C.POINTER(Detector), i, vp, i, d, i,
@@ -303,14 +304,24 @@ class Context:
 
     def potential(self, atoms, cell, nx, ny, nslices, radius, table=TABLE_LMULTISLICE,
                   formula=FORMULA_KIRKLAND, prec=F32, out=None):
-        """ModelPotential::calculatePotentialGrid -> V[nslices][nx][ny] (V*Angstrom)."""
-        at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        """ModelPotential::calculatePotentialGrid -> V[nslices][nx][ny] (V*Angstrom).
+        atoms: structured ATOM_DTYPE numpy array (binned on the host,
+        msx_potential) or a contiguous torch CUDA uint8 view of one, e.g. the
+        output of displace() on the GPU (binned on the GPU,
+        msx_potential_device_atoms; the same potential)."""
         if out is None:
             out = np.zeros((nslices, nx, ny), np.float64 if prec == F64 else np.float32)
         p, where, keep, arr = _buf(out, prec, complex_=False, writable=True)
-        _check(load().msx_potential(self.h, C.c_void_p(at.ctypes.data), len(at), float(cell[0]), float(cell[1]),
-                                    float(cell[2]), nx, ny, nslices, float(radius), table, formula, p, prec,
-                                    where))
+        geo = (float(cell[0]), float(cell[1]), float(cell[2]), nx, ny, nslices, float(radius), table, formula, p,
+               prec, where)
+        if _is_torch(atoms):
+            if not atoms.is_cuda or not atoms.is_contiguous() or atoms.numel() % ATOM_DTYPE.itemsize:
+                raise ValueError("device atoms: a contiguous CUDA tensor of whole ATOM_DTYPE records")
+            _check(load().msx_potential_device_atoms(self.h, C.c_void_p(atoms.data_ptr()),
+                                                     atoms.numel() // ATOM_DTYPE.itemsize, *geo))
+            return out
+        at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        _check(load().msx_potential(self.h, C.c_void_p(at.ctypes.data), len(at), *geo))
         return out
 
     def potential_rs(self, atoms, cell, nx, ny, nslices, atrad, periodic=True, dw=None, ads=None,

@@ -3,8 +3,9 @@
 `frozen_phonon_image` is the large-HRTEM path of the reference (LMultiSlice
 imageCalculation per configuration, ModelSimulated.cu:21-192, with Kirkland's
 objective lens, mulslice.cpp:612-656): every configuration's atoms are
-displaced on the GPU (msx_displace), its potential and transmission stack are
-built on the GPU, all configurations advance through one batched slice loop
+displaced on the GPU (msx_displace), binned and splatted into its potential
+there (msx_potential_device_atoms) and turned into its transmission stack;
+all configurations advance through one batched slice loop
 (msx_multislice_batch: one wave per configuration, each with its own
 transmission), and the objective-lens images are averaged.
 
@@ -39,12 +40,13 @@ def frozen_phonon_image(ctx, atoms, cell, n, nslices, kev, u_rms, nconf, seed, c
     cz = np.full(nslices, cell[2] / nslices)
     layer = np.arange(nslices)
     atd = torch.from_numpy(np.ascontiguousarray(atoms, engine.ATOM_DTYPE).view(np.uint8).copy()).cuda()
+    atc = torch.empty_like(atd)
     V = torch.empty((nslices, n, n), dtype=rdt, device="cuda")
     tm = {}
     t0 = _sync()
     trans = []
     for c in range(nconf):
-        at = engine.displace(atd, u_rms, seed, c, ctx=ctx).cpu().numpy().view(engine.ATOM_DTYPE)
+        at = engine.displace(atd, u_rms, seed, c, ctx=ctx, out=atc)
         ctx.potential(at, cell, n, n, nslices, radius, prec=prec, out=V)
         trans.append(ctx.transmission(V, cell[0], cell[1], kev))
     t1 = _sync()

@@ -80,3 +80,75 @@ def test_potential_edge_cases():
     V2 = ctx.potential(big, cell, 64, 64, 2, 3.0)
     np.testing.assert_array_equal(V1, V2)
     ctx.close()
+
+
+def _device_atoms(at):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(at).view(np.uint8).copy()).cuda()
+
+
+@pytest.mark.parametrize("radius", [3.0, 1.1])
+def test_device_binning_equals_host_binning(radius):
+    """msx_potential_device_atoms bins on the GPU (cell key per atom, stable
+    radix sort, cell starts by lower bound): the potential must be identical,
+    bit for bit, to msx_potential's host binning in both precisions, with atoms
+    on slice boundaries, on the cell edges, at negative and far-away
+    coordinates, outside the slab, and several atoms in one cell in input
+    order."""
+    ctx = msx.Context(0)
+    rng = np.random.default_rng(11 + int(radius))
+    cell = (12.3, 9.7, 4.0)
+    at = specimen(rng, 900, cell, [1, 6, 8, 14, 22, 38, 79])
+    nsl = 5
+    dz = cell[2] / nsl
+    at["z"][10:20] = np.arange(10) % nsl * dz          # on slice boundaries
+    at["x"][20:25] = [0.0, cell[0], -cell[0], 3 * cell[0] + 0.25, -1e-300]
+    at["y"][25:30] = [cell[1], -0.5, 1e7, -1e7, np.nextafter(cell[1], 0)]
+    at["z"][30:33] = [-1e-12, cell[2], 1e9]           # outside the slab
+    at[40:60] = at[40]                                # one cell, many atoms
+    at["occ"][40:60] = np.linspace(0.1, 1.0, 20)
+    for prec in (msx.F32, msx.F64):
+        kw = dict(prec=prec, table=0, formula=msx.FORMULA_KIRKLAND)
+        Vh = ctx.potential(at, cell, 64, 48, nsl, radius, **kw)
+        Vd = ctx.potential(_device_atoms(at), cell, 64, 48, nsl, radius, **kw)
+        np.testing.assert_array_equal(Vd, Vh)
+        # device output buffer too
+        import torch
+        out = torch.empty((nsl, 64, 48), dtype=torch.float32 if prec == msx.F32 else torch.float64, device="cuda")
+        ctx.potential(_device_atoms(at), cell, 64, 48, nsl, radius, out=out, **kw)
+        np.testing.assert_array_equal(out.cpu().numpy(), Vh)
+    ctx.close()
+
+
+def test_device_binning_edge_cases():
+    """No atoms, all atoms outside the slab, invalid atoms (the first one is
+    named, as by the host binning), and calls in a row of different sizes."""
+    ctx = msx.Context(0)
+    cell = (10.0, 10.0, 4.0)
+    V = ctx.potential(_device_atoms(np.zeros(0, dtype=msx.ATOM_DTYPE)), cell, 32, 32, 2, 3.0)
+    assert V.shape == (2, 32, 32) and not V.any()
+    out = np.zeros(3, dtype=msx.ATOM_DTYPE)
+    out["Z"], out["occ"] = 14, 1.0
+    out["z"] = [-0.5, 4.0, 7.0]
+    assert not ctx.potential(_device_atoms(out), cell, 32, 32, 2, 3.0).any()
+    rng = np.random.default_rng(5)
+    good = specimen(rng, 50, cell, [8, 14])
+    for field, val, msg in (("Z", 104, "atom 7: Z = 104"), ("Z", 0, "atom 7: Z = 0"),
+                            ("x", np.nan, "atom 7: coordinate not finite"),
+                            ("z", np.inf, "atom 7: coordinate not finite")):
+        bad = good.copy()
+        bad[field][7] = val
+        bad["Z"][30] = 200                             # a later invalid atom is not the one named
+        for atoms in (bad, _device_atoms(bad)):
+            with pytest.raises(msx.MsxError) as e:
+                ctx.potential(atoms, cell, 32, 32, 2, 3.0)
+            assert msg in str(e.value)
+    big = specimen(rng, 3000, cell, [8, 22, 38])
+    small = specimen(rng, 5, cell, [79])
+    V1 = ctx.potential(_device_atoms(big), cell, 64, 64, 2, 3.0)
+    Vs = ctx.potential(_device_atoms(small), cell, 64, 64, 2, 3.0)
+    V2 = ctx.potential(_device_atoms(big), cell, 64, 64, 2, 3.0)
+    np.testing.assert_array_equal(V1, V2)
+    np.testing.assert_array_equal(Vs, ctx.potential(small, cell, 64, 64, 2, 3.0))
+    np.testing.assert_array_equal(V1, ctx.potential(big, cell, 64, 64, 2, 3.0))
+    ctx.close()
