@@ -237,7 +237,8 @@ MSX_API int msx_potential(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, do
  * host, and the potential is identical to msx_potential's on the same atoms.
  * No counterpart in the reference, whose atoms always start on the host
  * (ModelPotential.cu:27-99); for frozen-phonon loops that displace on the
- * GPU.  At most INT_MAX - 1 atoms and slice x bin cells. */
+ * GPU.  At most INT_MAX - 1 atoms and slice x bin cells; atoms must lie in
+ * one device allocation of the context's GPU (checked: MSX_EINVAL). */
 MSX_API int msx_potential_device_atoms(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a, double b,
                                        double c, int nx, int ny, int nslices, double radius, int table, int formula,
                                        void* V, int prec, int where);

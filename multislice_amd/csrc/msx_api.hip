@@ -1346,6 +1346,26 @@ int potential_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, double a,
     return splat_binned(ctx, ra, g, nx, ny, radius, table, formula, V, prec, where);
 }
 
+// device atoms: natoms records inside one device allocation of this context's
+// GPU (checked before any kernel reads them)
+int check_device_atoms(msx_ctx ctx, const msx_atom* atoms, int64_t natoms) {
+    if (natoms == 0) return MSX_OK;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, atoms) != hipSuccess || attr.type != hipMemoryTypeDevice ||
+        attr.device != ctx->device) {
+        (void)hipGetLastError();
+        return fail(MSX_EINVAL, "atoms: not device memory of GPU %d", ctx->device);
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<msx_atom*>(atoms)) != hipSuccess ||
+        reinterpret_cast<const char*>(atoms) + natoms * sizeof(msx_atom) > static_cast<const char*>(base) + size) {
+        (void)hipGetLastError();
+        return fail(MSX_EINVAL, "atoms: %lld records run past the end of their allocation", (long long)natoms);
+    }
+    return MSX_OK;
+}
+
 // The same potential from atoms in device memory: binned on the GPU (stable
 // radix sort by cell), so the records and cell starts equal the host
 // binning's and the potential is identical.
@@ -1356,6 +1376,7 @@ int potential_dev_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, doubl
     if (natoms >= INT_MAX || ncell >= INT_MAX)
         return fail(MSX_EINVAL, "device atoms: %lld atoms, %lld cells (at most %d each)", (long long)natoms, ncell,
                     INT_MAX - 1);
+    CHK(check_device_atoms(ctx, atoms, natoms));
     const int n = (int)natoms;
     size_t tb = 0;
     HIPCHK(bin_sort_temp_bytes(n, (int)ncell, &tb));
@@ -1381,7 +1402,8 @@ int potential_dev_impl(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, doubl
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if (flags[0] != INT_MAX) {
         msx_atom at;
-        HIPCHK(hipMemcpy(&at, atoms + flags[0], sizeof at, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(&at, atoms + flags[0], sizeof at, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         return bad_atom(flags[0], at.Z);
     }
     RadArgs ra;

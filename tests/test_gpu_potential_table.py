@@ -152,3 +152,35 @@ def test_device_binning_edge_cases():
     np.testing.assert_array_equal(Vs, ctx.potential(small, cell, 64, 64, 2, 3.0))
     np.testing.assert_array_equal(V1, ctx.potential(big, cell, 64, 64, 2, 3.0))
     ctx.close()
+
+
+def test_device_atoms_pointer_checked():
+    """msx_potential_device_atoms refuses, before any kernel runs, atoms in
+    host memory and a record count that runs past the end of the device
+    allocation holding them."""
+    import ctypes as C
+    import torch
+    from multislice_amd import engine
+    ctx = msx.Context(0)
+    cell = (10.0, 10.0, 4.0)
+    good = specimen(np.random.default_rng(9), 50, cell, [8, 14])
+    V = torch.empty((2, 32, 32), dtype=torch.float32, device="cuda")
+    fn = msx.load().msx_potential_device_atoms
+    geo = (10.0, 10.0, 4.0, 32, 32, 2, 3.0, msx.TABLE_LMULTISLICE, msx.FORMULA_LMULTISLICE, C.c_void_p(V.data_ptr()),
+           msx.F32, msx.DEVICE)
+    host = np.ascontiguousarray(good)
+    with pytest.raises(msx.MsxError) as e:
+        engine._check(fn(ctx.h, C.c_void_p(host.ctypes.data), len(host), *geo))
+    assert "not device memory" in str(e.value)
+    dev = _device_atoms(good)
+    seg = torch.cuda.memory_snapshot()
+    seg_bytes = max(s["total_size"] for s in seg if s["address"] <= dev.data_ptr() < s["address"] + s["total_size"])
+    too_many = seg_bytes // msx.ATOM_DTYPE.itemsize + 1
+    with pytest.raises(msx.MsxError) as e:
+        engine._check(fn(ctx.h, C.c_void_p(dev.data_ptr()), too_many, *geo))
+    assert "past the end" in str(e.value)
+    engine._check(fn(ctx.h, C.c_void_p(dev.data_ptr()), len(good), *geo))
+    np.testing.assert_array_equal(V.cpu().numpy(), ctx.potential(good, cell, 32, 32, 2, 3.0,
+                                                                 table=msx.TABLE_LMULTISLICE,
+                                                                 formula=msx.FORMULA_LMULTISLICE))
+    ctx.close()
