@@ -274,9 +274,27 @@ public:
     void setCs(double v) { Cs_ = v; }
     void setAperture(double v) { aperture_ = v; }
     void setDefocus(double v) { defocus_ = v; }
-    // Kirkland aberration phase chi(k) = pi lambda k^2 (0.5 Cs lambda^2 k^2 - df)
-    // for |k|^2 = k2 (1/A^2); replaces Microscope::alpha (Microscope.cpp:21-25).
-    double alpha(double k2) const {
+    // Microscope::Es (Microscope.cpp:17-21): the envelope
+    // exp(-(pi aperture / (1000 lambda))^2 (Cs lambda^3 k^3 + defocus lambda k)^2)
+    // for a spatial frequency k, with Cs and defocus taken as stored (no unit
+    // conversion, as there).
+    double Es(double k) const {
+        const double l = getLambda();
+        const double a = M_PI * aperture_ / (l * 1000.0);
+        const double b = Cs_ * l * l * l * k * k * k + defocus_ * l * k;
+        return std::exp(-(a * a) * (b * b));
+    }
+    // Microscope::alpha (Microscope.cpp:23-25): pi lambda k^2 (0.5 Cs lambda^2 k^2
+    // - 10 defocus) for a spatial frequency k, Cs as stored.
+    double alpha(double k) const {
+        const double l = getLambda();
+        return M_PI * l * k * k * (0.5 * Cs_ * l * l * k * k - defocus_ * 10.0);
+    }
+    // Kirkland's aberration phase chi = pi lambda k2 (0.5 Cs lambda^2 k2 - df) for
+    // |k|^2 = k2 (1/A^2), with Cs converted from mm and defocus from nm: the
+    // phase of the objective lens that imageCalculation applies
+    // (slicelib.cpp:258-318 without the higher aberrations).
+    double chi_k2(double k2) const {
         const double l = getLambda(), cs = Cs_ * 1e7, df = defocus_ * 10.0;
         return M_PI * l * k2 * (0.5 * cs * l * l * k2 - df);
     }
@@ -293,14 +311,18 @@ private:
 // Kirkland's vzatom.
 template <class ModelT = amodel::Model> class ModelPotentialT {
 public:
+    // ModelPotential(void) (ModelPotential.h:29): no model and an empty grid;
+    // calculatePotentialGrid() then returns -1.
+    ModelPotentialT() : potential(nullptr), model_(nullptr), nx_(0), ny_(0), nz_(0), dpa_(0), radius_(0) {}
     ModelPotentialT(ModelT* model, size_t nx, size_t ny, size_t nz, double dpa, double radius)
         : potential(nullptr), model_(model), nx_(nx), ny_(ny), nz_(nz), dpa_(dpa), radius_(radius) {
         store_.assign(nx * ny * nz, 0.0);
         potential = store_.data();
     }
     int calculatePotentialGrid() {
+        if (!model_ || nx_ * ny_ * nz_ == 0) return -1;
         msx_ctx c = context();
-        if (!c || !model_) return -1;
+        if (!c) return -1;
         const size_t n = model_->getNumberAtoms();
         auto* cells = model_->getTableCell();
         const double a = model_->getA(), b = model_->getB(), cc = model_->getC();
@@ -361,11 +383,15 @@ using ModelPotential = ModelPotentialT<amodel::Model>;
 // doubles (nChannels 2), like Image::copyFFTtoImage (Image.h:62-73).
 template <class MP = ModelPotential> class ModelSimulatedT {
 public:
+    // ModelSimulated(void) (ModelSimulated.h:9): no potential; imageCalculation
+    // then returns -1.
+    ModelSimulatedT() : mp_(nullptr), nx_(0), ny_(0), nz_(0), dpa_(0) {}
     ModelSimulatedT(MP* mp, size_t nx, size_t ny, size_t nz, double dpa)
         : mp_(mp), nx_(nx), ny_(ny), nz_(nz), dpa_(dpa) {}
     template <class ImageT> int imageCalculation(ImageT* result, const Microscope* mic) {
+        if (!mp_ || !mp_->getModel() || !result || !mic || nx_ * ny_ * nz_ == 0) return -1;
         msx_ctx c = context();
-        if (!c || !mp_ || !result || !mic) return -1;
+        if (!c) return -1;
         auto* model = mp_->getModel();
         const double a = model->getA(), b = model->getB(), cc = model->getC();
         const size_t n = nx_ * ny_;
@@ -425,14 +451,17 @@ private:
 using ModelSimulated = ModelSimulatedT<ModelPotential>;
 
 // ----------------------------------------------------------------- PotentialBuilder
-// PotentialBuilder.h:14-22: the reference DLL is an empty skeleton whose only
-// function returns 42; kept for link compatibility.
-class CPotentialBuilder {
+// PotentialBuilder.h:14-22: the reference DLL is an empty skeleton that
+// exports a class with an empty constructor, the variable nPotentialBuilder
+// (0) and the function fnPotentialBuilder() (returns 42).  The same three
+// symbols are defined once in libmsx.so (msx_formats.hip), so a caller links
+// them from the library as it linked them from PotentialBuilder.dll.
+class MSX_API CPotentialBuilder {
 public:
-    CPotentialBuilder() = default;
-    int fnPotentialBuilder() { return 42; }
+    CPotentialBuilder(void);
 };
-inline int nPotentialBuilder = 0;
+extern MSX_API int nPotentialBuilder;
+MSX_API int fnPotentialBuilder(void);
 
 // ----------------------------------------------------------------- Dispatcher
 // The run command of an LMultiSlice XML file (Dispatcher.h, Dispatcher.cu:
@@ -568,6 +597,8 @@ inline int dispatcher_run(const char* fileNameXML) {
 
 #ifdef MSX_LMS_GLOBAL_NAMES
 using msx::lms::CPotentialBuilder;
+using msx::lms::fnPotentialBuilder;
+using msx::lms::nPotentialBuilder;
 using msx::lms::Microscope;
 using msx::lms::ModelPotential;
 using msx::lms::ModelSimulated;

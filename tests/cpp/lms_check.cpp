@@ -10,11 +10,44 @@
 
 #include "msx/lmultislice.hpp"
 
+// The reference's call sequence on the Microscope and PotentialBuilder
+// surfaces (Microscope.h:3-34, PotentialBuilder.h:14-22), written to those
+// headers' signatures; prints Es(k) and alpha(k) for tests/test_lms_surface.py.
+// Needs no GPU.
+static int surface() {
+    Microscope set;  // Microscope(void), then the setters
+    set.setKeV(300.0);
+    set.setCs(1.2);
+    set.setAperture(20.0);
+    set.setDefocus(-60.0);
+    // keV, Cs, aperture, defocus as the reference takes them (no unit conversion)
+    Microscope mics[] = {Microscope(200.0, 0.5, 10.0, 35.0), Microscope(200.0, 0.0, 10.0, 35.0),
+                         Microscope(200.0, 0.5, 10.0, 0.0), set};
+    const double ks[] = {0.0, 0.1, 0.5, 1.0, 2.5};
+    for (int m = 0; m < 4; m++) {
+        Microscope& mic = mics[m];
+        std::printf("mic %d keV=%.17g cs=%.17g aperture=%.17g defocus=%.17g lambda=%.17g sigma=%.17g\n", m,
+                    mic.getKeV(), mic.getCs(), mic.getAperture(), mic.getDefocus(), mic.getLambda(), mic.getSigma());
+        for (double k : ks) std::printf("val %d k=%.17g Es=%.17g alpha=%.17g\n", m, k, mic.Es(k), mic.alpha(k));
+    }
+    CPotentialBuilder builder;  // CPotentialBuilder(void)
+    (void)builder;
+    const int r = fnPotentialBuilder();
+    nPotentialBuilder += 1;
+    ModelPotential mp0;  // ModelPotential(void): no model
+    ModelSimulated ms0;  // ModelSimulated(void): no potential
+    msx::lms::Image img(4, 4, 1, 8, 2);
+    std::printf("fnPotentialBuilder=%d nPotentialBuilder=%d empty_potential=%d empty_image=%d\n", r,
+                nPotentialBuilder, mp0.calculatePotentialGrid(), ms0.imageCalculation(&img, &mics[0]));
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: lms_check <output prefix>\n");
+        std::fprintf(stderr, "usage: lms_check <output prefix> | --surface\n");
         return 2;
     }
+    if (std::string(argv[1]) == "--surface") return surface();
     const std::string out = argv[1];
     // Si [001], 2x2x1 conventional cells, fractional coordinates, planes at slice centres
     const float a = 5.431f;
@@ -47,8 +80,9 @@ int main(int argc, char** argv) {
     std::fwrite(result.data(), sizeof(double), 2 * nx * ny, f);
     std::fclose(f);
     CPotentialBuilder pb;
+    (void)pb;
     std::printf("lms_check ok atoms=%zu potential_sum=%.6e builder=%d\n", model.getNumberAtoms(),
                 [&] { double s = 0; for (size_t i = 0; i < nx * ny * nz; i++) s += mp.potential[i]; return s; }(),
-                pb.fnPotentialBuilder());
+                fnPotentialBuilder());
     return 0;
 }
