@@ -68,6 +68,15 @@ MSX_API int msx_destroy(msx_ctx ctx);
 /* Use an external HIP stream (hipStream_t passed as void*; NULL restores the
  * context's own stream). */
 MSX_API int msx_set_stream(msx_ctx ctx, void* hip_stream);
+/* Order the context stream after all work queued so far on hip_stream (a
+ * hipStream_t passed as void*; NULL = the null stream): an event recorded on
+ * hip_stream that the context stream waits for, with no host wait.  Call it
+ * before an MSX_DEVICE call whose inputs another stream produced (for example
+ * a framework's current stream); the MSX_DEVICE calls themselves return only
+ * after the context stream has finished, so their outputs need no wait.  No
+ * reference counterpart: the reference ran everything on one synchronous
+ * stream (Dispatcher.cu:204-207). */
+MSX_API int msx_wait_stream(msx_ctx ctx, void* hip_stream);
 MSX_API const char* msx_last_error(void);
 MSX_API int msx_version(void);
 /* Replay slice loops (msx_multislice, msx_stem batches) from cached HIP graphs

@@ -124,6 +124,7 @@ struct msx_ctx_s {
     // kernel profiling (msx_profile): per kernel kind launch count and time
     bool prof = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_order = nullptr;  // msx_wait_stream
     long long prof_n[MSX_PROF_KINDS] = {0};
     double prof_ms[MSX_PROF_KINDS] = {0};
     double prof_bytes[MSX_PROF_KINDS] = {0};
@@ -248,6 +249,7 @@ struct msx_ctx_s {
         if (h_pin) hipHostFree(h_pin);
         if (ev0) hipEventDestroy(ev0);
         if (ev1) hipEventDestroy(ev1);
+        if (ev_order) hipEventDestroy(ev_order);
         if (own) hipStreamDestroy(own);
     }
 };
@@ -1817,6 +1819,15 @@ MSX_API int msx_graph_stats(msx_ctx ctx, int64_t* captures, int64_t* replays) {
 MSX_API int msx_set_stream(msx_ctx ctx, void* s) {
     if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
     ctx->stream = s ? static_cast<hipStream_t>(s) : ctx->own;
+    return MSX_OK;
+}
+
+MSX_API int msx_wait_stream(msx_ctx ctx, void* s) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->ev_order) HIPCHK(hipEventCreateWithFlags(&ctx->ev_order, hipEventDisableTiming));
+    HIPCHK(hipEventRecord(ctx->ev_order, static_cast<hipStream_t>(s)));
+    HIPCHK(hipStreamWaitEvent(ctx->stream, ctx->ev_order, 0));
     return MSX_OK;
 }
 

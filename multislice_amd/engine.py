@@ -82,7 +82,7 @@ class Lens(C.Structure):
 
 PROF_ROW, PROF_COL, PROF_SPLAT, PROF_DET = 0, 1, 2, 3
 
-EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_set_graphs", "msx_graph_stats", "msx_autotune",
+EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_wait_stream", "msx_set_graphs", "msx_graph_stats", "msx_autotune",
            "msx_set_tuning", "msx_last_error", "msx_version",
            "msx_profile", "msx_profile_read", "msx_copy_probe", "msx_displace", "msx_philox4x32_10",
            "msx_wavelength", "msx_sigma", "msx_vzatom", "msx_featom", "msx_fft2d", "msx_potential",
@@ -130,6 +130,7 @@ def load():
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     sig = {
         "msx_create": (i, [i, C.POINTER(vp)]), "msx_destroy": (i, [vp]), "msx_set_stream": (i, [vp, vp]),
+        "msx_wait_stream": (i, [vp, vp]),
         "msx_set_graphs": (i, [vp, i]),
         "msx_autotune": (i, [vp, i, i, i, i, C.POINTER(Tuning)]), "msx_set_tuning": (i, [vp, i, i, i, C.POINTER(Tuning)]), "msx_graph_stats": (i, [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
         "msx_last_error": (C.c_char_p, []), "msx_version": (i, []),
@@ -222,6 +223,41 @@ def _ptr_where(args):
     return wh.pop()
 
 
+def _need(cond, msg):
+    """Argument check of a wrapper: raise ValueError before anything reaches
+    the C ABI, whose kernels size their reads and writes from these shapes."""
+    if not cond:
+        raise ValueError(msg)
+
+
+def _numel(x):
+    return int(np.prod(tuple(x.shape))) if len(tuple(x.shape)) else 1
+
+
+def _check_layers(nl, cz, layer):
+    """cz: thickness per layer; layer: layer index per slice, each below both
+    len(cz) and the number of transmission images nl."""
+    cz = np.asarray(cz)
+    layer = np.asarray(layer)
+    _need(cz.ndim == 1 and len(cz) >= 1, "cz must be a non-empty 1-D sequence of slice thicknesses")
+    _need(layer.ndim == 1 and len(layer) >= 1, "layer must be a non-empty 1-D sequence of layer indices")
+    _need(np.issubdtype(layer.dtype, np.integer), "layer indices must be integers")
+    top = min(int(nl), len(cz))
+    _need(int(layer.min()) >= 0 and int(layer.max()) < top,
+          f"layer indices must lie in [0, {top}) (transmission images {nl}, thicknesses {len(cz)})")
+
+
+def _check_trans(trans):
+    _need(len(tuple(trans.shape)) == 3, "trans must be (nlayer, nx, ny)")
+    _need(trans.shape[0] >= 1, "trans holds no layer")
+    return tuple(trans.shape)
+
+
+def _check_waves(name, x, nx, ny):
+    shp = tuple(x.shape)
+    _need(len(shp) >= 2 and shp[-2:] == (nx, ny), f"{name} must end in ({nx}, {ny}), got {shp}")
+
+
 class Context:
     """An msx context bound to one GPU (msx_create / msx_destroy)."""
 
@@ -242,6 +278,17 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def _order(self, *xs):
+        """Make the context stream wait for the work torch has queued on its
+        current stream (msx_wait_stream), before a call that reads device
+        tensors which that work may still be producing."""
+        for x in xs:
+            if x is not None and _is_torch(x) and x.is_cuda:
+                import torch
+                raw = torch.cuda.current_stream(x.device).cuda_stream
+                _check(load().msx_wait_stream(self.h, C.c_void_p(raw) if raw else None))
+                return
 
     def set_stream(self, stream):
         """Queue work on an external stream (torch.cuda.Stream or raw handle)."""
@@ -288,6 +335,7 @@ class Context:
         if src.numel() < nimg * nx * ny or dst.numel() < nimg * nx * ny:
             raise ValueError("copy_probe: buffers smaller than nimg * nx * ny")
         ms = C.c_double()
+        self._order(src, dst)
         _check(load().msx_copy_probe(self.h, int(mode), C.c_void_p(src.data_ptr()), C.c_void_p(dst.data_ptr()),
                                      int(nimg), int(nx), int(ny), C.byref(ms)))
         return ms.value
@@ -298,7 +346,9 @@ class Context:
         prec = _prec_of(data)
         p, where, keep, arr = _buf(data, prec, writable=True)
         nx, ny = data.shape[-2], data.shape[-1]
+        _need(len(tuple(data.shape)) >= 2, "fft2d needs (..., nx, ny)")
         nb = int(np.prod(data.shape[:-2])) if len(data.shape) > 2 else 1
+        self._order(data)
         _check(load().msx_fft2d(self.h, p, nb, nx, ny, int(sign), prec, where))
         return data
 
@@ -311,16 +361,20 @@ class Context:
         msx_potential_device_atoms; the same potential)."""
         if out is None:
             out = np.zeros((nslices, nx, ny), np.float64 if prec == F64 else np.float32)
+        _need(tuple(out.shape) == (nslices, nx, ny), f"out must be ({nslices}, {nx}, {ny}), got {tuple(out.shape)}")
         p, where, keep, arr = _buf(out, prec, complex_=False, writable=True)
         geo = (float(cell[0]), float(cell[1]), float(cell[2]), nx, ny, nslices, float(radius), table, formula, p,
                prec, where)
         if _is_torch(atoms):
             if not atoms.is_cuda or not atoms.is_contiguous() or atoms.numel() % ATOM_DTYPE.itemsize:
                 raise ValueError("device atoms: a contiguous CUDA tensor of whole ATOM_DTYPE records")
+            _need(where == DEVICE, "device atoms need a device output (a CUDA tensor)")
+            self._order(atoms, out)
             _check(load().msx_potential_device_atoms(self.h, C.c_void_p(atoms.data_ptr()),
                                                      atoms.numel() // ATOM_DTYPE.itemsize, *geo))
             return out
         at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
+        self._order(out)
         _check(load().msx_potential(self.h, C.c_void_p(at.ctypes.data), len(at), *geo))
         return out
 
@@ -340,6 +394,7 @@ class Context:
         cv = np.ascontiguousarray(curves, np.float64)
         if cv.ndim != 2:
             raise ValueError("curves must be (ntypes, npoints)")
+        _need(tp.ndim == 1 and len(tp) == len(np.atleast_1d(atoms)), "types: one curve row per atom")
         self._keep_curves = (tp, cv)
         head = (C.c_void_p(tp.ctypes.data), C.c_void_p(cv.ctypes.data), cv.shape[0], cv.shape[1], float(dds))
         extra = (int(nmult), int(mmult)) if kind == "rec" else (float(atrad), int(periodic))
@@ -357,6 +412,9 @@ class Context:
             out = np.zeros((nslices, nx, ny), dt)
         if absorptive and out_abs is None:
             out_abs = np.zeros((nslices, nx, ny), dt) if not _is_torch(out) else out.new_empty(out.shape)
+        for nm, o in (("out", out), ("out_abs", out_abs)):
+            _need(o is None or tuple(o.shape) == (nslices, nx, ny),
+                  f"{nm} must be ({nslices}, {nx}, {ny}), got {None if o is None else tuple(o.shape)}")
         pv = _buf(out, prec, complex_=False, writable=True)
         pa = _buf(out_abs, prec, complex_=False, writable=True) if out_abs is not None else (None, pv[1], None, None)
         where = _ptr_where([pv, pa])
@@ -369,6 +427,7 @@ class Context:
                  prec, where]
         fn = {"rs": load().msx_potential_rs, "rec": load().msx_potential_rec,
               "rs_curve": load().msx_potential_rs_curve, "rec_curve": load().msx_potential_rec_curve}[kind]
+        self._order(out, out_abs)
         _check(fn(*args))
         return (out, out_abs) if out_abs is not None else out
 
@@ -383,7 +442,10 @@ class Context:
     def transmission(self, V, ax, by, kev, bw=2.0 / 3.0, out=None, Vabs=None):
         """exp(i sigma V / 1000), band-limited; V: (nlayer, nx, ny) real."""
         prec = _prec_of(V)
+        _need(len(tuple(V.shape)) == 3, "V must be (nlayer, nx, ny)")
         nl, nx, ny = V.shape
+        _need(out is None or tuple(out.shape) == (nl, nx, ny), f"out must be ({nl}, {nx}, {ny})")
+        _need(Vabs is None or tuple(Vabs.shape) == (nl, nx, ny), f"Vabs must be ({nl}, {nx}, {ny})")
         pv = _buf(V, prec, complex_=False)
         if out is None:
             if _is_torch(V):
@@ -399,6 +461,7 @@ class Context:
             pa = _buf(Vabs, prec, complex_=False)
             bufs.append(pa)
         where = _ptr_where(bufs)
+        self._order(V, Vabs, out)
         _check(load().msx_transmission_abs(self.h, pv[0], None if pa is None else pa[0], nl, nx, ny, ax, by, kev, bw,
                                            pt[0], prec, where))
         return out
@@ -416,7 +479,10 @@ class Context:
         """Slice loop (mulslice.cpp:703-739) on nwave waves psi (nwave, nx, ny) in place.
         With psi0 (one image) every wave starts from psi0 and psi is output only."""
         prec = _prec_of(trans)
-        nl, nx, ny = trans.shape
+        nl, nx, ny = _check_trans(trans)
+        _check_layers(nl, cz, layer)
+        _check_waves("psi", psi, nx, ny)
+        _need(psi0 is None or _numel(psi0) == nx * ny, f"psi0 must be one ({nx}, {ny}) image")
         st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
         pt = _buf(trans, prec)
         pw = _buf(psi, prec, writable=True)
@@ -428,6 +494,7 @@ class Context:
             p0 = b0[0]
         where = _ptr_where(bufs)
         nwave = int(np.prod(psi.shape[:-2])) if len(psi.shape) > 2 else 1
+        self._order(trans, psi, psi0)
         _check(load().msx_multislice(self.h, C.byref(st), pt[0], p0, pw[0], nwave, prec, where))
         return psi
 
@@ -441,7 +508,10 @@ class Context:
         prec = _prec_of(jobs[0]["trans"])
         keep, arr = [], (Job * len(jobs))()
         for k, j in enumerate(jobs):
-            nl, nx, ny = j["trans"].shape
+            nl, nx, ny = _check_trans(j["trans"])
+            _check_layers(nl, j["cz"], j["layer"])
+            _need(_numel(j["psi"]) == nx * ny, f"job {k}: psi must be one ({nx}, {ny}) image")
+            _need(j.get("psi0") is None or _numel(j["psi0"]) == nx * ny, f"job {k}: psi0 must be one ({nx}, {ny}) image")
             st, kp = self._stack(nx, ny, j["ax"], j["by"], j["kev"], j["cz"], j["layer"], j.get("tilt", (0.0, 0.0)),
                                  j.get("bw", 2.0 / 3.0))
             keep += [st, kp, j]
@@ -451,6 +521,7 @@ class Context:
                     raise ValueError(f"job {k}: {key} must be a contiguous CUDA tensor of the batch precision")
             arr[k] = Job(C.addressof(st), j["trans"].data_ptr(),
                          j["psi0"].data_ptr() if j.get("psi0") is not None else None, j["psi"].data_ptr())
+        self._order(jobs[0]["trans"])
         _check(load().msx_multislice_batch(self.h, arr, len(jobs), prec, DEVICE))
 
     def stem(self, trans, cz, layer, ax, by, kev, probe, positions, detectors, checkpoints,
@@ -459,9 +530,11 @@ class Context:
         probe: dict(alpha_max, alpha_min=0, cs=0, df=0) (rad, Angstrom);
         detectors: [(k_inner, k_outer)] in 1/Angstrom; tilt: crystal tilt (rad)."""
         prec = _prec_of(trans)
-        nl, nx, ny = trans.shape
+        nl, nx, ny = _check_trans(trans)
+        _check_layers(nl, cz, layer)
         st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
         pt = _buf(trans, prec)
+        _need(np.size(positions) % 2 == 0 and np.size(positions) > 0, "positions must be (nprobe, 2)")
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 2)
         det = np.ascontiguousarray(detectors, np.float64).reshape(-1, 2)
         ck = np.ascontiguousarray(checkpoints, np.int32)
@@ -472,6 +545,11 @@ class Context:
         if out is None:
             out = np.zeros((len(ck), len(det), len(pos)), np.float64)
             acc = 0
+        _need(isinstance(out, np.ndarray) and out.dtype == np.float64 and out.flags.c_contiguous
+              and out.flags.writeable, "out must be a writable C-contiguous float64 numpy array")
+        _need(out.shape == (len(ck), len(det), len(pos)),
+              f"out must be ({len(ck)}, {len(det)}, {len(pos)}) = (checkpoints, detectors, probes), got {out.shape}")
+        self._order(trans)
         _check(load().msx_stem(self.h, C.byref(st), pt[0], C.byref(pr), C.c_void_p(pos.ctypes.data), len(pos),
                                dets, len(det), C.c_void_p(ck.ctypes.data), len(ck), float(weight), acc,
                                C.c_void_p(out.ctypes.data), int(batch), prec, pt[1]))
@@ -486,7 +564,8 @@ class Context:
         of angles, number of defocus steps); the image is a CUDA tensor when
         trans is one, else a numpy array."""
         prec = _prec_of(trans)
-        nl, nx, ny = trans.shape
+        nl, nx, ny = _check_trans(trans)
+        _check_layers(nl, cz, layer)
         st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
         pt = _buf(trans, prec)
         g = lambda k: float(lens.get(k, 0.0))  # noqa: E731
@@ -499,6 +578,7 @@ class Context:
             img = np.zeros((nx, ny), np.float64)
             pimg = C.c_void_p(img.ctypes.data)
         na, nd = C.c_int32(0), C.c_int32(0)
+        self._order(trans)
         _check(load().msx_partial_coherence(self.h, C.byref(st), pt[0], C.byref(ln), float(acmin), float(acmax),
                                             float(sigmaf), float(dfdelt), int(batch), pimg, C.byref(na),
                                             C.byref(nd), prec, pt[1]))
@@ -507,6 +587,7 @@ class Context:
     def ctf_image(self, psi, ax, by, kev, cs, df, aobj, want_wave=False):
         """Objective-lens image |ifft(fft(psi) exp(-i chi) A)|^2 of (nwave, nx, ny) waves."""
         prec = _prec_of(psi)
+        _need(len(tuple(psi.shape)) >= 2, "psi must be (..., nx, ny)")
         nx, ny = psi.shape[-2], psi.shape[-1]
         nw = int(np.prod(psi.shape[:-2])) if len(psi.shape) > 2 else 1
         pw = _buf(psi, prec)
@@ -519,6 +600,7 @@ class Context:
             img = np.zeros(psi.shape, psi.dtype) if want_wave else None
         pi = _buf(inten, prec, complex_=False, writable=True)
         pimg = _buf(img, prec, writable=True)[0] if want_wave else None
+        self._order(psi)
         _check(load().msx_ctf_image(self.h, pw[0], nw, nx, ny, ax, by, kev, cs, df, aobj, pimg, pi[0], prec,
                                     pw[1]))
         return (inten, img) if want_wave else inten
@@ -534,10 +616,21 @@ def unet_conv(ctx, mode, ks, out, w, x=None, bias=None, g=None, y=None, relu=Fal
         if not (_is_torch(t) and t.is_cuda and t.is_contiguous() and str(t.dtype) == "torch.float32"):
             raise ValueError("msx_unet_conv takes contiguous float32 CUDA tensors")
         return C.c_void_p(t.data_ptr())
+    _need(mode in (0, 1, 2) and ks in (1, 3), "mode must be 0, 1 or 2 and ks 1 or 3")
+    _need(len(tuple(w.shape)) == 2 and w.shape[1] % (ks * ks) == 0, "w must be (cout, cin*ks*ks)")
     ref = x if x is not None else g
+    _need(ref is not None and len(tuple(ref.shape)) == 4, "x (or g) must be (n, c, h, w)")
     n, _, hh, ww = ref.shape
     cout = w.shape[0]
     cin = w.shape[1] // (ks * ks)
+    shapes = {"x": (n, cin, hh, ww), "g": (n, cout, hh, ww), "y": (n, cout, hh, ww),
+              "out": {0: (n, cout, hh, ww), 1: (n, cin, hh, ww), 2: (cout, cin * ks * ks)}[mode]}
+    for nm, t in (("x", x), ("g", g), ("y", y), ("out", out)):
+        _need(t is None or tuple(t.shape) == shapes[nm], f"{nm} must be {shapes[nm]}, got {None if t is None else tuple(t.shape)}")
+    _need(bias is None or tuple(bias.shape) == (cout,), f"bias must be ({cout},)")
+    _need(out is not None and (x is not None if mode in (0, 2) else g is not None) and (mode == 0 or g is not None),
+          "mode 0 needs x and out, mode 1 g and out, mode 2 x, g and out")
+    ctx._order(x, w, bias, g, y, out)
     _check(load().msx_unet_conv(ctx.h, mode, ks, ptr(x), ptr(w), ptr(bias), ptr(g), ptr(y), ptr(out), n, cin, cout,
                                 hh, ww, int(relu)))
     return out
@@ -592,8 +685,10 @@ class UNet:
         n, hh, ww = bx[2].shape
         if out is None:
             out = np.empty((n, hh, ww), np.float32) if bx[1] == HOST else x.new_empty((n, hh, ww))
+        _need(tuple(out.shape) == (n, hh, ww), f"out must be ({n}, {hh}, {ww})")
         bo = _buf(out, F32, complex_=False, writable=True)
         where = _ptr_where([bx, bo])
+        self.ctx._order(x, out)
         _check(load().msx_unet_forward(self.h, bx[0], bo[0], n, hh, ww, where))
         return out
 
@@ -601,9 +696,11 @@ class UNet:
         """(loss, packed gradient as numpy float32)."""
         bx, bt = self._img(x), self._img(target)
         n, hh, ww = bx[2].shape
+        _need(tuple(bt[2].shape) == (n, hh, ww), "target must have the shape of x")
         g = np.empty(self.nparam, np.float32)
         loss = C.c_double()
         where = _ptr_where([bx, bt])
+        self.ctx._order(x, target)
         if where == DEVICE:
             import torch
             gd = torch.empty(self.nparam, dtype=torch.float32, device=x.device)
@@ -618,7 +715,9 @@ class UNet:
     def train_step(self, x, target, lr):
         bx, bt = self._img(x), self._img(target)
         n, hh, ww = bx[2].shape
+        _need(tuple(bt[2].shape) == (n, hh, ww), "target must have the shape of x")
         loss = C.c_double()
+        self.ctx._order(x, target)
         _check(load().msx_unet_train_step(self.h, bx[0], bt[0], n, hh, ww, float(lr), C.byref(loss),
                                           _ptr_where([bx, bt])))
         return loss.value
@@ -733,15 +832,22 @@ def displace(atoms, u_rms, seed, config, ctx=None, out=None):
     if _is_torch(atoms):
         if ctx is None:
             raise ValueError("device displacement needs a Context")
+        _need(atoms.is_cuda and atoms.is_contiguous() and atoms.numel() % ATOM_DTYPE.itemsize == 0,
+              "device atoms: a contiguous CUDA tensor of whole ATOM_DTYPE records")
         n = atoms.numel() // ATOM_DTYPE.itemsize
         if out is None:
             out = atoms.clone()
+        _need(_is_torch(out) and out.is_cuda and out.is_contiguous() and out.numel() == atoms.numel(),
+              "out must be a contiguous CUDA tensor of the atoms' size")
+        ctx._order(atoms, out)
         _check(load().msx_displace(ctx.h, C.c_void_p(atoms.data_ptr()), n, C.c_void_p(u.ctypes.data), seed, config,
                                    C.c_void_p(out.data_ptr()), DEVICE))
         return out
     at = np.ascontiguousarray(atoms, dtype=ATOM_DTYPE)
     if out is None:
         out = np.empty_like(at)
+    _need(isinstance(out, np.ndarray) and out.dtype == ATOM_DTYPE and out.shape == at.shape and out.flags.c_contiguous,
+          "out must be a C-contiguous ATOM_DTYPE array of the atoms' length")
     _check(load().msx_displace(ctx.h if ctx else None, C.c_void_p(at.ctypes.data), len(at),
                                C.c_void_p(u.ctypes.data), seed, config, C.c_void_p(out.ctypes.data), HOST))
     return out
