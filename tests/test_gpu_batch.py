@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import multislice_amd as msx
-from conftest import rel_l2
+from conftest import max_rel, rel_l2
 from oracle import Oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -97,6 +97,45 @@ def test_amorphous_au_jobs_match_oracle(ctx):
               f"FP32 {e32:.1e}, FP64 {e64:.1e}")
         assert e32 < 1e-5
         assert e64 < 1e-10
+
+
+@pytest.mark.timeout(600)
+def test_large_grid_jobs_fp32_vs_fp64_and_oracle(ctx):
+    """The uneven-job configuration's large grids with its slice range: jobs
+    of 512^2, 768^2 and 1024^2 with 20-200 slices of 2 A (amorphous carbon +
+    Au, 200/300 kV), batched, FP32 against the FP64 engine per job (rel-L2
+    1e-5, max-rel 1e-4), bit-identical to the same job run alone; the 768^2
+    job (the radix-3 grid) against the oracle's FP64 pipeline."""
+    import torch
+    from multislice_amd import hrtem
+    kw = dict(sizes=(512, 768, 1024), seed=33, max_au=50, slices=(20, 200))
+    jobs32, specs = hrtem.uneven_jobs(ctx, np.random.default_rng(33), 5, **kw)
+    jobs64, _ = hrtem.uneven_jobs(ctx, np.random.default_rng(33), 5, prec=msx.F64, **kw)
+    assert {j["n"] for j in jobs32} == {512, 768, 1024}  # seed 33 draws 1024, 512, 768, 1024, 512
+    outs = []
+    for jobs, cdt in ((jobs32, torch.complex64), (jobs64, torch.complex128)):
+        for j in jobs:
+            j["psi"] = torch.ones((j["n"], j["n"]), dtype=cdt, device="cuda")
+        ctx.multislice_batch(jobs)
+        outs.append([j["psi"].cpu().numpy() for j in jobs])
+    for j, a, b in zip(jobs32, *outs):
+        e, m = rel_l2(a, b), max_rel(a, b)
+        print(f"{j['n']}^2, {len(j['layer'])} slices, {j['kev']:.0f} kV: FP32 vs FP64 rel-L2 {e:.1e} max-rel {m:.1e}")
+        assert e < 1e-5 and m <= 1e-4
+        psi = torch.ones((j["n"], j["n"]), dtype=torch.complex64, device="cuda")
+        ctx.multislice(j["trans"], j["cz"], j["layer"], psi, j["ax"], j["by"], j["kev"])
+        assert np.array_equal(psi.cpu().numpy(), a)
+    for j, (at, cell), b in zip(jobs64, specs, outs[1]):
+        if j["n"] != 768:
+            continue
+        n, ns = j["n"], len(j["layer"])
+        V = O.potential(at["Z"], np.stack([at["x"], at["y"], at["z"]], 1), cell, n, n, ns, 3.0)
+        t = O.transmission(V, cell[0], cell[1], j["kev"])
+        ref = O.multislice(t, j["cz"], j["layer"], np.ones((n, n), np.complex128), cell[0], cell[1], j["kev"])
+        e64 = rel_l2(b, ref)
+        print(f"768^2 job, {ns} slices: FP64 engine vs oracle rel-L2 {e64:.1e}")
+        assert e64 < 1e-10
+        break
 
 
 def test_batch_rejects_host_buffers(ctx):

@@ -13,7 +13,7 @@ import pytest
 import multislice_amd as msx
 from multislice_amd import specimen
 from oracle import Oracle as O
-from conftest import rel_l2
+from conftest import max_rel, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -162,3 +162,6 @@ def test_ctf_image_matches_oracle(ctx, si_layers, prec):
     tol = 1e-12 if prec == "f64" else 2e-6
     assert rel_l2(w[0], ref_w) < tol
     assert rel_l2(inten[0], ref_i) < 2 * tol
+    mr = max_rel(inten[0], ref_i)
+    print(f"CTF image {prec}: rel-L2 {rel_l2(inten[0], ref_i):.2e}, max-rel {mr:.2e}")
+    assert mr <= (1e-11 if prec == "f64" else 1e-4)

@@ -137,6 +137,68 @@ def test_repeat_calls_are_independent(ctx, small):
     assert np.array_equal(a1, a2)
 
 
+def _uniform53(hi, lo):
+    """(u + 0.5) 2^-53 from the top 53 bits of two 32-bit words (msx_philox.h)."""
+    return ((((int(hi) << 32) | int(lo)) >> 11) + 0.5) * 2.0 ** -53
+
+
+def _device_displace(ctx, at, u, seed, config):
+    torch = pytest.importorskip("torch")
+    dev_in = torch.from_numpy(at.view(np.uint8).copy()).cuda()
+    return msx.displace(dev_in, u, seed=seed, config=config, ctx=ctx).cpu().numpy().view(msx.ATOM_DTYPE)
+
+
+def test_device_displacement_known_answer(ctx):
+    """Atom 0 of configuration 0 with seed 0 draws Philox-4x32-10 of counter
+    (0, 0, 0, 0) and key (0, 0): the first published known-answer vector
+    (Salmon et al. 2011, Random123 kat_vectors), 6627e8d5 e169c58d bc57ac4c
+    9b00dbd8.  Its Box-Muller pair is the x and y offset of that atom (in units
+    of u); z uses the next counter (0, 0, 0, 1)."""
+    at = np.zeros(3, msx.ATOM_DTYPE)
+    at["Z"], at["occ"] = 38, 1.0
+    d = _device_displace(ctx, at, {38: 1.0}, 0, 0)
+    u1, u2 = _uniform53(0x6627e8d5, 0xe169c58d), _uniform53(0xbc57ac4c, 0x9b00dbd8)
+    r = np.sqrt(-2.0 * np.log(u1))
+    assert abs(d["x"][0] - r * np.cos(2 * np.pi * u2)) < 1e-14
+    assert abs(d["y"][0] - r * np.sin(2 * np.pi * u2)) < 1e-14
+    w = msx.philox4x32_10([0, 0, 0, 1], [0, 0])
+    rz = np.sqrt(-2.0 * np.log(_uniform53(w[0], w[1])))
+    assert abs(d["z"][0] - rz * np.cos(2 * np.pi * _uniform53(w[2], w[3]))) < 1e-14
+
+
+def test_device_displacement_against_block_function(ctx):
+    """Every atom's offsets from the device equal Box-Muller of the host
+    Philox block function (pinned by the known-answer vectors) at counter
+    (atom, 0, configuration, 0 / 1) and key (seed lo, seed hi)."""
+    n, seed, config = 512, 0x1234_5678_9ABC, 7
+    at = np.zeros(n, msx.ATOM_DTYPE)
+    at["Z"], at["occ"] = 8, 1.0
+    at["x"] = np.arange(n) * 0.37
+    d = _device_displace(ctx, at, {8: 0.095}, seed, config)
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for i in range(0, n, 7):
+        a = msx.philox4x32_10([i, 0, config, 0], key)
+        b = msx.philox4x32_10([i, 0, config, 1], key)
+        ra, rb = np.sqrt(-2 * np.log(_uniform53(a[0], a[1]))), np.sqrt(-2 * np.log(_uniform53(b[0], b[1])))
+        assert abs(d["x"][i] - at["x"][i] - 0.095 * ra * np.cos(2 * np.pi * _uniform53(a[2], a[3]))) < 1e-13
+        assert abs(d["y"][i] - 0.095 * ra * np.sin(2 * np.pi * _uniform53(a[2], a[3]))) < 1e-14
+        assert abs(d["z"][i] - 0.095 * rb * np.cos(2 * np.pi * _uniform53(b[2], b[3]))) < 1e-14
+
+
+def test_device_displacement_moments(ctx):
+    """Mean 0 and RMS u per axis, independent axes, Gaussian 4th moment, on
+    200,000 atoms displaced on the GPU."""
+    n, u = 200000, 0.088
+    at = np.zeros(n, msx.ATOM_DTYPE)
+    at["Z"], at["occ"] = 38, 1.0
+    d = _device_displace(ctx, at, {38: u}, 7, 3)
+    dx = np.stack([d["x"], d["y"], d["z"]], 1)
+    assert np.abs(dx.mean(0)).max() < 4 * u / np.sqrt(n)
+    assert np.abs(dx.std(0) / u - 1).max() < 0.01
+    assert abs(np.corrcoef(dx[:, 0], dx[:, 1])[0, 1]) < 0.01
+    assert abs((dx[:, 2] ** 4).mean() / u ** 4 / 3 - 1) < 0.03
+
+
 def test_displacement_host_equals_device(ctx):
     torch = pytest.importorskip("torch")
     at, _ = specimen.srtio3_001((6, 6, 10))

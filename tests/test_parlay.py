@@ -1,7 +1,8 @@
 """Kirkland's layer stacking sequence (msx_parlay, replacing parlay of
 slicelib.cpp:558-713): the reference's own parser, compiled into oracle/_ref,
 pins every code and index -- as committed golden vectors and, when the build
-is present, live on random sequences.  CPU only (host code of libmsx.so)."""
+is present, live on random sequences.  Host code of libmsx.so: the golden
+vectors run in both the CPU and the GPU suite."""
 import ctypes as C
 import json
 import os
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 
 import multislice_amd as msx
+from conftest import BOTH_SUITES
 from oracle import ref
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "parlay_vectors.json")
@@ -22,7 +24,8 @@ def ours(s, lmax, nsmax):
     return rc, out[:n.value].tolist()
 
 
-def test_golden_vectors():
+@BOTH_SUITES
+def test_golden_vectors(suite):
     with open(GOLD) as f:
         cases = json.load(f)["cases"]
     assert len(cases) == 20
@@ -46,7 +49,8 @@ def test_random_sequences_against_reference():
         assert ours(s, lmax, nsmax) == (rc, out[:n.value].tolist()), repr(s)
 
 
-def test_python_wrapper():
+@BOTH_SUITES
+def test_python_wrapper(suite):
     assert msx.parlay("2(abc)d").tolist() == [0, 1, 2, 0, 1, 2, 3]
     assert len(msx.parlay("50(ab)")) == 100
     with pytest.raises(msx.MsxError) as e:

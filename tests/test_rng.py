@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import multislice_amd as msx
+from conftest import BOTH_SUITES
 from multislice_amd import specimen
 
 
@@ -15,7 +16,8 @@ from multislice_amd import specimen
     ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
      [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
 ])
-def test_philox_known_answers(ctr, key, want):
+@BOTH_SUITES
+def test_philox_known_answers(ctr, key, want, suite):
     assert list(msx.philox4x32_10(ctr, key)) == want
 
 
@@ -27,7 +29,8 @@ def _atoms(n, Z=38):
     return a
 
 
-def test_displacement_moments():
+@BOTH_SUITES
+def test_displacement_moments(suite):
     n = 200000
     at = _atoms(n)
     d = msx.displace(at, {38: 0.088}, seed=7, config=3)
@@ -39,7 +42,8 @@ def test_displacement_moments():
     assert abs((dx[:, 2] ** 4).mean() / 0.088 ** 4 / 3 - 1) < 0.03
 
 
-def test_displacement_reproducible_and_per_element():
+@BOTH_SUITES
+def test_displacement_reproducible_and_per_element(suite):
     at, _ = specimen.srtio3_001((3, 3, 2))
     a = msx.displace(at, specimen.SRTIO3_U, seed=11, config=0)
     b = msx.displace(at, specimen.SRTIO3_U, seed=11, config=0)

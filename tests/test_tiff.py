@@ -2,7 +2,9 @@
 replacing floatTIFF::write / read of floatTIFF.cpp): the reference's own
 floatTIFF, compiled into oracle/_ref, pins the format -- files written by both
 agree byte for byte except the 20-byte date, and each side reads the other's
-files -- plus a committed file written by the reference.  CPU only."""
+files -- plus a committed file written by the reference.  Host code of
+libmsx.so: the tests that need no compiled reference run in both the CPU and
+the GPU suite."""
 import ctypes as C
 import json
 import os
@@ -11,6 +13,7 @@ import numpy as np
 import pytest
 
 import multislice_amd as msx
+from conftest import BOTH_SUITES
 from oracle import ref
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "tiff")
@@ -83,7 +86,8 @@ def test_bytes_match_reference_writer(tmp_path, ny, nx, npix, dx, dy):
     assert gp[0] == npix and np.array_equal(gp[1:], prm[1:])
 
 
-def test_reads_committed_reference_file():
+@BOTH_SUITES
+def test_reads_committed_reference_file(suite):
     """A 12 x 5 complex layer written by the reference's floatTIFF (date
     included), with its expected contents."""
     with open(os.path.join(GOLD, "ref_12x5_complex.json")) as f:
@@ -96,7 +100,8 @@ def test_reads_committed_reference_file():
     assert img.shape == (5, 12) and np.array_equal(img.real, pix[:, :12]) and np.array_equal(img.imag, pix[:, 12:])
 
 
-def test_python_round_trip_and_errors(tmp_path):
+@BOTH_SUITES
+def test_python_round_trip_and_errors(tmp_path, suite):
     rng = np.random.default_rng(1)
     z = (rng.standard_normal((40, 24)) + 1j * rng.standard_normal((40, 24))).astype(np.complex64)
     msx.tiff_write(tmp_path / "w.tif", z, params=np.arange(512, dtype=np.float32), dx=0.1, dy=0.1)
