@@ -109,23 +109,25 @@ def copy_probe():
 
 def tile_probe():
     """Bandwidth ceilings of the access patterns (msx_copy_probe): a contiguous
-    copy and copies with the column sweep's pattern (8- and 16-column tiles)
-    of 128 images of 512 x 512 complex64 (256 MiB read, 256 MiB written)."""
+    copy and copies with the column sweep's pattern (8- and 16-column tiles,
+    and 8-column tiles of images stored in 8-row bands) of 128 images of
+    512 x 512 complex64 (256 MiB read, 256 MiB written)."""
     import torch
     import multislice_amd as msx
     ctx = msx.Context(0)
     nimg, n = 128, 512
-    src = torch.ones((nimg, n, n), dtype=torch.complex64, device="cuda")
+    src = torch.randn((nimg, n, n), dtype=torch.complex64, device="cuda")  # distinct values: a copy in the wrong place shows
     dst = torch.empty_like(src)
     bytes_ = 2 * src.numel() * 8
     out = {}
-    for mode in (0, 8, 16):
+    for mode in (0, 8, 16, 1008):
+        dst.zero_()
         ctx.copy_probe(mode, src, dst, nimg, n, n)
         ms = sorted(ctx.copy_probe(mode, src, dst, nimg, n, n) for _ in range(10))
         med = ms[len(ms) // 2]
-        out[{0: "contiguous", 8: "tile8", 16: "tile16"}[mode]] = {"ms_median": round(med, 4),
-                                                                  "GBps": round(bytes_ / med / 1e6, 1)}
-    assert bool((dst == src).all())
+        out[{0: "contiguous", 8: "tile8", 16: "tile16", 1008: "band8"}[mode]] = {
+            "ms_median": round(med, 4), "GBps": round(bytes_ / med / 1e6, 1)}
+        assert bool((dst == src).all()), mode
     ctx.close()
     print(json.dumps({"copy_bytes_per_launch": bytes_, "images": nimg, "grid": n, **out}))
 

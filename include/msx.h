@@ -96,11 +96,17 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
  * the built-in one or its alternative for this line length) and the STEM
  * batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
  * ms_per_slice is the time of one wave through one slice at that batch.
+ * At 512 x 512 the tuner also times msx_stem's waves stored in 8-row bands
+ * (band below) against the image layout and keeps the faster.
  * msx_set_tuning installs a saved result (NULL clears it). */
 typedef struct msx_tuning {
     int32_t row_wpb, col_wpb, batch;
     double ms_per_slice;
     int32_t col_tile;
+    /* layout of msx_stem's wave batches: 0 = the image layout, 1 = 8-row
+     * bands (element (ix, iy) at ((ix >> 3) ny + iy) 8 + (ix & 7); 512 x 512
+     * only), whose sweeps read and write contiguous 512-byte chunks */
+    int32_t band;
 } msx_tuning;
 MSX_API int msx_autotune(msx_ctx ctx, int nx, int ny, int prec, int max_batch, msx_tuning* best);
 MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuning* t);
@@ -116,10 +122,14 @@ enum { MSX_PROF_ROW = 0, MSX_PROF_COL = 1, MSX_PROF_SPLAT = 2, MSX_PROF_DET = 3,
 /* Bandwidth probe (no counterpart in the reference; measurement only): one
  * launch copying device buffers src -> dst of nimg nx x ny complex64 images,
  * timed with HIP events on the context stream (*ms).  mode 0: a contiguous
- * copy (nimg*nx*ny a multiple of 4096, 16-byte aligned buffers); mode 8 or 16 (nx = 512, ny a multiple of it): the column sweep's
- * access pattern (that many adjacent columns per work-group, 16 strided rows
- * per thread, XCD order) without the transforms -- the ceiling of that
- * pattern for the roofline of the column sweep. */
+ * copy (nimg*nx*ny a multiple of 4096, 16-byte aligned buffers); mode 8 or 16
+ * (nx = 512, ny a multiple of it): the column sweep's access pattern (that
+ * many adjacent columns per work-group, 16 strided rows per thread, XCD
+ * order) without the transforms -- the ceiling of that pattern for the
+ * roofline of the column sweep; mode 1008 (nx = 512, ny a multiple of 8): the
+ * same 8-column pattern on images stored in 8-row bands (element (ix, iy) at
+ * ((ix >> 3) * ny + iy) * 8 + (ix & 7)), each instruction one contiguous
+ * 512-byte chunk. */
 MSX_API int msx_copy_probe(msx_ctx ctx, int mode, const void* src, void* dst, int nimg, int nx, int ny,
                            double* ms);
 MSX_API int msx_profile(msx_ctx ctx, int enable);
@@ -328,6 +338,16 @@ typedef struct msx_stack {
     const double* cz;        /* host array [nlayer]: layer thickness (Angstrom) */
     int32_t nslice;          /* number of slices */
     const int32_t* layer;    /* host array [nslice]: layer index of each slice */
+    /* angle between the cell axes a (x) and b (rad); 0 or pi/2: an orthogonal
+     * cell.  Otherwise (an oblique cell: the slice loop of msx_multislice;
+     * the other entry points refuse it) the propagator is Musli's
+     * (Propagator.cpp:55-107, in this engine's sign convention):
+     *   P = exp(-i pi dz (lambda s - 2 (x tan(tilt_x) + y tan(tilt_y))))
+     * with x = fx/(ax sin gam), y = fy/(by sin gam) for the frequency indices
+     * fx, fy of freqn, s = x^2 + y^2 - 2 x y cos(gam), all in FP64, and P = 0
+     * where s >= (bw min(nx/(2 ax), ny/(2 by)))^2 (bw = 1: Musli's full
+     * Nyquist disk). */
+    double gam;
 } msx_stack;
 
 /* ---- multislice (HRTEM / any incident waves) ------------------------------ */
@@ -342,6 +362,7 @@ typedef struct msx_stack {
  * wave) and psi is output only. */
 MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
                            void* psi, int nwave, int prec, int where);
+
 
 /* Many independent HRTEM jobs of uneven sizes and thicknesses in as few
  * launches as possible (the reference ran one job per process,

@@ -74,14 +74,40 @@ struct DevBuf {
     }
 };
 
-// radix-independent twiddle table exp(+2 pi i m / n), m < n, from FP64
+// radix-independent twiddle table exp(+2 pi i m / n), m < n, from FP64.
+// FP32 entries: among the float neighbours of the rounded (cos, sin) the pair
+// whose c^2 + s^2 is closest to 1 (then the nearest to the exact value).
+// Rounding cos and sin separately leaves |w|^2 - 1 of up to ~1e-7 with a
+// fixed sign per entry (and a mean of -1.6e-8 over the 64th roots that the
+// middle pass of a 512-point transform uses); the slice loop applies the same
+// twiddles to the same strong beams every slice, so the wave's norm would
+// drift by ~1e-5 over 100 slices.  The chosen pairs stay within 1.1e-7 rad
+// of the exact angle.
 template <typename C> std::vector<C> make_twiddles(int n) {
+    using Re = decltype(C{}.x);
     std::vector<C> w(n);
     for (int m = 0; m < n; m++) {
-        // angle reduced to [0, pi/4] by symmetry before the libm call
         const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
-        w[m].x = (decltype(w[m].x))std::cos((double)a);
-        w[m].y = (decltype(w[m].y))std::sin((double)a);
+        const double c = std::cos((double)a), s = std::sin((double)a);
+        w[m].x = (Re)c;
+        w[m].y = (Re)s;
+        if (sizeof(Re) == 4) {
+            double best_n = 1e30, best_d = 1e30;
+            for (int dc = -1; dc <= 1; dc++)
+                for (int ds = -1; ds <= 1; ds++) {
+                    float fc = (float)c, fs = (float)s;
+                    if (dc) fc = std::nextafter(fc, dc > 0 ? 2.0f : -2.0f);
+                    if (ds) fs = std::nextafter(fs, ds > 0 ? 2.0f : -2.0f);
+                    const double nerr = std::fabs((double)fc * fc + (double)fs * fs - 1.0);
+                    const double derr = std::hypot((double)fc - c, (double)fs - s);
+                    if (nerr < best_n || (nerr == best_n && derr < best_d)) {
+                        best_n = nerr;
+                        best_d = derr;
+                        w[m].x = (Re)fc;
+                        w[m].y = (Re)fs;
+                    }
+                }
+        }
     }
     return w;
 }
@@ -96,6 +122,12 @@ std::vector<float> k2_table(int n, double a) {
     }
     return k2;
 }
+// An oblique cell: gam set and not a right angle (to 1e-12 rad).
+bool is_oblique(const msx_stack* st) { return st->gam != 0.0 && std::fabs(st->gam - M_PI / 2) > 1e-12; }
+
+// grids whose STEM waves may be stored in 8-row bands (the band variants of
+// the sweeps are built for 512-point lines)
+bool band_layout_ok(int nx, int ny) { return nx == 512 && ny == 512; }
 int fidx(int i, int n) {
     int imid = (int)(n / 2.0 + 0.5);
     return i > imid ? i - n : i;
@@ -270,18 +302,33 @@ int check_prec_where(int prec, int where) {
 // Per-call device tables for the slice loop: freqn k^2 tables and propagator
 // factors per layer, packed in ctx->tabs.
 template <typename C> struct SliceTables {
-    const float* kx2;
-    const float* ky2;
-    const C* px;  // [nlayer][nx]
-    const C* py;  // [nlayer][ny]
-    float k2max;
+    const float* kx2 = nullptr;
+    const float* ky2 = nullptr;
+    const C* px = nullptr;  // [nlayer][nx]
+    const C* py = nullptr;  // [nlayer][ny]
+    float k2max = 0.f;
+    // twiddles of the x and y transforms; null: the context's cached tables
+    const C* twx = nullptr;
+    const C* twy = nullptr;
 };
 
-template <typename C>
-int build_tables(msx_ctx ctx, const msx_stack* st, bool with_prop, SliceTables<C>* out) {
+// Host images of the slice tables: freqn k^2 tables (float, the reference's
+// arithmetic) and the per-layer propagator factors, with their byte offsets
+// in one 256-byte-aligned device block.
+template <typename C> struct HostTables {
+    std::vector<float> kx2, ky2;
+    std::vector<C> px, py;
+    size_t o_ky = 0, o_px = 0, o_py = 0, tot = 0;
+};
+
+inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+template <typename C> void host_tables(const msx_stack* st, bool with_prop, HostTables<C>* h) {
     const int nx = st->nx, ny = st->ny, nl = with_prop ? st->nlayer : 0;
-    std::vector<float> kx2 = k2_table(nx, st->ax), ky2 = k2_table(ny, st->by);
-    std::vector<C> px((size_t)nl * nx), py((size_t)nl * ny);
+    h->kx2 = k2_table(nx, st->ax);
+    h->ky2 = k2_table(ny, st->by);
+    h->px.assign((size_t)nl * nx, C{});
+    h->py.assign((size_t)nl * ny, C{});
     const double lam = msx_wavelength(st->kev);
     const double tx = 2.0 * std::tan(st->ctilt_x), ty = 2.0 * std::tan(st->ctilt_y);
     for (int l = 0; l < nl; l++) {
@@ -291,36 +338,52 @@ int build_tables(msx_ctx ctx, const msx_stack* st, bool with_prop, SliceTables<C
         for (int i = 0; i < nx; i++) {
             const double k = (double)fidx(i, nx) / st->ax;
             const double t = M_PI * cz * (k * k * lam - k * tx);
-            px[(size_t)l * nx + i].x = std::cos(t);
-            px[(size_t)l * nx + i].y = -std::sin(t);
+            h->px[(size_t)l * nx + i].x = std::cos(t);
+            h->px[(size_t)l * nx + i].y = -std::sin(t);
         }
         for (int i = 0; i < ny; i++) {
             const double k = (double)fidx(i, ny) / st->by;
             const double t = M_PI * cz * (k * k * lam - k * ty);
-            py[(size_t)l * ny + i].x = std::cos(t);
-            py[(size_t)l * ny + i].y = -std::sin(t);
+            h->py[(size_t)l * ny + i].x = std::cos(t);
+            h->py[(size_t)l * ny + i].y = -std::sin(t);
         }
     }
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t o_ky = al(sizeof(float) * nx), o_px = o_ky + al(sizeof(float) * ny),
-                 o_py = o_px + al(sizeof(C) * px.size()), tot = o_py + al(sizeof(C) * py.size() + 1);
-    CHK(ctx->tabs.ensure(tot));
-    char* base = static_cast<char*>(ctx->tabs.p);
-    HIPCHK(hipMemcpyAsync(base, kx2.data(), sizeof(float) * nx, hipMemcpyHostToDevice, ctx->stream));
-    HIPCHK(hipMemcpyAsync(base + o_ky, ky2.data(), sizeof(float) * ny, hipMemcpyHostToDevice, ctx->stream));
-    if (nl) {
-        HIPCHK(hipMemcpyAsync(base + o_px, px.data(), sizeof(C) * px.size(), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(base + o_py, py.data(), sizeof(C) * py.size(), hipMemcpyHostToDevice, ctx->stream));
+    h->o_ky = al256(sizeof(float) * nx);
+    h->o_px = h->o_ky + al256(sizeof(float) * ny);
+    h->o_py = h->o_px + al256(sizeof(C) * h->px.size());
+    h->tot = h->o_py + al256(sizeof(C) * h->py.size() + 1);
+}
+
+// Copies the host tables to the device block at `base` (at least h.tot
+// bytes) on the context stream and waits for the copies.
+template <typename C>
+int upload_tables(msx_ctx ctx, const msx_stack* st, const HostTables<C>& h, char* base, SliceTables<C>* out) {
+    const int nx = st->nx, ny = st->ny;
+    HIPCHK(hipMemcpyAsync(base, h.kx2.data(), sizeof(float) * nx, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(base + h.o_ky, h.ky2.data(), sizeof(float) * ny, hipMemcpyHostToDevice, ctx->stream));
+    if (!h.px.empty()) {
+        HIPCHK(hipMemcpyAsync(base + h.o_px, h.px.data(), sizeof(C) * h.px.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+        HIPCHK(hipMemcpyAsync(base + h.o_py, h.py.data(), sizeof(C) * h.py.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
     }
     // the host vectors must outlive the async copies
     HIPCHK(hipStreamSynchronize(ctx->stream));
     out->kx2 = reinterpret_cast<const float*>(base);
-    out->ky2 = reinterpret_cast<const float*>(base + o_ky);
-    out->px = reinterpret_cast<const C*>(base + o_px);
-    out->py = reinterpret_cast<const C*>(base + o_py);
+    out->ky2 = reinterpret_cast<const float*>(base + h.o_ky);
+    out->px = reinterpret_cast<const C*>(base + h.o_px);
+    out->py = reinterpret_cast<const C*>(base + h.o_py);
     out->k2max = k2max_of(nx, ny, st->ax, st->by, st->bw);
-    ctx->band = band_fraction(ky2.data(), ny, out->k2max);
+    ctx->band = band_fraction(h.ky2.data(), ny, out->k2max);
     return MSX_OK;
+}
+
+template <typename C>
+int build_tables(msx_ctx ctx, const msx_stack* st, bool with_prop, SliceTables<C>* out) {
+    HostTables<C> h;
+    host_tables<C>(st, with_prop, &h);
+    CHK(ctx->tabs.ensure(h.tot));
+    return upload_tables<C>(ctx, st, h, static_cast<char*>(ctx->tabs.p), out);
 }
 
 template <typename C> RowArgs<C> row_args(C* psi, int nx, int nb, const C* tw) {
@@ -453,19 +516,27 @@ struct DetPlan {
 // with mixed_out, just before it: psi then holds IFFT_x(P FFT_x FFT_y(t psi))
 // of the last slice, i.e. FFT_y of the exit wave times nx, in the columns
 // ky2 < k2max (the others are not written).
+// band: the waves are stored in 8-row bands (the row and column sweeps'
+// band variants; start = 1 loops without the final inverse row sweep only).
 template <typename C>
 int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-                    const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false) {
+                    const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false,
+                    int band = 0) {
     using Re = typename real_of<C>::type;
     const int nx = st->nx, ny = st->ny;
     const long long img = (long long)nx * ny;
-    const C *twx, *twy;
-    CHK(ctx->twiddles<C>(nx, &twx));
-    CHK(ctx->twiddles<C>(ny, &twy));
+    const C *twx = tb.twx, *twy = tb.twy;
+    if (!twx) CHK(ctx->twiddles<C>(nx, &twx));
+    if (!twy) CHK(ctx->twiddles<C>(ny, &twy));
     const Re inv = (Re)1 / ((Re)nx * (Re)ny);
     const msx_tuning* tu = ctx->tuning(nx, ny, sizeof(Re) == 8 ? MSX_F64 : MSX_F32);
     const int tile = tu ? tu->col_tile : 0;
     const int groups = col_groups<C>(nx, ny, tile);
+    // oblique cell: the per-element FP64 propagator, no band pruning (the
+    // columns beyond the band are not all-zero columns of the oblique disk)
+    const bool obl = is_oblique(st);
+    const double sg = obl ? std::sin(st->gam) : 1.0, cg = obl ? std::cos(st->gam) : 0.0;
+    const double obl_k = std::min(nx / (2.0 * st->ax), ny / (2.0 * st->by)) * st->bw;
     int ick = 0;
     for (int s = 0; s < st->nslice; s++) {
         const int l = st->layer[s];
@@ -477,7 +548,8 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         r.trans = trans + (long long)l * img;
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
-        r.prune = (s == 0) ? 2 : 3;
+        r.prune = obl ? 0 : ((s == 0) ? 2 : 3);
+        r.band = band;
         if (tu) r.wpb = tu->row_wpb;
         if (s == 0 && start == 0) {
             r.src = src;
@@ -488,13 +560,26 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
             LAUNCH_ROW(ny, kRowSlice, r);
         }
         ColArgs<C> c = col_args<C>(psi, ny, nb, twx);
-        c.prune = 1;
+        c.prune = obl ? 0 : 1;
         c.kx2 = tb.kx2;
         c.ky2 = tb.ky2;
         c.k2max = tb.k2max;
         c.px = tb.px + (long long)l * nx;
         c.py = tb.py + (long long)l * ny;
         c.ops = kColProp | kColStore;
+        if (obl) {
+            // Propagator.cpp:65-72 with this slice's thickness
+            const double lam = msx_wavelength(st->kev), dz = st->cz[l];
+            c.ops |= kColObl;
+            c.obl_asg = st->ax * sg;
+            c.obl_bsg = st->by * sg;
+            c.obl_2c = 2.0 * cg;
+            c.obl_smax = obl_k * obl_k;
+            c.obl_pl = M_PI * lam * dz;
+            c.obl_tx = 2.0 * M_PI * dz * std::tan(st->ctilt_x);
+            c.obl_ty = 2.0 * M_PI * dz * std::tan(st->ctilt_y);
+        }
+        c.band = band;
         if (tu) c.wpb = tu->col_wpb;
         c.tile = tile;
         bool is_ck = det && ick < det->nck && det->ck[ick] == s + 1;
@@ -535,7 +620,7 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         r.scale = inv;
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
-        r.prune = 1;
+        r.prune = obl ? 0 : 1;
         LAUNCH_ROW(ny, kRowInv, r);
     }
     return MSX_OK;
@@ -548,12 +633,12 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
 // that fails falls back to launching the loop directly.
 template <typename C>
 int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-               const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false) {
+               const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false, int band = 0) {
     if (!ctx->graphs || ctx->prof)
-        return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
+        return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
     ctx->gkey.clear();
     ctx->keying = true;
-    const int rk = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
+    const int rk = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
     ctx->keying = false;
     CHK(rk);
     auto it = ctx->gcache.find(ctx->gkey);
@@ -562,9 +647,9 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             (void)hipGetLastError();  // e.g. the legacy default stream: launch directly
-            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
         }
-        const int rc = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
+        const int rc = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         hipGraphExec_t ex = nullptr;
         hipError_t ei = (rc == MSX_OK && ec == hipSuccess && g) ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0)
@@ -572,7 +657,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         if (g) hipGraphDestroy(g);
         if (ei != hipSuccess) {
             (void)hipGetLastError();
-            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
         }
         it = ctx->gcache.emplace(ctx->gkey, ex).first;
         ctx->graph_captures++;
@@ -630,14 +715,16 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     auto key = std::make_tuple(nx, ny, prec);
     const bool had = ctx->tune.count(key) > 0;
     const msx_tuning saved = had ? ctx->tune[key] : msx_tuning{};
-    // times (ms per launch) of the row and column sweeps for one shape
-    auto trial = [&](int rw, int cw, int tile, int nb, double* row_ms, double* col_ms) -> int {
-        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0, tile};
+    // times (ms per launch) of the row and column sweeps for one shape, on
+    // the STEM loop's sequence (waves start in k-space, no final inverse row
+    // sweep), with the waves in the image layout or in 8-row bands
+    auto trial = [&](int rw, int cw, int tile, int nb, int band, double* row_ms, double* col_ms) -> int {
+        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0, tile, band};
         ctx->prof = true;
         for (int rep = 0; rep < 2; rep++) {  // the first pass warms up
             for (int k = 0; k < MSX_PROF_KINDS; k++) ctx->prof_n[k] = 0, ctx->prof_ms[k] = 0, ctx->prof_bytes[k] = 0;
             CHK(slice_loop_body<C>(ctx, &st, tb, static_cast<const C*>(ctx->trans.p), static_cast<C*>(ctx->wave.p),
-                                   nb, nullptr, 0, 0, nullptr));
+                                   nb, nullptr, 0, 1, nullptr, true, band));
         }
         *row_ms = ctx->prof_ms[MSX_PROF_ROW] / std::max(1LL, ctx->prof_n[MSX_PROF_ROW]);
         *col_ms = ctx->prof_ms[MSX_PROF_COL] / std::max(1LL, ctx->prof_n[MSX_PROF_COL]);
@@ -645,38 +732,44 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     };
     int tdef, talt;
     col_tiles<C>(nx, &tdef, &talt);
-    std::vector<int> tiles{tdef};
-    if (talt != tdef) tiles.push_back(talt);
+    const int nlayout = band_layout_ok(nx, ny) ? 2 : 1;
     int rc = MSX_OK;
-    msx_tuning best{1, 1, 0, 0.0, 0};
+    msx_tuning best{1, 1, 0, 0.0, 0, 0};
     do {
-        // per batch size: the fastest waves-per-work-group of the row sweep,
-        // the fastest (tile, waves-per-work-group) of the column sweep, and
-        // the resulting probe-slice rate
-        double rate[kNB] = {0}, top = 0.0;
-        int rw_of[kNB], cw_of[kNB], tile_of[kNB];
-        for (int i = 0; i < kNB; i++) rw_of[i] = cw_of[i] = 1, tile_of[i] = tdef;
-        for (int i = 0; i < kNB && rc == MSX_OK; i++) {
-            if (batches[i] > bmax) break;
-            double brow = 1e30, bcol = 1e30;
-            for (int tile : tiles)
-                for (int w : {1, 2, 4, 8, -1, -2}) {
-                    // w < 0: the column sweep's balanced schedule (the row
-                    // sweep is timed at positive w only)
-                    double rm, cm;
-                    if ((rc = trial(w > 0 ? w : 8, w, tile, batches[i], &rm, &cm)) != MSX_OK) break;
-                    if (tile == tdef && w > 0 && rm < brow) brow = rm, rw_of[i] = w;
-                    if (cm < bcol) bcol = cm, cw_of[i] = w, tile_of[i] = tile;
-                }
-            rate[i] = batches[i] / (brow + bcol);
-            top = std::max(top, rate[i]);
+        // per layout and batch size: the fastest waves-per-work-group of the
+        // row sweep, the fastest (tile, waves-per-work-group) of the column
+        // sweep, and the resulting probe-slice rate
+        double rate[2][kNB] = {}, top = 0.0;
+        int rw_of[2][kNB], cw_of[2][kNB], tile_of[2][kNB];
+        for (int L = 0; L < 2; L++)
+            for (int i = 0; i < kNB; i++) rw_of[L][i] = cw_of[L][i] = 1, tile_of[L][i] = tdef;
+        for (int L = 0; L < nlayout && rc == MSX_OK; L++) {
+            std::vector<int> tiles{tdef};
+            if (L == 0 && talt != tdef) tiles.push_back(talt);  // the band layout takes the built-in 8 columns
+            for (int i = 0; i < kNB && rc == MSX_OK; i++) {
+                if (batches[i] > bmax) break;
+                double brow = 1e30, bcol = 1e30;
+                for (int tile : tiles)
+                    for (int w : {1, 2, 4, 8, -1, -2}) {
+                        // w < 0: the column sweep's balanced schedule (the row
+                        // sweep is timed at positive w only)
+                        double rm, cm;
+                        if ((rc = trial(w > 0 ? w : 8, w, tile, batches[i], L, &rm, &cm)) != MSX_OK) break;
+                        if (tile == tdef && w > 0 && rm < brow) brow = rm, rw_of[L][i] = w;
+                        if (cm < bcol) bcol = cm, cw_of[L][i] = w, tile_of[L][i] = tile;
+                    }
+                rate[L][i] = batches[i] / (brow + bcol);
+                top = std::max(top, rate[L][i]);
+            }
         }
         if (rc != MSX_OK) break;
-        for (int i = 0; i < kNB && batches[i] <= bmax; i++)
-            if (rate[i] >= 0.97 * top) {
-                best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i], tile_of[i]};
+        for (int i = 0; i < kNB && batches[i] <= bmax; i++) {
+            const int L = (nlayout > 1 && rate[1][i] > rate[0][i]) ? 1 : 0;
+            if (rate[L][i] >= 0.97 * top) {
+                best = msx_tuning{rw_of[L][i], cw_of[L][i], batches[i], 1.0 / rate[L][i], tile_of[L][i], L};
                 break;
             }
+        }
     } while (0);
     ctx->prof = prof0;
     ctx->graphs = graphs0;
@@ -857,6 +950,8 @@ template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs
 
 int check_stack(const msx_stack* st) {
     if (!st) return fail(MSX_EINVAL, "stack is NULL");
+    if (!(st->gam == 0.0 || (st->gam > 0.0 && st->gam < M_PI)))
+        return fail(MSX_EINVAL, "gam = %g: the cell angle must be 0 (orthogonal) or in (0, pi)", st->gam);
     CHK(check_grid(st->nx, st->ny));
     if (!(st->ax > 0) || !(st->by > 0) || !(st->kev > 0)) return fail(MSX_EINVAL, "ax, by, kev must be > 0");
     if (st->nlayer <= 0 || !st->cz) return fail(MSX_EINVAL, "nlayer must be > 0 with cz set");
@@ -950,6 +1045,8 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
     if (nap == 0) return fail(MSX_EINVAL, "probe aperture contains no pixel");
     const C *twx;
     CHK(ctx->twiddles<C>(nx, &twx));
+    // waves in 8-row bands where the tuning chose them (512 x 512 only)
+    const int band = (tu && tu->band && band_layout_ok(nx, ny)) ? 1 : 0;
 
     DetPlan dp;
     dp.ndet = ndet;
@@ -984,8 +1081,9 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
         g.kap_lo = kap_lo;
         g.kap_hi = kap_hi;
         g.amp = (Re)std::sqrt((double)img / (double)nap);
+        g.band = band;
         LAUNCH_COL(nx, kColGenInv, g);
-        CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nb, nullptr, 0, 1, &dp));
+        CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nb, nullptr, 0, 1, &dp, false, band));
         // this batch's [nck][ndet][nb] block into columns p0.. of the results
         HIPCHK(hipMemcpy2DAsync(o_dev + p0, sizeof(double) * nprobe, s_dev, sizeof(double) * batch,
                                 sizeof(double) * nb, (size_t)nck * ndet, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1749,8 +1847,8 @@ MSX_API int msx_copy_probe(msx_ctx ctx, int mode, const void* src, void* dst, in
                            double* ms) {
     if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
     if (!src || !dst || !ms || nimg < 1 || nx < 1 || ny < 1) return fail(MSX_EINVAL, "copy probe: bad arguments");
-    if (mode != 0 && !(nx == 512 && (mode == 8 || mode == 16) && ny % mode == 0))
-        return fail(MSX_EINVAL, "copy probe: mode 0, or 8 / 16 with nx = 512 and ny a multiple of the mode");
+    if (mode != 0 && !(nx == 512 && (mode == 8 || mode == 16 || mode == 1008) && ny % (mode % 1000) == 0))
+        return fail(MSX_EINVAL, "copy probe: mode 0, or 8 / 16 / 1008 with nx = 512 and ny a multiple of 8 / 16 / 8");
     const long long n = (long long)nimg * nx * ny;
     if (mode == 0 && (n % 4096 || (uintptr_t)src % 16 || (uintptr_t)dst % 16))
         return fail(MSX_EINVAL, "copy probe: mode 0 needs a multiple of 4096 values and 16-byte aligned buffers");
@@ -1789,8 +1887,13 @@ MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuni
         ctx->tune.erase(std::make_tuple(nx, ny, prec));
         return MSX_OK;
     }
-    if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < -8 || t->col_wpb > 64 || t->batch < 0)
+    if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < -8 || t->col_wpb > 64 || t->batch < 0 || t->band < 0 ||
+        t->band > 1)
         return fail(MSX_EINVAL, "bad tuning values");
+    if (t->band && !band_layout_ok(nx, ny))
+        return fail(MSX_EINVAL, "the band layout is built for 512 x 512 grids only");
+    if (t->band && t->col_tile != 0 && t->col_tile != 8)
+        return fail(MSX_EINVAL, "the band layout takes the 8-column tile");
     int tdef, talt;
     if (prec == MSX_F64) col_tiles<double2>(nx, &tdef, &talt);
     else col_tiles<float2>(nx, &tdef, &talt);
@@ -2094,8 +2197,11 @@ MSX_API int msx_multislice_batch(msx_ctx ctx, const msx_job* jobs, int njobs, in
     CHK(check_prec_where(prec, where));
     if (where != MSX_DEVICE) return fail(MSX_EINVAL, "msx_multislice_batch takes device buffers (MSX_DEVICE)");
     if (njobs < 0 || (njobs > 0 && !jobs)) return fail(MSX_EINVAL, "bad jobs");
-    for (int i = 0; i < njobs; i++)
+    for (int i = 0; i < njobs; i++) {
         if (!jobs[i].stack || !jobs[i].trans || !jobs[i].psi) return fail(MSX_EINVAL, "job %d: stack, trans, psi required", i);
+        if (is_oblique(jobs[i].stack))
+            return fail(MSX_EINVAL, "job %d: oblique cell (gam != pi/2): run it through msx_multislice", i);
+    }
     HIPCHK(hipSetDevice(ctx->device));
     if (prec == MSX_F64) return multislice_batch_impl<double2>(ctx, jobs, njobs);
     return multislice_batch_impl<float2>(ctx, jobs, njobs);
@@ -2125,6 +2231,9 @@ MSX_API int msx_stem(msx_ctx ctx, const msx_stack* stack, const void* trans, con
         if (ck[i] < 1 || ck[i] > stack->nslice || (i && ck[i] <= ck[i - 1]))
             return fail(MSX_EINVAL, "checkpoints must increase within 1..nslice");
     if (!(probe->alpha_max > 0)) return fail(MSX_EINVAL, "alpha_max must be > 0");
+    if (is_oblique(stack))
+        return fail(MSX_EINVAL, "msx_stem: oblique cells (gam != pi/2) are not supported (the probe and the detectors "
+                                "are built on an orthogonal cell)");
     HIPCHK(hipSetDevice(ctx->device));
     if (prec == MSX_F64)
         return stem_impl<double2>(ctx, stack, trans, probe, pos, nprobe, det, ndet, ck, nck, weight, accumulate, out,
@@ -2147,6 +2256,7 @@ MSX_API int msx_partial_coherence(msx_ctx ctx, const msx_stack* stack, const voi
     if (sigmaf > 0 && !(dfdelt > 0)) return fail(MSX_EINVAL, "dfdelt must be > 0 when sigmaf > 0");
     if (sigmaf > 0 && (2.5 * sigmaf / dfdelt) > 4096) return fail(MSX_EINVAL, "more than 8193 defocus steps");
     if (batch < 0) return fail(MSX_EINVAL, "batch must be >= 0");
+    if (is_oblique(stack)) return fail(MSX_EINVAL, "msx_partial_coherence: oblique cells are not supported");
     HIPCHK(hipSetDevice(ctx->device));
     if (prec == MSX_F64)
         return partial_impl<double2>(ctx, stack, trans, lens, acmin, acmax, sigmaf, dfdelt, batch, image, nangles,

@@ -80,12 +80,56 @@ struct Roots48 {
     static constexpr double s(int j) { return c[(j + 36) % 48]; }  // sin(x) = cos(x - pi/2)
 };
 
+// The same roots as float pairs chosen for their modulus: for each angle the
+// (cos, sin) pair among the float neighbours of the rounded values whose
+// c^2 + s^2 is closest to 1 (then the nearest to the exact root).  Rounding
+// cos and sin separately leaves |root|^2 - 1 of up to ~1e-7 with a sign fixed
+// per root; the FP32 slice loop applies the same roots to the same strong
+// beams every slice, so those errors add up as a drift of the wave's norm
+// (about 1e-5 over 100 slices).  These pairs keep |root|^2 - 1 below 4.3e-8
+// and their angles within 8.6e-8 rad.
+struct RootsF48 {
+    static constexpr float cs[48][2] = {
+        {1.000000000e+00f, 0.000000000e+00f}, {9.914448857e-01f, 1.305261701e-01f}, {9.659258127e-01f, 2.588190734e-01f},
+        {9.238795638e-01f, 3.826833963e-01f}, {8.660253882e-01f, 5.000000000e-01f}, {7.933533788e-01f, 6.087613702e-01f},
+        {7.071068287e-01f, 7.071067095e-01f}, {6.087613702e-01f, 7.933533788e-01f}, {5.000000000e-01f, 8.660253882e-01f},
+        {3.826833963e-01f, 9.238795638e-01f}, {2.588190734e-01f, 9.659258127e-01f}, {1.305261701e-01f, 9.914448857e-01f},
+        {0.000000000e+00f, 1.000000000e+00f}, {-1.305261701e-01f, 9.914448857e-01f}, {-2.588190734e-01f, 9.659258127e-01f},
+        {-3.826833963e-01f, 9.238795638e-01f}, {-5.000000000e-01f, 8.660253882e-01f}, {-6.087613702e-01f, 7.933533788e-01f},
+        {-7.071067095e-01f, 7.071068287e-01f}, {-7.933533788e-01f, 6.087613702e-01f}, {-8.660253882e-01f, 5.000000000e-01f},
+        {-9.238795638e-01f, 3.826833963e-01f}, {-9.659258127e-01f, 2.588190734e-01f}, {-9.914448857e-01f, 1.305261701e-01f},
+        {-1.000000000e+00f, 0.000000000e+00f}, {-9.914448857e-01f, -1.305261701e-01f}, {-9.659258127e-01f, -2.588190734e-01f},
+        {-9.238795638e-01f, -3.826833963e-01f}, {-8.660253882e-01f, -5.000000000e-01f}, {-7.933533788e-01f, -6.087613702e-01f},
+        {-7.071068287e-01f, -7.071067095e-01f}, {-6.087613702e-01f, -7.933533788e-01f}, {-5.000000000e-01f, -8.660253882e-01f},
+        {-3.826833963e-01f, -9.238795638e-01f}, {-2.588190734e-01f, -9.659258127e-01f}, {-1.305261701e-01f, -9.914448857e-01f},
+        {0.000000000e+00f, -1.000000000e+00f}, {1.305261701e-01f, -9.914448857e-01f}, {2.588190734e-01f, -9.659258127e-01f},
+        {3.826833963e-01f, -9.238795638e-01f}, {5.000000000e-01f, -8.660253882e-01f}, {6.087613702e-01f, -7.933533788e-01f},
+        {7.071067095e-01f, -7.071068287e-01f}, {7.933533788e-01f, -6.087613702e-01f}, {8.660253882e-01f, -5.000000000e-01f},
+        {9.238795638e-01f, -3.826833963e-01f}, {9.659258127e-01f, -2.588190734e-01f}, {9.914448857e-01f, -1.305261701e-01f}};
+};
+
 // omega_R^m = exp(S*2*pi*i*m/R) for R dividing 48
 template <int R, int S, typename C> __device__ __forceinline__ C root(int m) {
     static_assert(48 % R == 0, "radix must divide 48");
     using Re = typename real_of<C>::type;
     const int j = ((m % R) * (48 / R)) % 48;
-    return mk<C>((Re)Roots48::c[j], (Re)(S * Roots48::s(j)));
+    if constexpr (sizeof(Re) == 4) return mk<C>(RootsF48::cs[j][0], (Re)S * RootsF48::cs[j][1]);
+    else return mk<C>((Re)Roots48::c[j], (Re)(S * Roots48::s(j)));
+}
+
+// v * k for an irrational constant k (1/sqrt2, sqrt3/2) given as its FP64
+// value: in FP32, k = k_hi + k_lo with k_hi the float nearest to k, computed
+// as fma(k_lo, v, k_hi v).  The float constant alone scales |v|^2 by
+// (k_hi / k)^2 (for 1/sqrt2: 1 - 3.4e-8), the same loss on every butterfly of
+// every pass, which compounds over the slices of the loop; the compensated
+// product is exact to one rounding.
+template <typename Re> __device__ __forceinline__ Re mul_const(Re v, double k) {
+    if constexpr (sizeof(Re) == 4) {
+        const float hi = (float)k, lo = (float)(k - (double)hi);
+        return __builtin_fmaf(lo, v, hi * v);
+    } else {
+        return (Re)k * v;
+    }
 }
 // x * omega_R^m with the trivial roots (+-1, +-i) done as moves/negations and
 // the odd multiples of pi/4 as two adds and two multiplies (IEEE rules forbid
@@ -94,16 +138,17 @@ template <int R, int S, typename C> __device__ __forceinline__ C mul_root(C x, i
     using Re = typename real_of<C>::type;
     const int j = ((m % R) * (48 / R)) % 48;  // angle in units of 2*pi/48
     const int jj = S > 0 ? j : (48 - j) % 48;
-    const Re h = (Re)0.7071067811865475244008444;
+    constexpr double h = 0.7071067811865475244008444;
+    auto H = [](Re v) { return mul_const(v, h); };
     switch (jj) {
         case 0: return x;
         case 12: return mk<C>(-x.y, x.x);    // * i
         case 24: return mk<C>(-x.x, -x.y);   // * -1
         case 36: return mk<C>(x.y, -x.x);    // * -i
-        case 6: return mk<C>(h * (x.x - x.y), h * (x.x + x.y));    // * (1+i)/sqrt2
-        case 18: return mk<C>(-h * (x.x + x.y), h * (x.x - x.y));  // * (-1+i)/sqrt2
-        case 30: return mk<C>(h * (x.y - x.x), -h * (x.x + x.y));  // * (-1-i)/sqrt2
-        case 42: return mk<C>(h * (x.x + x.y), h * (x.y - x.x));   // * (1-i)/sqrt2
+        case 6: return mk<C>(H(x.x - x.y), H(x.x + x.y));    // * (1+i)/sqrt2
+        case 18: return mk<C>(-H(x.x + x.y), H(x.x - x.y));  // * (-1+i)/sqrt2
+        case 30: return mk<C>(H(x.y - x.x), -H(x.x + x.y));  // * (-1-i)/sqrt2
+        case 42: return mk<C>(H(x.x + x.y), H(x.y - x.x));   // * (1-i)/sqrt2
         default: return cmul(x, root<R, S, C>(m));
     }
 }
@@ -125,11 +170,11 @@ template <int S, typename C> struct Dft<2, S, C> {
 template <int S, typename C> struct Dft<3, S, C> {
     static __device__ __forceinline__ void run(C* a) {
         using Re = typename real_of<C>::type;
-        const Re h = (Re)0.8660254037844386467637232;  // sqrt(3)/2
+        constexpr double h = 0.8660254037844386467637232;  // sqrt(3)/2
         C s = cadd(a[1], a[2]);
         C d = csub(a[1], a[2]);
         C m = mk<C>(a[0].x - (Re)0.5 * s.x, a[0].y - (Re)0.5 * s.y);
-        C r = mul_si<S>(cscale(d, h));
+        C r = mul_si<S>(mk<C>(mul_const(d.x, h), mul_const(d.y, h)));
         a[0] = cadd(a[0], s);
         a[1] = cadd(m, r);
         a[2] = csub(m, r);

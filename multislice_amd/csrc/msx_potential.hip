@@ -438,11 +438,29 @@ __global__ __launch_bounds__(32 * TW) void tile_copy_kernel(const float2* __rest
 #pragma unroll
     for (int e = 0; e < 16; ++e) dst[base + (long long)(t + 32 * e) * ny] = v[e];
 }
+// The column-sweep pattern on images stored in 8-row bands (element (ix, iy)
+// at ((ix >> 3) * ny + iy) * 8 + (ix & 7)): a work-group takes 8 adjacent
+// columns of 512 rows, wave w and lane l hold column l / 8 and rows
+// 8 w + l % 8 + 32 e, so each load or store instruction covers one
+// contiguous 512-byte chunk (8 columns x 8 rows of one band).
+__global__ __launch_bounds__(256) void band_copy_kernel(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                       int ny, int groups) {
+    const int bid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int g = bid % groups, b = bid / groups;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const long long base = (long long)b * 512 * ny + (long long)g * 64 + l + (long long)w * ny * 8;
+    float2 v[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = src[base + (long long)e * ny * 32];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) dst[base + (long long)e * ny * 32] = v[e];
+}
 hipError_t launch_tile_copy(const float2* src, float2* dst, int nimg, int ny, int tw, hipStream_t s) {
-    const int groups = ny / tw;  // ny % tw == 0, checked by the caller
+    const int groups = ny / (tw % 1000);  // ny % (tw % 1000) == 0, checked by the caller
     const dim3 grid((unsigned)(groups * nimg));
     if (tw == 8) hipLaunchKernelGGL(tile_copy_kernel<8>, grid, dim3(256), 0, s, src, dst, ny, groups);
     else if (tw == 16) hipLaunchKernelGGL(tile_copy_kernel<16>, grid, dim3(512), 0, s, src, dst, ny, groups);
+    else if (tw == 1008) hipLaunchKernelGGL(band_copy_kernel, grid, dim3(256), 0, s, src, dst, ny, groups);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

@@ -112,6 +112,7 @@ template <typename C> struct RowArgs {
     int iacc;
     Re iw;
     int nostore;  // 1: only the intensity is written, not the wave
+    int band;     // 1: psi is stored in 8-row bands (BandLayout; STEM waves at 512^2)
 };
 
 // IN/OUT: 0 none, +1 forward (exp +i), -1 inverse (exp -i, unscaled)
@@ -122,22 +123,38 @@ template <typename C> struct RowArgs {
 // transmission build); a compile-time switch so the slice kernels carry none
 // of its sincos code.  INTEN: the |psi|^2 output (p.inten) is written; also
 // compile-time, so the slice kernels hold no registers for it.
-template <typename C, int N, int IN, int OUT, int LPW, bool VREAL, bool INTEN = false>
+// BAND: psi is stored in 8-row bands, element (ix, iy) at
+// ((ix >> 3) * ny + iy) * 8 + (ix & 7); a work-group takes one band (LPW =
+// 8 lines of T = 64 threads) and lane l of wavefront w holds line l & 7 as
+// FFT thread 8 w + (l >> 3), so each load or store instruction covers one
+// contiguous 512-byte chunk (8 lines x 8 elements).  A line is then spread
+// over the 8 wavefronts of the work-group: the exchanges use work-group
+// barriers, double-buffered (one barrier per exchange).  The transforms are
+// the same, so are the values.
+template <typename C, int N, int IN, int OUT, int LPW, bool VREAL, bool INTEN = false, bool BAND = false>
 __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
     using Re = typename real_of<C>::type;
     using PL = Plan<N>;
     constexpr int T = PL::T, E = N / T;
-    constexpr bool WAVE = (64 % T) == 0;  // a line never straddles two wavefronts
+    constexpr bool WAVE = !BAND && (64 % T) == 0;  // a line never straddles two wavefronts
+    static_assert(!BAND || (T == 64 && LPW == 8 && !VREAL && !INTEN), "band layout: 64-thread lines, one band");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int lw = threadIdx.x / T, t = threadIdx.x % T;
+    const int lw = BAND ? (int)(threadIdx.x & 7) : (int)(threadIdx.x / T);
+    const int t = BAND ? (int)(8 * (threadIdx.x >> 6) + ((threadIdx.x & 63) >> 3)) : (int)(threadIdx.x % T);
     const int nxg = (p.nx + LPW - 1) / LPW;
     const int gx = blockIdx.x % nxg, chunk = blockIdx.x / nxg;
     const int ix_raw = gx * LPW + lw;
     const bool live = ix_raw < p.nx;
     const int ix = live ? ix_raw : p.nx - 1;  // dead rows compute, never store
     const int b0 = chunk * p.wpb, b1 = min(p.nb, b0 + p.wpb);
-    const LdsView<C> lds[2] = {{smem + lw * (N + N / 16), 1, true}, {smem + lw * (N + N / 16), 1, true}};
+    constexpr int LINE = N + N / 16;
+    const LdsView<C> lds[2] = {{smem + lw * LINE, 1, true}, {smem + ((BAND ? LPW : 0) + lw) * LINE, 1, true}};
+    // element offset of (this line, y) in a wave's image
+    auto eo = [&](int y) -> unsigned {
+        if constexpr (BAND) return ((unsigned)gx * (unsigned)N + (unsigned)y) * 8u + (unsigned)lw;
+        else return (unsigned)ix * (unsigned)N + (unsigned)y;
+    };
 
     TwReg<PL, C> tw;
     if constexpr (IN != 0 || OUT != 0) load_twiddles<PL>(tw, t, p.tw);
@@ -190,10 +207,8 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
                                  : p.psi + (long long)__builtin_amdgcn_readfirstlane(b) * p.nx * N;
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<C*>(img), 0, valid ? p.nx * N * (int)sizeof(C) : 0, kBufWord3);
-            const unsigned row = (unsigned)ix * (unsigned)N;
 #pragma unroll
-            for (int e = 0; e < E; ++e)
-                v[e] = buf_load<C>(rs, (row + (unsigned)(((ld_mask >> e) & 1u) ? t + e * T : t)) * (unsigned)sizeof(C));
+            for (int e = 0; e < E; ++e) v[e] = buf_load<C>(rs, eo(((ld_mask >> e) & 1u) ? t + e * T : t) * (unsigned)sizeof(C));
         }
     };
 
@@ -264,7 +279,7 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
             }
         }
         if (p.trans_tab) tnext = p.trans_tab[b + 1 < b1 ? b + 1 : b];
-        if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
+        if constexpr (IN != 0) fft_line<PL, IN, WAVE, BAND, 0>(a, t, lds, tw);
         if (p.scale != (Re)1 && !fold) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = cscale(a[e], p.scale);
@@ -286,7 +301,7 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
                 buf_store(v, ri, live ? (row0 + (unsigned)(t + e * T)) * (unsigned)sizeof(Re) : kBufDrop);
             }
         }
-        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(a, t, lds, tw);
+        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE, BAND, (IN != 0 ? PL::P - 1 : 0)>(a, t, lds, tw);
         {
             // buffer stores into wave b's image; dead rows, pruned elements
             // and a sweep that stores nothing aim past the bound (dropped):
@@ -294,11 +309,9 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
             C* img = p.psi + (long long)__builtin_amdgcn_readfirstlane(b) * p.nx * N;
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 img, 0, p.nostore ? 0 : p.nx * N * (int)sizeof(C), kBufWord3);
-            const unsigned row = (unsigned)ix * (unsigned)N;
 #pragma unroll
             for (int e = 0; e < E; ++e)
-                buf_store(a[e], rs, (live && ((st_mask >> e) & 1u)) ? (row + (unsigned)(t + e * T)) * (unsigned)sizeof(C)
-                                                                     : kBufDrop);
+                buf_store(a[e], rs, (live && ((st_mask >> e) & 1u)) ? eo(t + e * T) * (unsigned)sizeof(C) : kBufDrop);
         }
         if constexpr (PF) {
 #pragma unroll
@@ -320,6 +333,7 @@ enum ColOp : int {
     kColGen = 8,     // generate a STEM probe in k-space (input ignored)
     kColCtf = 16,    // objective lens exp(-i chi) inside the aperture, 0 outside
     kColStore = 32,  // write the result back
+    kColObl = 64,    // with kColProp: the oblique-cell propagator (FP64 metric per element)
 };
 
 template <typename C> struct ColArgs {
@@ -353,6 +367,11 @@ template <typename C> struct ColArgs {
     // prune != 0: columns with ky2[iy] >= k2max are skipped entirely (their
     // k-space values are all masked to zero, so nothing downstream reads them)
     int prune;
+    int band;  // 1: psi (and out) stored in 8-row bands (BandLayout)
+    // oblique cell (kColObl): x = fx / obl_asg, y = fy / obl_bsg (a sin gam,
+    // b sin gam), s = x (x - obl_2c y) + y^2 (obl_2c = 2 cos gam), kept where
+    // s < obl_smax with the factor exp(-i (obl_pl s - obl_tx x - obl_ty y))
+    double obl_asg, obl_bsg, obl_2c, obl_smax, obl_pl, obl_tx, obl_ty;
 };
 
 __device__ __forceinline__ int freq_index(int i, int n) {
@@ -414,16 +433,33 @@ template <typename C, int N, int TW, bool DB> constexpr size_t col_list_offset()
 //        of work-groups idles most of the chip; at each step the work-groups
 //        of one XCD hold consecutive groups of one wave, as in the chunked
 //        order.
-template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB>
+//
+// BAND: the waves are stored in 8-row bands, element (ix, iy) at
+// ((ix >> 3) * ny + iy) * 8 + (ix & 7) (BandLayout below), and lane l of
+// wavefront w holds column l >> 3 and row thread t = 8 w + (l & 7): each
+// load or store instruction then covers 8 columns x 8 rows of one band, one
+// contiguous 512-byte chunk, where the plain layout's instructions touch
+// 8 rows x 8 columns at the row stride (64-byte pieces).  The transforms and
+// their LDS exchanges are the same (they only see the column lw and the row
+// thread t), so the results are the same values.
+template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB, bool BAND = false>
 __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(ColArgs<C> p) {
     using Re = typename real_of<C>::type;
     using PL = ColPlan<N>;
     constexpr int T = PL::T, E = N / T;
     constexpr bool WAVE = false;  // a column is spread over several wavefronts
     static_assert(TW <= 32, "live-column masks are 32 bits");
+    static_assert(!BAND || (TW == 8 && T % 8 == 0), "band layout: 8-column tiles, rows in whole bands per thread");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int lw = threadIdx.x % TW, t = threadIdx.x / TW;
+    const int lw = BAND ? (int)((threadIdx.x & 63) >> 3) : (int)(threadIdx.x % TW);
+    const int t = BAND ? (int)(8 * (threadIdx.x >> 6) + (threadIdx.x & 7)) : (int)(threadIdx.x / TW);
+    // element offset of (row t, column y) of an image (row t + e T adds e T ny
+    // in both layouts, since T is a multiple of 8 in the band layout)
+    auto elem = [&](int y) -> unsigned {
+        if constexpr (BAND) return ((unsigned)(t >> 3) * (unsigned)p.ny + (unsigned)y) * 8u + (unsigned)(t & 7);
+        else return (unsigned)t * (unsigned)p.ny + (unsigned)y;
+    };
     const int groups = (p.ny + TW - 1) / TW;
     // XCD-aware order: work-groups are dispatched round-robin over the 8 XCDs;
     // renumber them so that consecutive logical groups (neighbouring columns,
@@ -562,7 +598,7 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         const C* img = p.psi + (long long)__builtin_amdgcn_readfirstlane(b) * N * p.ny;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<C*>(img), 0,
                                                                              valid ? img_bytes : 0, kBufWord3);
-        const unsigned off0 = ((unsigned)t * (unsigned)p.ny + (unsigned)(g * TW + ld_col)) * (unsigned)sizeof(C);
+        const unsigned off0 = elem(g * TW + ld_col) * (unsigned)sizeof(C);
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = buf_load<C>(rs, off0 + (unsigned)e * ostep * (unsigned)sizeof(C));
     };
@@ -595,6 +631,7 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         const int iy = col_y(g);
         const bool live = (m >> lw) & 1u;
         const double ky = (gen || (OPS & kColCtf)) ? (double)freq_index(iy, p.ny) / p.by : 0.0;
+        const double oy = (OPS & kColObl) ? (double)freq_index(iy, p.ny) / p.obl_bsg : 0.0;
         if constexpr (gen) {
             const double2 pos = p.pos[b];
 #pragma unroll
@@ -621,7 +658,17 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const float k2 = KX2(e) + ky2;
-                if constexpr ((OPS & (kColProp | kColMask)) != 0) {
+                if constexpr ((OPS & kColObl) != 0) {
+                    // Propagator.cpp:55-107 (this engine's sign), FP64 metric
+                    // and phase, the phase reduced mod 2 pi before the
+                    // engine-precision sincos
+                    const double ox = (double)freq_index(t + e * T, N) / p.obl_asg;
+                    const double s2 = ox * (ox - p.obl_2c * oy) + oy * oy;
+                    if (s2 < p.obl_smax)
+                        a[e] = cmul(a[e], expi_reduced<C>(-(p.obl_pl * s2 - p.obl_tx * ox - p.obl_ty * oy)));
+                    else
+                        a[e] = mk<C>((Re)0, (Re)0);
+                } else if constexpr ((OPS & (kColProp | kColMask)) != 0) {
                     if (k2 < p.k2max) {
                         // slicelib.cpp:765-770: (w * py) * px
                         if constexpr ((OPS & kColProp) != 0) a[e] = cmul(cmul(a[e], py), PX(e));
@@ -692,8 +739,7 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
             C* img = (p.out ? p.out : p.psi) + (long long)__builtin_amdgcn_readfirstlane(b) * N * p.ny;
             const int bytes = (p.ops & kColStore) ? N * p.ny * (int)sizeof(C) : 0;
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(img, 0, bytes, kBufWord3);
-            const unsigned so = live ? ((unsigned)t * (unsigned)p.ny + (unsigned)(g * TW + lw)) * (unsigned)sizeof(C)
-                                     : kBufDrop;
+            const unsigned so = live ? elem(g * TW + lw) * (unsigned)sizeof(C) : kBufDrop;
 #pragma unroll
             for (int e = 0; e < E; ++e) buf_store(a[e], rs, so + (unsigned)e * ostep * (unsigned)sizeof(C));
         }

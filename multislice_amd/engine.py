@@ -55,7 +55,7 @@ class Stack(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("ax", C.c_double), ("by", C.c_double),
                 ("kev", C.c_double), ("ctilt_x", C.c_double), ("ctilt_y", C.c_double), ("bw", C.c_double),
                 ("nlayer", C.c_int32), ("cz", C.POINTER(C.c_double)), ("nslice", C.c_int32),
-                ("layer", C.POINTER(C.c_int32))]
+                ("layer", C.POINTER(C.c_int32)), ("gam", C.c_double)]
 
 
 class Probe(C.Structure):
@@ -64,7 +64,7 @@ class Probe(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("row_wpb", C.c_int32), ("col_wpb", C.c_int32), ("batch", C.c_int32), ("ms_per_slice", C.c_double),
-                ("col_tile", C.c_int32)]
+                ("col_tile", C.c_int32), ("band", C.c_int32)]
 
 
 class Job(C.Structure):
@@ -300,12 +300,13 @@ class Context:
         t = Tuning()
         _check(load().msx_autotune(self.h, nx, ny or nx, prec, max_batch, C.byref(t)))
         return dict(row_wpb=t.row_wpb, col_wpb=t.col_wpb, batch=t.batch, ms_per_slice=t.ms_per_slice,
-                    col_tile=t.col_tile)
+                    col_tile=t.col_tile, band=t.band)
 
     def set_tuning(self, nx, ny=None, prec=F32, tuning=None):
         """Install a saved autotuner result (dict as returned by autotune), or clear it (None)."""
         t = None if tuning is None else C.byref(Tuning(tuning["row_wpb"], tuning["col_wpb"], tuning["batch"],
-                                                       tuning.get("ms_per_slice", 0.0), tuning.get("col_tile", 0)))
+                                                       tuning.get("ms_per_slice", 0.0), tuning.get("col_tile", 0),
+                                                       tuning.get("band", 0)))
         _check(load().msx_set_tuning(self.h, nx, ny or nx, prec, t))
 
     def set_graphs(self, enable=True):
@@ -368,7 +369,6 @@ class Context:
         if _is_torch(atoms):
             if not atoms.is_cuda or not atoms.is_contiguous() or atoms.numel() % ATOM_DTYPE.itemsize:
                 raise ValueError("device atoms: a contiguous CUDA tensor of whole ATOM_DTYPE records")
-            _need(where == DEVICE, "device atoms need a device output (a CUDA tensor)")
             self._order(atoms, out)
             _check(load().msx_potential_device_atoms(self.h, C.c_void_p(atoms.data_ptr()),
                                                      atoms.numel() // ATOM_DTYPE.itemsize, *geo))
@@ -467,23 +467,25 @@ class Context:
         return out
 
     @staticmethod
-    def _stack(nx, ny, ax, by, kev, cz, layer, tilt=(0.0, 0.0), bw=2.0 / 3.0):
+    def _stack(nx, ny, ax, by, kev, cz, layer, tilt=(0.0, 0.0), bw=2.0 / 3.0, gam=0.0):
         cz = np.ascontiguousarray(cz, np.float64)
         layer = np.ascontiguousarray(layer, np.int32)
         st = Stack(nx, ny, ax, by, kev, tilt[0], tilt[1], bw, len(cz),
                    cz.ctypes.data_as(C.POINTER(C.c_double)), len(layer),
-                   layer.ctypes.data_as(C.POINTER(C.c_int32)))
+                   layer.ctypes.data_as(C.POINTER(C.c_int32)), float(gam))
         return st, (cz, layer)
 
-    def multislice(self, trans, cz, layer, psi, ax, by, kev, tilt=(0.0, 0.0), bw=2.0 / 3.0, psi0=None):
+    def multislice(self, trans, cz, layer, psi, ax, by, kev, tilt=(0.0, 0.0), bw=2.0 / 3.0, psi0=None, gam=0.0):
         """Slice loop (mulslice.cpp:703-739) on nwave waves psi (nwave, nx, ny) in place.
-        With psi0 (one image) every wave starts from psi0 and psi is output only."""
+        With psi0 (one image) every wave starts from psi0 and psi is output only.
+        gam: angle between the cell axes (rad; 0 or pi/2 orthogonal): an oblique
+        cell propagates with Musli's metric (Propagator.cpp:55-107)."""
         prec = _prec_of(trans)
         nl, nx, ny = _check_trans(trans)
         _check_layers(nl, cz, layer)
         _check_waves("psi", psi, nx, ny)
         _need(psi0 is None or _numel(psi0) == nx * ny, f"psi0 must be one ({nx}, {ny}) image")
-        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
+        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw, gam)
         pt = _buf(trans, prec)
         pw = _buf(psi, prec, writable=True)
         bufs = [pt, pw]
@@ -534,7 +536,7 @@ class Context:
         _check_layers(nl, cz, layer)
         st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw)
         pt = _buf(trans, prec)
-        _need(np.size(positions) % 2 == 0 and np.size(positions) > 0, "positions must be (nprobe, 2)")
+        _need(np.size(positions) % 2 == 0, "positions must be (nprobe, 2)")
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 2)
         det = np.ascontiguousarray(detectors, np.float64).reshape(-1, 2)
         ck = np.ascontiguousarray(checkpoints, np.int32)

@@ -27,6 +27,6 @@ for f in "$FW"/*.c; do
   gcc -O2 -fPIC -w -DFFTW_ENABLE_FLOAT -I"$FW" -c "$f" -o "$o"
   objs+=("$o")
 done
-gcc -O2 -fPIC -w -DFFTW_ENABLE_FLOAT -I"$FW" -c "$HERE/ref_fftw2_harness.c" -o "$TMP/h.o"
-gcc -shared "${objs[@]}" "$TMP/h.o" -lm -o "$OUT/libfftw2f_ref.so"
+gcc -O2 -fPIC -w -fopenmp -DFFTW_ENABLE_FLOAT -I"$FW" -c "$HERE/ref_fftw2_harness.c" -o "$TMP/h.o"
+gcc -shared -fopenmp "${objs[@]}" "$TMP/h.o" -lm -o "$OUT/libfftw2f_ref.so"
 echo "built $OUT/libkirkland_ref.so $OUT/libfftw2f_ref.so"

@@ -542,6 +542,58 @@ ORC_API int orc_multislice_f32(const float *trans, int nlayer, const float *cz, 
     return 0;
 }
 
+/* Oblique-cell slice loop: Musli's ApplyPropagator (Musli console 1.3
+ * (STEM) 2/Muslib/Propagator.cpp:55-107) in Kirkland's sign convention (the
+ * reference applies exp(+i con) with its FFT signs reversed; here exp(-i con),
+ * the transform pair of cfpix):
+ *   x = i/(a sin gam), y = j/(b sin gam)  (freqn indices; Musli's i < N/2
+ *      rule differs only at the Nyquist index, which the mask removes)
+ *   s = x (x - 2 y cos gam) + y^2                     (Propagator.cpp:86)
+ *   con = pi lambda dz s - 2 pi dz (x tan tx + y tan ty) (:67-72, :94)
+ *   P = exp(-i con) where s < smax, else 0, smax = (bw min(N/(2a), M/(2b)))^2
+ *      (:69-70 with bw = 1; bw = 2/3 gives Kirkland's band limit radius)
+ * in FP64 throughout, with Kirkland's lambda (the engine's, not Musli's
+ * 0.387818/sqrt(...) fit, :67).  Slice: w *= t; fft; w *= P; ifft. */
+ORC_API int orc_multislice_oblique_f64(const double *trans, int nlayer, const double *cz, const int *layer,
+                                       int nslice, int nx, int ny, double ax, double by, double gam, double kev,
+                                       double ctiltx, double ctilty, double bw, double *wave) {
+    const double sg = std::sin(gam), cg = std::cos(gam);
+    const double lam = wavelength(kev);
+    double kmax = std::min(nx / (2.0 * ax), ny / (2.0 * by)) * bw;
+    const double smax = kmax * kmax;
+    std::complex<double> *w = reinterpret_cast<std::complex<double> *>(wave);
+    const std::complex<double> *T = reinterpret_cast<const std::complex<double> *>(trans);
+    Fft1d<double> fyp(ny, +1), fxp(nx, +1), fym(ny, -1), fxm(nx, -1);
+    for (int is = 0; is < nslice; is++) {
+        const int l = layer[is];
+        if (l < 0 || l >= nlayer) return -1;
+        const std::complex<double> *tl = T + (size_t)l * nx * ny;
+        for (size_t j = 0; j < (size_t)nx * ny; j++) w[j] *= tl[j];
+        for (int ix = 0; ix < nx; ix++) fyp.run(w + (size_t)ix * ny, 1);
+        for (int iy = 0; iy < ny; iy++) fxp.run(w + iy, ny);
+        const double gk = M_PI * lam * cz[l], xtg = 2.0 * M_PI * cz[l] * std::tan(ctiltx),
+                     ytg = 2.0 * M_PI * cz[l] * std::tan(ctilty);
+        for (int ix = 0; ix < nx; ix++) {
+            const double x = (double)freq_index(ix, nx) / (ax * sg);
+            for (int iy = 0; iy < ny; iy++) {
+                const double y = (double)freq_index(iy, ny) / (by * sg);
+                const double s2 = x * (x - 2.0 * y * cg) + y * y;
+                std::complex<double> &v = w[(size_t)ix * ny + iy];
+                if (!(s2 < smax)) {
+                    v = 0;
+                } else {
+                    const double con = gk * s2 - xtg * x - ytg * y;
+                    v *= std::complex<double>(std::cos(con), -std::sin(con));
+                }
+            }
+        }
+        for (int ix = 0; ix < nx; ix++) fym.run(w + (size_t)ix * ny, 1);
+        for (int iy = 0; iy < ny; iy++) fxm.run(w + iy, ny);
+        scale_inv(w, nx, ny);
+    }
+    return 0;
+}
+
 ORC_API int orc_multislice_f64(const double *trans, int nlayer, const double *cz, const int *layer,
                                int nslice, int nx, int ny, double ax, double by, double kev,
                                double ctiltx, double ctilty, double bw, double *wave, double *intensity) {

@@ -44,6 +44,7 @@ def lib():
             ("orc_propagator_f64", None, [i, i, d, d, d, d, d, d, _dp, _dp, _dp, _dp]),
             ("orc_multislice_f32", i, [_fp, i, _fp, _ip, i, i, i, d, d, d, d, d, d, _fp, C.c_void_p]),
             ("orc_multislice_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, _dp, C.c_void_p]),
+            ("orc_multislice_oblique_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, d, _dp]),
             ("orc_probe_f64", i, [i, i, d, d, d, d, d, d, d, d, d, _dp]),
             ("orc_stem_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
             ("orc_stem_f32", i, [_fp, i, _fp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
@@ -227,6 +228,22 @@ class Oracle:
         if rc != 0:
             raise ValueError("orc_multislice failed %d" % rc)
         return (w, inten) if want_intensity else w
+
+    @staticmethod
+    def multislice_oblique(trans, cz, layer, wave, ax, by, gam, kev, tilt=(0.0, 0.0), bw=2.0 / 3.0):
+        """Slice loop of an oblique cell (angle gam between a and b): Musli's
+        propagator (Propagator.cpp:55-107) in Kirkland's convention, FP64."""
+        trans = np.asarray(trans)
+        nl, nx, ny = trans.shape
+        layer = np.ascontiguousarray(layer, np.int32)
+        w = _c128(wave).copy()
+        rc = lib().orc_multislice_oblique_f64(_c128(trans).view(np.float64).reshape(-1), nl,
+                                              np.ascontiguousarray(cz, np.float64), layer, len(layer), nx, ny,
+                                              ax, by, gam, kev, tilt[0], tilt[1], bw,
+                                              w.view(np.float64).reshape(-1))
+        if rc != 0:
+            raise ValueError("orc_multislice_oblique failed %d" % rc)
+        return w
 
     @staticmethod
     def probe(nx, ny, ax, by, kev, alpha_max, alpha_min=0.0, cs=0.0, df=0.0, pos=(0.0, 0.0)):

@@ -78,10 +78,32 @@ def test_create_reports_missing_device_or_succeeds():
         assert len(L.msx_last_error()) > 0
 
 
-def test_structs_match_header_layout():
-    # msx_atom: int32 Z, float occ, double x, y, z -> 32 bytes
+def test_structs_match_header_layout(tmp_path):
+    """The ctypes structs of the binding have the sizes and field offsets the
+    C compiler gives the header's structs."""
+    import subprocess
+    from multislice_amd.engine import Stack, Probe, Detector, Tuning, Job, Lens
     assert msx.ATOM_DTYPE.itemsize == 32
-    from multislice_amd.engine import Stack, Probe, Detector
-    assert C.sizeof(Probe) == 32 and C.sizeof(Detector) == 16
-    # int32 nx, ny; 6 doubles; int32 nlayer (+pad); ptr; int32 nslice (+pad); ptr
-    assert C.sizeof(Stack) == 8 + 6 * 8 + 8 + 8 + 8 + 8
+    fields = {"msx_stack": (Stack, ["nx", "ny", "ax", "by", "kev", "ctilt_x", "ctilt_y", "bw", "nlayer", "cz",
+                                     "nslice", "layer", "gam"]),
+              "msx_tuning": (Tuning, ["row_wpb", "col_wpb", "batch", "ms_per_slice", "col_tile", "band"]),
+              "msx_probe": (Probe, ["alpha_max", "alpha_min", "cs", "df"]),
+              "msx_detector": (Detector, ["k_inner", "k_outer"]),
+              "msx_job": (Job, ["stack", "trans", "psi0", "psi"]),
+              "msx_lens": (Lens, ["cs", "df", "aobj", "dfa2", "dfa2phi", "dfa3", "dfa3phi"])}
+    src = ["#include <stdio.h>", "#include <stddef.h>", "#include \"msx.h\"", "int main(void) {"]
+    for st, (_, names) in fields.items():
+        src.append(f'printf("{st} %zu\\n", sizeof({st}));')
+        for n in names:
+            src.append(f'printf("{st}.{n} %zu\\n", offsetof({st}, {n}));')
+    src.append("return 0; }")
+    (tmp_path / "lay.c").write_text("\n".join(src))
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run(["gcc", "-I" + os.path.join(root, "include"), str(tmp_path / "lay.c"), "-o", str(tmp_path / "lay")],
+                   check=True)
+    out = dict(line.split() for line in subprocess.run([str(tmp_path / "lay")], capture_output=True, text=True,
+                                                      check=True).stdout.splitlines())
+    for st, (cls, names) in fields.items():
+        assert int(out[st]) == C.sizeof(cls), st
+        for n in names:
+            assert int(out[f"{st}.{n}"]) == getattr(cls, n).offset, (st, n)

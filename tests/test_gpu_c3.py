@@ -1,5 +1,6 @@
 """Parity of the batched-STEM benchmark workload itself, at its full size and
-with its launch shapes: SrTiO3 [001] 12x12x50 cells (46.86 A, 36,000 atoms,
+with its launch shapes, in both layouts of the wave batches (the image layout
+and 8-row bands, between which the autotuner chooses): SrTiO3 [001] 12x12x50 cells (46.86 A, 36,000 atoms,
 frozen-phonon configuration 0 of the benchmark's seed), 512^2, 100 slices of
 1.9525 A, 300 kV, 21 mrad, Cs 1 um, BF 0-10 mrad and ADF 70-200 mrad (the
 ADF band clipped at the 2/3 band limit, 71.7 mrad, on this grid), launch
@@ -29,6 +30,7 @@ A, CELLS, N, NSL, KEV, ALPHA, CS = 3.905, (12, 12, 50), 512, 100, 300.0, 0.021, 
 SCAN = 128
 PICK = [0, 45, 91, 127, 128, 131, 133, 135]  # probes of both launch batches
 TUNING = dict(row_wpb=8, col_wpb=-1, batch=128, col_tile=8)
+LAYOUTS = {"image": 0, "band": 1}  # waves in the image layout / in 8-row bands
 
 
 def scan_positions(n):
@@ -63,11 +65,13 @@ def c3():
     return at, cell, t64, pos, ref
 
 
-@pytest.fixture(scope="module")
-def bench_ctx():
+@pytest.fixture(scope="module", params=sorted(LAYOUTS))
+def bench_ctx(request):
     c = msx.Context(0)
-    c.set_tuning(N, N, msx.F32, TUNING)
+    c.set_tuning(N, N, msx.F32, dict(TUNING, band=LAYOUTS[request.param]))
+    c.set_tuning(N, N, msx.F64, dict(TUNING, band=LAYOUTS[request.param]))
     c.set_graphs(True)
+    c.layout = request.param
     yield c
     c.close()
 
@@ -85,7 +89,7 @@ def test_c3_fp32_chain_from_atoms_matches_oracle(bench_ctx, c3):
     assert caps >= 1 and replays >= 1  # the graph path ran (the second call replays)
     assert np.array_equal(out[0], out[1])
     mr, pw = errors(out[1][:, :, PICK], ref)
-    print(f"C3 FP32 chain from atoms vs oracle FP64 (8 probes): BF max-rel {mr[0]:.2e} pointwise {pw[0]:.2e}; "
+    print(f"C3 [{bench_ctx.layout} layout] FP32 chain from atoms vs oracle FP64 (8 probes): BF max-rel {mr[0]:.2e} pointwise {pw[0]:.2e}; "
           f"ADF max-rel {mr[1]:.2e} pointwise {pw[1]:.2e}")
     assert mr[0] <= 1e-5 and pw[0] <= 1e-5
     assert mr[1] <= 1e-4 and pw[1] <= 1e-4
@@ -105,7 +109,7 @@ def test_c3_stack_fp32_vs_fp64_engine_and_oracle(bench_ctx, c3):
     mr64, pw64 = errors(got64[:, :, PICK], ref)
     mr32, pw32 = errors(got32[:, :, PICK], ref)
     mra, pwa = errors(got32, got64)
-    print(f"C3 FP64 engine vs oracle: max-rel {mr64.max():.2e}; FP32 engine vs oracle: BF {mr32[0]:.2e} "
+    print(f"C3 [{bench_ctx.layout} layout] FP64 engine vs oracle: max-rel {mr64.max():.2e}; FP32 engine vs oracle: BF {mr32[0]:.2e} "
           f"ADF {mr32[1]:.2e}; FP32 vs FP64 engine, all 136 probes: BF {mra[0]:.2e} (pointwise {pwa[0]:.2e}), "
           f"ADF {mra[1]:.2e} (pointwise {pwa[1]:.2e})")
     assert mr64.max() <= 1e-11 and pw64.max() <= 1e-11

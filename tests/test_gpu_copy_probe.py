@@ -9,7 +9,8 @@ import multislice_amd as msx
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("mode,nimg,ny", [(0, 3, 512), (0, 1, 8), (8, 3, 512), (16, 2, 256), (8, 1, 64)])
+@pytest.mark.parametrize("mode,nimg,ny", [(0, 3, 512), (0, 1, 8), (8, 3, 512), (16, 2, 256), (8, 1, 64), (1008, 3, 512),
+                                         (1008, 1, 64)])
 def test_copy_probe_copies(mode, nimg, ny):
     import torch
     ctx = msx.Context(0)
@@ -27,7 +28,7 @@ def test_copy_probe_rejects():
     ctx = msx.Context(0)
     a = torch.zeros((1, 512, 512), dtype=torch.complex64, device="cuda")
     b = torch.zeros_like(a)
-    for mode, nx, ny in ((8, 256, 512), (16, 512, 504), (4, 512, 512), (0, 512, 4)):
+    for mode, nx, ny in ((8, 256, 512), (16, 512, 504), (4, 512, 512), (0, 512, 4), (1008, 512, 500)):
         with pytest.raises(msx.MsxError):
             ctx.copy_probe(mode, a, b, 1, nx, ny)
     ctx.close()

@@ -51,6 +51,6 @@ def test_displace_out_none_waits_for_its_clone():
     out = msx.displace(dev, {8: 0.1}, 11, 3, ctx=ctx)  # out=None: cloned on torch's stream, then written
     torch.cuda.synchronize()
     got = out.cpu().numpy().view(msx.ATOM_DTYPE)
-    for k in ("x", "y", "z"):
-        assert np.array_equal(got[k], host[k]), k
+    for k in ("x", "y", "z"):  # device and host agree to FP64 rounding; undisplaced atoms would be ~0.1 A off
+        assert np.abs(got[k] - host[k]).max() < 1e-12, k
     ctx.close()
