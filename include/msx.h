@@ -96,17 +96,11 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
  * the built-in one or its alternative for this line length) and the STEM
  * batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
  * ms_per_slice is the time of one wave through one slice at that batch.
- * At 512 x 512 the tuner also times msx_stem's waves stored in 8-row bands
- * (band below) against the image layout and keeps the faster.
  * msx_set_tuning installs a saved result (NULL clears it). */
 typedef struct msx_tuning {
     int32_t row_wpb, col_wpb, batch;
     double ms_per_slice;
     int32_t col_tile;
-    /* layout of msx_stem's wave batches: 0 = the image layout, 1 = 8-row
-     * bands (element (ix, iy) at ((ix >> 3) ny + iy) 8 + (ix & 7); 512 x 512
-     * only), whose sweeps read and write contiguous 512-byte chunks */
-    int32_t band;
 } msx_tuning;
 MSX_API int msx_autotune(msx_ctx ctx, int nx, int ny, int prec, int max_batch, msx_tuning* best);
 MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuning* t);
@@ -292,6 +286,22 @@ MSX_API int msx_potential_rs(msx_ctx ctx, const msx_atom* atoms, int64_t natoms,
 MSX_API int msx_potential_rec(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const double* dw,
                               const double* ads, double a, double b, double c, double gam, int nmult, int mmult,
                               int nx, int ny, int nslices, void* V, void* Vabs, int prec, int where);
+
+/* The curve tables of the tabulated-curve potentials, built from the
+ * Doyle-Turner factors (host arithmetic; no context).  Rows nt = 0..ntypes-1
+ * of sc[ntypes][npoints] belong to element types[nt] (0..92; row 0 of the
+ * table is empty), sampled at s = 0, dds, 2 dds, ... (accumulated by
+ * repeated addition, as the reference):
+ *   msx_create_pots_rec_dt: CreatePots_rec_DT_DW (Muslib/Curves.cpp:45-63),
+ *     sc = f_e(s) exp(-dw[nt] s^2) vol with f_e(s) = sum_j a_j exp(-b_j s^2)
+ *     (scat_ampl_rec_DT_DW, :3-11), s = sin(theta)/lambda;
+ *   msx_create_pots_rs_dt: CreatePots_rs_DT_DW (:64-81), sc = 4 pi sum_j
+ *     a_j/(b_j + dw[nt]) exp(-4 pi^2 s^2/(b_j + dw[nt])) over the terms with
+ *     b_j > 0 (scat_ampl_rs_DT_DW, :12-21), s = distance in Angstrom. */
+MSX_API int msx_create_pots_rec_dt(int ntypes, int npoints, double dds, const int32_t* types, const double* dw,
+                                   double vol, double* sc);
+MSX_API int msx_create_pots_rs_dt(int ntypes, int npoints, double dds, const int32_t* types, const double* dw,
+                                  double* sc);
 
 /* The tabulated-curve forms of the two potentials above: Muslib
  * Potential_rec_curve_DW (Pot_rec_curve_DW.cpp:7-164) and

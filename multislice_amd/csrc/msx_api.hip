@@ -16,6 +16,7 @@
 #include "msx_internal.h"
 #include "msx_philox.h"
 #include "fparams_tables.inc"
+#include "doyle_turner.inc"
 
 using namespace msx;
 
@@ -125,9 +126,6 @@ std::vector<float> k2_table(int n, double a) {
 // An oblique cell: gam set and not a right angle (to 1e-12 rad).
 bool is_oblique(const msx_stack* st) { return st->gam != 0.0 && std::fabs(st->gam - M_PI / 2) > 1e-12; }
 
-// grids whose STEM waves may be stored in 8-row bands (the band variants of
-// the sweeps are built for 512-point lines)
-bool band_layout_ok(int nx, int ny) { return nx == 512 && ny == 512; }
 int fidx(int i, int n) {
     int imid = (int)(n / 2.0 + 0.5);
     return i > imid ? i - n : i;
@@ -516,12 +514,9 @@ struct DetPlan {
 // with mixed_out, just before it: psi then holds IFFT_x(P FFT_x FFT_y(t psi))
 // of the last slice, i.e. FFT_y of the exit wave times nx, in the columns
 // ky2 < k2max (the others are not written).
-// band: the waves are stored in 8-row bands (the row and column sweeps'
-// band variants; start = 1 loops without the final inverse row sweep only).
 template <typename C>
 int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-                    const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false,
-                    int band = 0) {
+                    const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false) {
     using Re = typename real_of<C>::type;
     const int nx = st->nx, ny = st->ny;
     const long long img = (long long)nx * ny;
@@ -549,7 +544,6 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
         r.prune = obl ? 0 : ((s == 0) ? 2 : 3);
-        r.band = band;
         if (tu) r.wpb = tu->row_wpb;
         if (s == 0 && start == 0) {
             r.src = src;
@@ -579,7 +573,6 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
             c.obl_tx = 2.0 * M_PI * dz * std::tan(st->ctilt_x);
             c.obl_ty = 2.0 * M_PI * dz * std::tan(st->ctilt_y);
         }
-        c.band = band;
         if (tu) c.wpb = tu->col_wpb;
         c.tile = tile;
         bool is_ck = det && ick < det->nck && det->ck[ick] == s + 1;
@@ -633,12 +626,12 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
 // that fails falls back to launching the loop directly.
 template <typename C>
 int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-               const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false, int band = 0) {
+               const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false) {
     if (!ctx->graphs || ctx->prof)
-        return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
+        return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
     ctx->gkey.clear();
     ctx->keying = true;
-    const int rk = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
+    const int rk = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
     ctx->keying = false;
     CHK(rk);
     auto it = ctx->gcache.find(ctx->gkey);
@@ -647,9 +640,9 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         hipGraph_t g = nullptr;
         if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             (void)hipGetLastError();  // e.g. the legacy default stream: launch directly
-            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
         }
-        const int rc = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
+        const int rc = slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
         const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
         hipGraphExec_t ex = nullptr;
         hipError_t ei = (rc == MSX_OK && ec == hipSuccess && g) ? hipGraphInstantiate(&ex, g, nullptr, nullptr, 0)
@@ -657,7 +650,7 @@ int slice_loop(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const
         if (g) hipGraphDestroy(g);
         if (ei != hipSuccess) {
             (void)hipGetLastError();
-            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out, band);
+            return slice_loop_body<C>(ctx, st, tb, trans, psi, nb, src, src_bstride, start, det, mixed_out);
         }
         it = ctx->gcache.emplace(ctx->gkey, ex).first;
         ctx->graph_captures++;
@@ -717,14 +710,14 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     const msx_tuning saved = had ? ctx->tune[key] : msx_tuning{};
     // times (ms per launch) of the row and column sweeps for one shape, on
     // the STEM loop's sequence (waves start in k-space, no final inverse row
-    // sweep), with the waves in the image layout or in 8-row bands
-    auto trial = [&](int rw, int cw, int tile, int nb, int band, double* row_ms, double* col_ms) -> int {
-        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0, tile, band};
+    // sweep)
+    auto trial = [&](int rw, int cw, int tile, int nb, double* row_ms, double* col_ms) -> int {
+        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0, tile};
         ctx->prof = true;
         for (int rep = 0; rep < 2; rep++) {  // the first pass warms up
             for (int k = 0; k < MSX_PROF_KINDS; k++) ctx->prof_n[k] = 0, ctx->prof_ms[k] = 0, ctx->prof_bytes[k] = 0;
             CHK(slice_loop_body<C>(ctx, &st, tb, static_cast<const C*>(ctx->trans.p), static_cast<C*>(ctx->wave.p),
-                                   nb, nullptr, 0, 1, nullptr, true, band));
+                                   nb, nullptr, 0, 1, nullptr, true));
         }
         *row_ms = ctx->prof_ms[MSX_PROF_ROW] / std::max(1LL, ctx->prof_n[MSX_PROF_ROW]);
         *col_ms = ctx->prof_ms[MSX_PROF_COL] / std::max(1LL, ctx->prof_n[MSX_PROF_COL]);
@@ -732,44 +725,38 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     };
     int tdef, talt;
     col_tiles<C>(nx, &tdef, &talt);
-    const int nlayout = band_layout_ok(nx, ny) ? 2 : 1;
+    std::vector<int> tiles{tdef};
+    if (talt != tdef) tiles.push_back(talt);
     int rc = MSX_OK;
-    msx_tuning best{1, 1, 0, 0.0, 0, 0};
+    msx_tuning best{1, 1, 0, 0.0, 0};
     do {
-        // per layout and batch size: the fastest waves-per-work-group of the
-        // row sweep, the fastest (tile, waves-per-work-group) of the column
-        // sweep, and the resulting probe-slice rate
-        double rate[2][kNB] = {}, top = 0.0;
-        int rw_of[2][kNB], cw_of[2][kNB], tile_of[2][kNB];
-        for (int L = 0; L < 2; L++)
-            for (int i = 0; i < kNB; i++) rw_of[L][i] = cw_of[L][i] = 1, tile_of[L][i] = tdef;
-        for (int L = 0; L < nlayout && rc == MSX_OK; L++) {
-            std::vector<int> tiles{tdef};
-            if (L == 0 && talt != tdef) tiles.push_back(talt);  // the band layout takes the built-in 8 columns
-            for (int i = 0; i < kNB && rc == MSX_OK; i++) {
-                if (batches[i] > bmax) break;
-                double brow = 1e30, bcol = 1e30;
-                for (int tile : tiles)
-                    for (int w : {1, 2, 4, 8, -1, -2}) {
-                        // w < 0: the column sweep's balanced schedule (the row
-                        // sweep is timed at positive w only)
-                        double rm, cm;
-                        if ((rc = trial(w > 0 ? w : 8, w, tile, batches[i], L, &rm, &cm)) != MSX_OK) break;
-                        if (tile == tdef && w > 0 && rm < brow) brow = rm, rw_of[L][i] = w;
-                        if (cm < bcol) bcol = cm, cw_of[L][i] = w, tile_of[L][i] = tile;
-                    }
-                rate[L][i] = batches[i] / (brow + bcol);
-                top = std::max(top, rate[L][i]);
-            }
+        // per batch size: the fastest waves-per-work-group of the row sweep,
+        // the fastest (tile, waves-per-work-group) of the column sweep, and
+        // the resulting probe-slice rate
+        double rate[kNB] = {0}, top = 0.0;
+        int rw_of[kNB], cw_of[kNB], tile_of[kNB];
+        for (int i = 0; i < kNB; i++) rw_of[i] = cw_of[i] = 1, tile_of[i] = tdef;
+        for (int i = 0; i < kNB && rc == MSX_OK; i++) {
+            if (batches[i] > bmax) break;
+            double brow = 1e30, bcol = 1e30;
+            for (int tile : tiles)
+                for (int w : {1, 2, 4, 8, -1, -2}) {
+                    // w < 0: the column sweep's balanced schedule (the row
+                    // sweep is timed at positive w only)
+                    double rm, cm;
+                    if ((rc = trial(w > 0 ? w : 8, w, tile, batches[i], &rm, &cm)) != MSX_OK) break;
+                    if (tile == tdef && w > 0 && rm < brow) brow = rm, rw_of[i] = w;
+                    if (cm < bcol) bcol = cm, cw_of[i] = w, tile_of[i] = tile;
+                }
+            rate[i] = batches[i] / (brow + bcol);
+            top = std::max(top, rate[i]);
         }
         if (rc != MSX_OK) break;
-        for (int i = 0; i < kNB && batches[i] <= bmax; i++) {
-            const int L = (nlayout > 1 && rate[1][i] > rate[0][i]) ? 1 : 0;
-            if (rate[L][i] >= 0.97 * top) {
-                best = msx_tuning{rw_of[L][i], cw_of[L][i], batches[i], 1.0 / rate[L][i], tile_of[L][i], L};
+        for (int i = 0; i < kNB && batches[i] <= bmax; i++)
+            if (rate[i] >= 0.97 * top) {
+                best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i], tile_of[i]};
                 break;
             }
-        }
     } while (0);
     ctx->prof = prof0;
     ctx->graphs = graphs0;
@@ -1045,8 +1032,6 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
     if (nap == 0) return fail(MSX_EINVAL, "probe aperture contains no pixel");
     const C *twx;
     CHK(ctx->twiddles<C>(nx, &twx));
-    // waves in 8-row bands where the tuning chose them (512 x 512 only)
-    const int band = (tu && tu->band && band_layout_ok(nx, ny)) ? 1 : 0;
 
     DetPlan dp;
     dp.ndet = ndet;
@@ -1081,9 +1066,8 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
         g.kap_lo = kap_lo;
         g.kap_hi = kap_hi;
         g.amp = (Re)std::sqrt((double)img / (double)nap);
-        g.band = band;
         LAUNCH_COL(nx, kColGenInv, g);
-        CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nb, nullptr, 0, 1, &dp, false, band));
+        CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nb, nullptr, 0, 1, &dp));
         // this batch's [nck][ndet][nb] block into columns p0.. of the results
         HIPCHK(hipMemcpy2DAsync(o_dev + p0, sizeof(double) * nprobe, s_dev, sizeof(double) * batch,
                                 sizeof(double) * nb, (size_t)nck * ndet, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1887,13 +1871,8 @@ MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuni
         ctx->tune.erase(std::make_tuple(nx, ny, prec));
         return MSX_OK;
     }
-    if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < -8 || t->col_wpb > 64 || t->batch < 0 || t->band < 0 ||
-        t->band > 1)
+    if (t->row_wpb < 0 || t->row_wpb > 64 || t->col_wpb < -8 || t->col_wpb > 64 || t->batch < 0)
         return fail(MSX_EINVAL, "bad tuning values");
-    if (t->band && !band_layout_ok(nx, ny))
-        return fail(MSX_EINVAL, "the band layout is built for 512 x 512 grids only");
-    if (t->band && t->col_tile != 0 && t->col_tile != 8)
-        return fail(MSX_EINVAL, "the band layout takes the 8-column tile");
     int tdef, talt;
     if (prec == MSX_F64) col_tiles<double2>(nx, &tdef, &talt);
     else col_tiles<float2>(nx, &tdef, &talt);
@@ -2115,6 +2094,45 @@ MSX_API int msx_potential_rs(msx_ctx ctx, const msx_atom* atoms, int64_t natoms,
     HIPCHK(hipSetDevice(ctx->device));
     return potential_rs_impl(ctx, atoms, natoms, dw, ads, a, b, c, gam, atrad, periodic, nx, ny, nslices, V, Vabs,
                              prec, where);
+}
+
+MSX_API int msx_create_pots_rec_dt(int ntypes, int npoints, double dds, const int32_t* types, const double* dw,
+                                   double vol, double* sc) {
+    if (ntypes < 1 || npoints < 1 || !types || !dw || !sc || !(dds > 0))
+        return fail(MSX_EINVAL, "curves: ntypes, npoints >= 1, dds > 0, types, dw and sc required");
+    for (int nt = 0; nt < ntypes; nt++)
+        if (types[nt] < 0 || types[nt] > 92) return fail(MSX_EINVAL, "type %d: Z = %d outside 0..92", nt, types[nt]);
+    for (int nt = 0; nt < ntypes; nt++) {
+        const double* ab = MSX_DOYLE_TURNER[types[nt]];
+        double s = 0.0;
+        for (int i = 0; i < npoints; i++, s += dds) {
+            const double s2 = s * s;
+            double f = 0.0;
+            for (int j = 0; j < 4; j++) f += ab[2 * j] * std::exp(-ab[2 * j + 1] * s2);
+            sc[(size_t)nt * npoints + i] = f * std::exp(-dw[nt] * s * s) * vol;
+        }
+    }
+    return MSX_OK;
+}
+
+MSX_API int msx_create_pots_rs_dt(int ntypes, int npoints, double dds, const int32_t* types, const double* dw,
+                                  double* sc) {
+    if (ntypes < 1 || npoints < 1 || !types || !dw || !sc || !(dds > 0))
+        return fail(MSX_EINVAL, "curves: ntypes, npoints >= 1, dds > 0, types, dw and sc required");
+    for (int nt = 0; nt < ntypes; nt++)
+        if (types[nt] < 0 || types[nt] > 92) return fail(MSX_EINVAL, "type %d: Z = %d outside 0..92", nt, types[nt]);
+    for (int nt = 0; nt < ntypes; nt++) {
+        const double* ab = MSX_DOYLE_TURNER[types[nt]];
+        double s = 0.0;
+        for (int i = 0; i < npoints; i++, s += dds) {
+            const double e = -4.0 * M_PI * M_PI * s * s;
+            double f = 0.0;
+            for (int j = 0; j < 4; j++)
+                if (ab[2 * j + 1] > 0) f += ab[2 * j] / (ab[2 * j + 1] + dw[nt]) * std::exp(e / (ab[2 * j + 1] + dw[nt]));
+            sc[(size_t)nt * npoints + i] = f * 4 * M_PI;
+        }
+    }
+    return MSX_OK;
 }
 
 MSX_API int msx_potential_rec_curve(msx_ctx ctx, const msx_atom* atoms, int64_t natoms, const int32_t* type,

@@ -89,31 +89,6 @@ hipError_t row_go(const RowArgs<C>& a0, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Row sweep on waves stored in 8-row bands: one band (8 lines) per
-// work-group, double-buffered exchanges (row_sweep BAND).  512-point lines
-// only (64 threads per line).
-template <typename C, int IN, int OUT> hipError_t row_go_band(const RowArgs<C>& a0, hipStream_t s) {
-    if constexpr (kN != 512) {
-        (void)a0;
-        (void)s;
-        return hipErrorInvalidValue;
-    } else {
-        constexpr int LPW = 8;
-        RowArgs<C> a = a0;
-        if (a.nx % LPW || a.src || a.vreal || a.inten || a.trans_tab) return hipErrorInvalidValue;
-        const int nxg = a.nx / LPW;
-        a.wpb = pick_wpb(a.wpb, (long long)nxg * a.nb, 4096);
-        const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
-        const unsigned grid = (unsigned)(nxg * nchunk);
-        const size_t smem = 2 * (size_t)LPW * (kN + kN / 16) * sizeof(C);
-        auto k = row_sweep<C, kN, IN, OUT, LPW, false, false, true>;
-        hipError_t e = set_smem(k, smem);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3(grid), dim3(kT * LPW), smem, s, a);
-        return hipGetLastError();
-    }
-}
-
 // work-groups of kernel k (block threads, smem bytes) resident on the whole
 // device at once, cached per device and LDS size in the caller's
 // per-kernel cache ((smem << 24) | (groups + 1); 0: empty)
@@ -140,15 +115,14 @@ hipError_t resident_groups(K k, int threads, size_t smem, std::atomic<long long>
     return hipSuccess;
 }
 
-template <typename C, int IN, int OUT, int OPS, int TW, bool BAND = false>
-hipError_t col_go_tw(const ColArgs<C>& a0, hipStream_t s) {
+template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(const ColArgs<C>& a0, hipStream_t s) {
     ColArgs<C> a = a0;
     const int groups = (a.ny + TW - 1) / TW;
     constexpr size_t tile = (size_t)kN * (TW + 1) * sizeof(C);
     // double-buffer the exchanges when two tiles fit in the work-group LDS
     // limit (a launch asking for the full 160 KiB is refused)
     constexpr bool DB = MSX_COL_DB && 2 * tile <= 150 * 1024;
-    auto k = col_sweep<C, kN, IN, OUT, TW, OPS, DB, BAND>;
+    auto k = col_sweep<C, kN, IN, OUT, TW, OPS, DB>;
     size_t smem = col_list_offset<C, kN, TW, DB>();  // exchange tiles, detector reduction scratch
     unsigned grid;
     if (a.wpb <= 0) {
@@ -185,10 +159,6 @@ template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<
 }  // namespace
 
 template <typename C> hipError_t MSX_CAT(launch_row_, MSX_N)(int mode, const RowArgs<C>& a, hipStream_t s) {
-    if (a.band) {  // waves in 8-row bands: the slice step of the STEM loop only
-        if (mode == kRowSlice) return row_go_band<C, -1, 1>(a, s);
-        return hipErrorInvalidValue;
-    }
     if (a.inten) {  // |psi|^2 output: only after the inverse transform
         if (mode == kRowInv && !a.vreal) return row_go<C, -1, 0, false, true>(a, s);
         return hipErrorInvalidValue;
@@ -213,18 +183,6 @@ template <typename C> hipError_t MSX_CAT(launch_row_, MSX_N)(int mode, const Row
 template <typename C> hipError_t MSX_CAT(launch_col_, MSX_N)(int mode, const ColArgs<C>& a, hipStream_t s) {
     const int ops = a.ops & ~kColStore;
     constexpr int kPD = kColProp | kColDetect;
-    if (a.band) {
-        // waves in 8-row bands (512^2 STEM loop): probe generation, slice
-        // steps with and without detectors, the last detector sweep
-        if constexpr (kN == 512 && col_tw<C>() == 8) {
-            if (a.tile != 0 && a.tile != 8) return hipErrorInvalidValue;
-            if (mode == kColGenInv && ops == kColGen) return col_go_tw<C, 0, -1, kColGen, 8, true>(a, s);
-            if (mode == kColSlice && ops == kColProp) return col_go_tw<C, 1, -1, kColProp, 8, true>(a, s);
-            if (mode == kColSlice && ops == kPD) return col_go_tw<C, 1, -1, kPD, 8, true>(a, s);
-            if (mode == kColFwd && ops == kPD) return col_go_tw<C, 1, 0, kPD, 8, true>(a, s);
-        }
-        return hipErrorInvalidValue;
-    }
     switch (mode) {
         case kColSlice:
             if (ops == kColProp) return col_go<C, 1, -1, kColProp>(a, s);

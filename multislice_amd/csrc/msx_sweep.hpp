@@ -112,7 +112,6 @@ template <typename C> struct RowArgs {
     int iacc;
     Re iw;
     int nostore;  // 1: only the intensity is written, not the wave
-    int band;     // 1: psi is stored in 8-row bands (BandLayout; STEM waves at 512^2)
 };
 
 // IN/OUT: 0 none, +1 forward (exp +i), -1 inverse (exp -i, unscaled)
@@ -123,38 +122,24 @@ template <typename C> struct RowArgs {
 // transmission build); a compile-time switch so the slice kernels carry none
 // of its sincos code.  INTEN: the |psi|^2 output (p.inten) is written; also
 // compile-time, so the slice kernels hold no registers for it.
-// BAND: psi is stored in 8-row bands, element (ix, iy) at
-// ((ix >> 3) * ny + iy) * 8 + (ix & 7); a work-group takes one band (LPW =
-// 8 lines of T = 64 threads) and lane l of wavefront w holds line l & 7 as
-// FFT thread 8 w + (l >> 3), so each load or store instruction covers one
-// contiguous 512-byte chunk (8 lines x 8 elements).  A line is then spread
-// over the 8 wavefronts of the work-group: the exchanges use work-group
-// barriers, double-buffered (one barrier per exchange).  The transforms are
-// the same, so are the values.
-template <typename C, int N, int IN, int OUT, int LPW, bool VREAL, bool INTEN = false, bool BAND = false>
+template <typename C, int N, int IN, int OUT, int LPW, bool VREAL, bool INTEN = false>
 __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
     using Re = typename real_of<C>::type;
     using PL = Plan<N>;
     constexpr int T = PL::T, E = N / T;
-    constexpr bool WAVE = !BAND && (64 % T) == 0;  // a line never straddles two wavefronts
-    static_assert(!BAND || (T == 64 && LPW == 8 && !VREAL && !INTEN), "band layout: 64-thread lines, one band");
+    constexpr bool WAVE = (64 % T) == 0;  // a line never straddles two wavefronts
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int lw = BAND ? (int)(threadIdx.x & 7) : (int)(threadIdx.x / T);
-    const int t = BAND ? (int)(8 * (threadIdx.x >> 6) + ((threadIdx.x & 63) >> 3)) : (int)(threadIdx.x % T);
+    const int lw = threadIdx.x / T, t = threadIdx.x % T;
     const int nxg = (p.nx + LPW - 1) / LPW;
     const int gx = blockIdx.x % nxg, chunk = blockIdx.x / nxg;
     const int ix_raw = gx * LPW + lw;
     const bool live = ix_raw < p.nx;
     const int ix = live ? ix_raw : p.nx - 1;  // dead rows compute, never store
     const int b0 = chunk * p.wpb, b1 = min(p.nb, b0 + p.wpb);
-    constexpr int LINE = N + N / 16;
-    const LdsView<C> lds[2] = {{smem + lw * LINE, 1, true}, {smem + ((BAND ? LPW : 0) + lw) * LINE, 1, true}};
+    const LdsView<C> lds[2] = {{smem + lw * (N + N / 16), 1, true}, {smem + lw * (N + N / 16), 1, true}};
     // element offset of (this line, y) in a wave's image
-    auto eo = [&](int y) -> unsigned {
-        if constexpr (BAND) return ((unsigned)gx * (unsigned)N + (unsigned)y) * 8u + (unsigned)lw;
-        else return (unsigned)ix * (unsigned)N + (unsigned)y;
-    };
+    auto eo = [&](int y) -> unsigned { return (unsigned)ix * (unsigned)N + (unsigned)y; };
 
     TwReg<PL, C> tw;
     if constexpr (IN != 0 || OUT != 0) load_twiddles<PL>(tw, t, p.tw);
@@ -279,7 +264,7 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
             }
         }
         if (p.trans_tab) tnext = p.trans_tab[b + 1 < b1 ? b + 1 : b];
-        if constexpr (IN != 0) fft_line<PL, IN, WAVE, BAND, 0>(a, t, lds, tw);
+        if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
         if (p.scale != (Re)1 && !fold) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = cscale(a[e], p.scale);
@@ -301,7 +286,7 @@ __global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
                 buf_store(v, ri, live ? (row0 + (unsigned)(t + e * T)) * (unsigned)sizeof(Re) : kBufDrop);
             }
         }
-        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE, BAND, (IN != 0 ? PL::P - 1 : 0)>(a, t, lds, tw);
+        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(a, t, lds, tw);
         {
             // buffer stores into wave b's image; dead rows, pruned elements
             // and a sweep that stores nothing aim past the bound (dropped):
@@ -367,7 +352,6 @@ template <typename C> struct ColArgs {
     // prune != 0: columns with ky2[iy] >= k2max are skipped entirely (their
     // k-space values are all masked to zero, so nothing downstream reads them)
     int prune;
-    int band;  // 1: psi (and out) stored in 8-row bands (BandLayout)
     // oblique cell (kColObl): x = fx / obl_asg, y = fy / obl_bsg (a sin gam,
     // b sin gam), s = x (x - obl_2c y) + y^2 (obl_2c = 2 cos gam), kept where
     // s < obl_smax with the factor exp(-i (obl_pl s - obl_tx x - obl_ty y))
@@ -433,33 +417,19 @@ template <typename C, int N, int TW, bool DB> constexpr size_t col_list_offset()
 //        of work-groups idles most of the chip; at each step the work-groups
 //        of one XCD hold consecutive groups of one wave, as in the chunked
 //        order.
-//
-// BAND: the waves are stored in 8-row bands, element (ix, iy) at
-// ((ix >> 3) * ny + iy) * 8 + (ix & 7) (BandLayout below), and lane l of
-// wavefront w holds column l >> 3 and row thread t = 8 w + (l & 7): each
-// load or store instruction then covers 8 columns x 8 rows of one band, one
-// contiguous 512-byte chunk, where the plain layout's instructions touch
-// 8 rows x 8 columns at the row stride (64-byte pieces).  The transforms and
-// their LDS exchanges are the same (they only see the column lw and the row
-// thread t), so the results are the same values.
-template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB, bool BAND = false>
+
+template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB>
 __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(ColArgs<C> p) {
     using Re = typename real_of<C>::type;
     using PL = ColPlan<N>;
     constexpr int T = PL::T, E = N / T;
     constexpr bool WAVE = false;  // a column is spread over several wavefronts
     static_assert(TW <= 32, "live-column masks are 32 bits");
-    static_assert(!BAND || (TW == 8 && T % 8 == 0), "band layout: 8-column tiles, rows in whole bands per thread");
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int lw = BAND ? (int)((threadIdx.x & 63) >> 3) : (int)(threadIdx.x % TW);
-    const int t = BAND ? (int)(8 * (threadIdx.x >> 6) + (threadIdx.x & 7)) : (int)(threadIdx.x / TW);
-    // element offset of (row t, column y) of an image (row t + e T adds e T ny
-    // in both layouts, since T is a multiple of 8 in the band layout)
-    auto elem = [&](int y) -> unsigned {
-        if constexpr (BAND) return ((unsigned)(t >> 3) * (unsigned)p.ny + (unsigned)y) * 8u + (unsigned)(t & 7);
-        else return (unsigned)t * (unsigned)p.ny + (unsigned)y;
-    };
+    const int lw = threadIdx.x % TW, t = threadIdx.x / TW;
+    // element offset of (row t, column y) of an image (row t + e T adds e T ny)
+    auto elem = [&](int y) -> unsigned { return (unsigned)t * (unsigned)p.ny + (unsigned)y; };
     const int groups = (p.ny + TW - 1) / TW;
     // XCD-aware order: work-groups are dispatched round-robin over the 8 XCDs;
     // renumber them so that consecutive logical groups (neighbouring columns,

@@ -64,7 +64,7 @@ class Probe(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("row_wpb", C.c_int32), ("col_wpb", C.c_int32), ("batch", C.c_int32), ("ms_per_slice", C.c_double),
-                ("col_tile", C.c_int32), ("band", C.c_int32)]
+                ("col_tile", C.c_int32)]
 
 
 class Job(C.Structure):
@@ -92,7 +92,8 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_wait_stream", "ms
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
            "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read",
-           "msx_musli_stem_write", "msx_musli_wave_write", "msx_potential_device_atoms"]
+           "msx_musli_stem_write", "msx_musli_wave_write", "msx_potential_device_atoms",
+           "msx_create_pots_rec_dt", "msx_create_pots_rs_dt"]
 
 _lib = None
 _lock = threading.Lock()
@@ -145,6 +146,8 @@ def load():
         "msx_potential_device_atoms": (i, [vp, vp, C.c_int64, d, d, d, i, i, i, d, i, i, vp, i, i]),
         "msx_transmission": (i, [vp, vp, i, i, i, d, d, d, d, vp, i, i]),
         "msx_multislice": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i, i]),
+        "msx_create_pots_rec_dt": (i, [i, i, d, vp, vp, d, vp]),
+        "msx_create_pots_rs_dt": (i, [i, i, d, vp, vp, vp]),
         "msx_stem": (i, [vp, C.POINTER(Stack), vp, C.POINTER(Probe), vp, i, C.POINTER(Detector), i, vp, i, d, i,
                          vp, i, i, i]),
         "msx_ctf_image": (i, [vp, vp, i, i, i, d, d, d, d, d, d, vp, vp, i, i]),
@@ -300,13 +303,12 @@ class Context:
         t = Tuning()
         _check(load().msx_autotune(self.h, nx, ny or nx, prec, max_batch, C.byref(t)))
         return dict(row_wpb=t.row_wpb, col_wpb=t.col_wpb, batch=t.batch, ms_per_slice=t.ms_per_slice,
-                    col_tile=t.col_tile, band=t.band)
+                    col_tile=t.col_tile)
 
     def set_tuning(self, nx, ny=None, prec=F32, tuning=None):
         """Install a saved autotuner result (dict as returned by autotune), or clear it (None)."""
         t = None if tuning is None else C.byref(Tuning(tuning["row_wpb"], tuning["col_wpb"], tuning["batch"],
-                                                       tuning.get("ms_per_slice", 0.0), tuning.get("col_tile", 0),
-                                                       tuning.get("band", 0)))
+                                                       tuning.get("ms_per_slice", 0.0), tuning.get("col_tile", 0)))
         _check(load().msx_set_tuning(self.h, nx, ny or nx, prec, t))
 
     def set_graphs(self, enable=True):
@@ -852,6 +854,28 @@ def displace(atoms, u_rms, seed, config, ctx=None, out=None):
           "out must be a C-contiguous ATOM_DTYPE array of the atoms' length")
     _check(load().msx_displace(ctx.h if ctx else None, C.c_void_p(at.ctypes.data), len(at),
                                C.c_void_p(u.ctypes.data), seed, config, C.c_void_p(out.ctypes.data), HOST))
+    return out
+
+
+def create_pots_rec_dt(types, dw, npoints, dds, vol):
+    """CreatePots_rec_DT_DW (Muslib/Curves.cpp:45-63): Doyle-Turner
+    reciprocal-space curves (len(types), npoints) at s = i dds, times vol."""
+    tp = np.ascontiguousarray(types, np.int32)
+    d = np.ascontiguousarray(np.broadcast_to(dw, tp.shape), np.float64)
+    out = np.zeros((len(tp), int(npoints)), np.float64)
+    _check(load().msx_create_pots_rec_dt(len(tp), int(npoints), float(dds), C.c_void_p(tp.ctypes.data),
+                                         C.c_void_p(d.ctypes.data), float(vol), C.c_void_p(out.ctypes.data)))
+    return out
+
+
+def create_pots_rs_dt(types, dw, npoints, dds):
+    """CreatePots_rs_DT_DW (Muslib/Curves.cpp:64-81): Doyle-Turner
+    real-space curves (len(types), npoints) at r = i dds."""
+    tp = np.ascontiguousarray(types, np.int32)
+    d = np.ascontiguousarray(np.broadcast_to(dw, tp.shape), np.float64)
+    out = np.zeros((len(tp), int(npoints)), np.float64)
+    _check(load().msx_create_pots_rs_dt(len(tp), int(npoints), float(dds), C.c_void_p(tp.ctypes.data),
+                                        C.c_void_p(d.ctypes.data), C.c_void_p(out.ctypes.data)))
     return out
 
 

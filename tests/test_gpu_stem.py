@@ -210,37 +210,6 @@ def test_displacement_host_equals_device(ctx):
     assert np.array_equal(dev["Z"], host["Z"])
 
 
-@pytest.mark.parametrize("prec", [msx.F32, msx.F64])
-def test_band_layout_equals_image_layout(prec):
-    """msx_stem at 512^2 with its wave batches in 8-row bands and in the image
-    layout: the same transforms on the same values, so the same detector
-    sums up to the order of the FP64 work-group reduction; two batches (a
-    full one and a partial one), three checkpoints, graphs on and off."""
-    at, cell = specimen.srtio3_001((12, 12, 2))
-    n, nsl = 512, 4
-    V = O.potential(at["Z"], np.stack([at["x"], at["y"], at["z"]], 1), cell, n, n, nsl, 3.0)
-    t = O.transmission(V, cell[0], cell[1], KEV)
-    if prec == msx.F32:
-        t = t.astype(np.complex64)
-    lam = msx.wavelength(KEV)
-    pos = np.random.default_rng(4).random((40, 2)) * np.array(cell[:2])
-    det = [(0.0, 0.010 / lam), (0.020 / lam, 0.050 / lam), (0.060 / lam, 0.2 / lam)]
-    cz, layer = np.full(nsl, 1.9525), np.arange(nsl)
-    probe = dict(alpha_max=0.021, cs=1e4, df=15.0)
-    out = {}
-    for band in (0, 1):
-        for graphs in (False, True):
-            c = msx.Context(0)
-            c.set_tuning(n, n, prec, dict(row_wpb=4, col_wpb=-1, batch=32, col_tile=8, band=band))
-            c.set_graphs(graphs)
-            out[band, graphs] = c.stem(t, cz, layer, cell[0], cell[1], KEV, probe, pos, det, [1, 3, nsl])
-            c.close()
-    ref = out[0, False]
-    for k, v in out.items():
-        assert np.abs(v - ref).max() <= 1e-13 * np.abs(ref).max(), k
-    assert np.array_equal(out[1, False], out[1, True])
-
-
 def test_stem_largest_grid(ctx):
     """The largest supported grid (2048^2: column tiles of 2 columns, the
     longest FFT plan) through the detector epilogue, against the oracle."""
