@@ -373,6 +373,28 @@ typedef struct msx_stack {
 MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
                            void* psi, int nwave, int prec, int where);
 
+/* Size of the fixed device array behind msx_multislice_noalloc. */
+#define MSX_NOALLOC_SCRATCH_BYTES (4u << 20)
+/* msx_multislice for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * No counterpart in the reference.  Device buffers only (trans, psi0, psi as
+ * msx_multislice with MSX_DEVICE); the call allocates nothing (no hipMalloc,
+ * no hipFree, no graph), launches the same kernels with the same tables as
+ * msx_multislice, and so returns the same exit waves.  The slice tables
+ * (freqn k^2 tables, the propagator factors of every layer) and the FFT
+ * twiddles go into one array of MSX_NOALLOC_SCRATCH_BYTES bytes per GPU, a
+ * static device array that each GPU the library runs on holds until the
+ * process ends.  It needs, with C the complex type of prec and A(b) = b
+ * rounded up to 256 bytes,
+ *   A(4 nx) + A(4 ny) + A(sizeof(C) nlayer nx) + A(sizeof(C) nlayer ny + 1)
+ *     + A(sizeof(C) nx) + A(sizeof(C) ny)  <=  MSX_NOALLOC_SCRATCH_BYTES
+ * (510 layers at 512^2 FP32, 126 at 2048^2 FP32, 62 at 2048^2 FP64);
+ * larger stacks are refused with MSX_ESIZE before anything runs.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * threads, contexts or streams) race.  The call copies its tables, waits for
+ * the copies, launches, and returns when the context stream has finished. */
+MSX_API int msx_multislice_noalloc(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
+                                   void* psi, int nwave, int prec);
 
 /* Many independent HRTEM jobs of uneven sizes and thicknesses in as few
  * launches as possible (the reference ran one job per process,

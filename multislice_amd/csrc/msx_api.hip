@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -147,8 +148,37 @@ float k2max_of(int nx, int ny, double ax, double by, double bw) {
 
 }  // namespace
 
+// The fixed device array of msx_multislice_noalloc: one copy per GPU, held
+// until the process ends.  Host code never uses this name as an address (in
+// host code it is only a stand-in for the symbol): scratch_address() looks
+// up each GPU's copy once, with that GPU current, and the entry point uses
+// that address.
+static __device__ __align__(256) unsigned char scratch[MSX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be the current device).
+int scratch_address(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kMaxDevices) return fail(MSX_EINVAL, "device %d: at most %d devices", device, kMaxDevices);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MSX_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return MSX_OK;
+}
+}  // namespace
+
 struct msx_ctx_s {
     int device = 0;
+    void* scratch = nullptr;  // this GPU's copy of `scratch` (msx_multislice_noalloc)
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     // kernel profiling (msx_profile): per kernel kind launch count and time
@@ -979,6 +1009,36 @@ int multislice_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const v
     return MSX_OK;
 }
 
+// msx_multislice on device buffers with the slice tables and the twiddles
+// in the GPU's `scratch` array: no allocation, no graph capture (a graph
+// instantiation may allocate); the same kernels and tables as msx_multislice.
+template <typename C>
+int multislice_noalloc_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const void* psi0, void* psi,
+                            int nwave) {
+    HostTables<C> h;
+    host_tables<C>(st, true, &h);
+    const size_t o_twx = h.tot, o_twy = o_twx + al256(sizeof(C) * st->nx),
+                 need = o_twy + al256(sizeof(C) * st->ny);
+    if (need > (size_t)MSX_NOALLOC_SCRATCH_BYTES)
+        return fail(MSX_ESIZE,
+                    "msx_multislice_noalloc: the tables of %d layers on a %d x %d grid take %zu bytes, more than "
+                    "the %zu-byte scratch array (MSX_NOALLOC_SCRATCH_BYTES)",
+                    st->nlayer, st->nx, st->ny, need, (size_t)MSX_NOALLOC_SCRATCH_BYTES);
+    if (!ctx->scratch) return fail(MSX_ESTATE, "msx_multislice_noalloc: no scratch array on device %d", ctx->device);
+    char* base = static_cast<char*>(ctx->scratch);
+    const std::vector<C> twx = make_twiddles<C>(st->nx), twy = make_twiddles<C>(st->ny);
+    HIPCHK(hipMemcpyAsync(base + o_twx, twx.data(), sizeof(C) * st->nx, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(base + o_twy, twy.data(), sizeof(C) * st->ny, hipMemcpyHostToDevice, ctx->stream));
+    SliceTables<C> tb;
+    CHK(upload_tables<C>(ctx, st, h, base, &tb));  // waits for every copy above
+    tb.twx = reinterpret_cast<const C*>(base + o_twx);
+    tb.twy = reinterpret_cast<const C*>(base + o_twy);
+    CHK(slice_loop_body<C>(ctx, st, tb, static_cast<const C*>(trans), static_cast<C*>(psi), nwave,
+                           static_cast<const C*>(psi0), 0, 0, nullptr));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
 template <typename C>
 int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_probe* pr, const double* pos,
               int nprobe, const msx_detector* det, int ndet, const int32_t* ck, int nck, double weight,
@@ -1784,6 +1844,10 @@ MSX_API int msx_create(int device, msx_ctx* out) {
         return fail(MSX_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     c->stream = c->own;
+    if (scratch_address(device, &c->scratch) != MSX_OK) {
+        delete c;
+        return MSX_EHIP;
+    }
     e = upload_fparams(c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
@@ -2234,6 +2298,17 @@ MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* tran
     HIPCHK(hipSetDevice(ctx->device));
     if (prec == MSX_F64) return multislice_impl<double2>(ctx, stack, trans, psi0, psi, nwave, where);
     return multislice_impl<float2>(ctx, stack, trans, psi0, psi, nwave, where);
+}
+
+MSX_API int msx_multislice_noalloc(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
+                                   void* psi, int nwave, int prec) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, MSX_DEVICE));
+    CHK(check_stack(stack));
+    if (!trans || !psi || nwave <= 0) return fail(MSX_EINVAL, "bad trans/psi/nwave");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (prec == MSX_F64) return multislice_noalloc_impl<double2>(ctx, stack, trans, psi0, psi, nwave);
+    return multislice_noalloc_impl<float2>(ctx, stack, trans, psi0, psi, nwave);
 }
 
 MSX_API int msx_stem(msx_ctx ctx, const msx_stack* stack, const void* trans, const msx_probe* probe, const double* pos,

@@ -92,7 +92,7 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_wait_stream", "ms
            "msx_unet_conv", "msx_potential_rec", "msx_transmission_abs", "msx_potential_rs",
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
            "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read",
-           "msx_musli_stem_write", "msx_musli_wave_write", "msx_potential_device_atoms",
+           "msx_musli_stem_write", "msx_musli_wave_write", "msx_potential_device_atoms", "msx_multislice_noalloc",
            "msx_create_pots_rec_dt", "msx_create_pots_rs_dt"]
 
 _lib = None
@@ -146,6 +146,7 @@ def load():
         "msx_potential_device_atoms": (i, [vp, vp, C.c_int64, d, d, d, i, i, i, d, i, i, vp, i, i]),
         "msx_transmission": (i, [vp, vp, i, i, i, d, d, d, d, vp, i, i]),
         "msx_multislice": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i, i]),
+        "msx_multislice_noalloc": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i]),
         "msx_create_pots_rec_dt": (i, [i, i, d, vp, vp, d, vp]),
         "msx_create_pots_rs_dt": (i, [i, i, d, vp, vp, vp]),
         "msx_stem": (i, [vp, C.POINTER(Stack), vp, C.POINTER(Probe), vp, i, C.POINTER(Detector), i, vp, i, d, i,
