@@ -373,6 +373,34 @@ typedef struct msx_stack {
 MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
                            void* psi, int nwave, int prec, int where);
 
+/* What msx_multislice_series records while the slice loop runs: the thickness
+ * series of mulslice.cpp's coherent loop (mulslice.cpp:682-737: the "(real,
+ * imag)" values of chosen beams after every slice, and the integrated
+ * intensity it prints after every slice) and exit waves at intermediate
+ * thicknesses (the checkpoints a run is continued from: mulslice.cpp:255-340
+ * "start from previous result", Musli's last.wav/IsContinue,
+ * MSProject.cs:423-440; pass such a wave back as psi0 to continue).  Every
+ * output lives where `where` says (host or device), C being the complex type
+ * of prec. */
+typedef struct msx_series {
+    const int32_t* at;  /* host [nat]: slice counts (1..nslice, strictly ascending) after which */
+    int32_t nat;        /*   the exit wave is kept */
+    void* waves;        /* [nat][nwave][nx][ny] C: exit waves after slice at[i] (NULL if nat = 0) */
+    const int32_t* hk;  /* host [nbeams][2]: beam indices (h, k) of kx and ky; as mulslice.cpp:693-700 */
+    int32_t nbeams;     /*   a negative index counts from the end (n + h), then clamps to [0, n-1] */
+    void* beams;        /* [nslice][nwave][nbeams] C: after slice s (propagated, before the inverse
+                         * transform) the wave's k-space value at (h, k) times 1/(nx ny), i.e. the
+                         * 2D transform (exponent +i) of the exit wave of slice s over nx ny;
+                         * exactly 0 outside the band limit of an orthogonal cell */
+    double* intensity;  /* [nslice][nwave] or NULL: sum |psi|^2 / (nx ny) of the exit wave of each
+                         * slice (mulslice.cpp:727-737) */
+} msx_series;
+/* msx_multislice that also fills `series` (maximum 4096 beams).  The exit
+ * waves psi are those msx_multislice returns, bit for bit; a kept wave at
+ * slice count nslice equals psi. */
+MSX_API int msx_multislice_series(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
+                                  void* psi, int nwave, int prec, int where, const msx_series* series);
+
 /* Size of the fixed device array behind msx_multislice_noalloc. */
 #define MSX_NOALLOC_SCRATCH_BYTES (4u << 20)
 /* msx_multislice for callers that cannot allocate device memory: code under

@@ -39,6 +39,7 @@
 #include <initializer_list>
 #include <iostream>
 #include <iterator>
+#include <limits>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -205,55 +206,112 @@ struct Image {
             m.store[i] = std::hypot(store[nChannels * i], nChannels > 1 ? store[nChannels * i + 1] : 0.0);
         return m;
     }
-    // <filename>.mrc of the first channel (Image::saveMRC, Image.cu:138-163 with
-    // MRC::save, MRC.cpp:206-210), cell lengths from the model; defined below.
-    template <class ModelT> int saveMRC(const char* filename, const ModelT* model, int w, int h, int thick) const;
+    // <filename>.mrc in an MRC mode (Image::saveMRC, Image.cu:138-163 with
+    // MRC::save, MRC.cpp:206-345), cell from the model; defined below.
+    template <class ModelT>
+    int saveMRC(const char* filename, const ModelT* model, int w, int h, int thick, int mode = 2) const;
 };
 
-// MRC2000 writer (mode 2, float data); the reference writes _pt.mrc and .mrc
-// through ImageSpace::MRC (MRC.cpp:206-345).  Returns 0 / -1.
-inline int write_mrc(const char* path, const double* v, int nx, int ny, int nz, float xlen, float ylen, float zlen) {
+// MRC data modes (ImageSpace::mrcMode, otherEnums.h:21-29) and their
+// channels per pixel (MRC::nChannels, MRC.h:108-113).
+enum mrcMode { mrc_BYTE = 0, mrc_SHORT = 1, mrc_FLOAT = 2, mrc_SHORT2 = 3, mrc_FLOAT2 = 4, mrc_INT = 6, mrc_CHAR3 = 16 };
+inline int mrc_channels(int mode) { return (mode == 3 || mode == 4) ? 2 : (mode == 16 ? 3 : 1); }
+
+namespace detail {
+// double -> integer element of an MRC mode: truncation toward zero as the
+// reference's C conversions (MRC.cpp:310-333), saturated at the type's range
+// (out of range the reference's conversion is undefined) and NaN -> 0
+template <typename T> T mrc_int(double x) {
+    if (!(x == x)) return 0;
+    const double lo = (double)std::numeric_limits<T>::min(), hi = (double)std::numeric_limits<T>::max();
+    return (T)std::trunc(std::min(std::max(x, lo), hi));
+}
+}  // namespace detail
+
+// MRC2000 writer (MRC::save, MRC.cpp:206-345): a 1024-byte header, then
+// nx*ny*nz pixels of mrc_channels(mode) values each, taken from v in order
+// (v holds nx*ny*nz*mrc_channels(mode) doubles) and stored as mode says:
+// 0 and 16 unsigned bytes, 1 and 3 signed shorts, 2 and 4 floats, 6 unsigned
+// shorts.  The reference writes nx*ny*nz*channels*4 bytes whatever the mode
+// (past the end of its converted buffer for modes 0, 1, 3 and 6, and from a
+// null buffer for mode 16, whose case lacks a break, MRC.cpp:304-307); here
+// each mode writes exactly its values.  Header: sizes, mode, cell lengths and
+// angles (alpha, beta, gamma; 90 when not given), axis order 1 2 3, the
+// minimum, maximum, mean and RMS deviation of the stored values, "MAP " and a
+// little-endian machine stamp.  Returns 0, or -1 for an unknown mode or a
+// failed write.
+inline int write_mrc(const char* path, const double* v, int nx, int ny, int nz, float xlen, float ylen, float zlen,
+                     int mode = mrc_FLOAT, float alpha = 90.f, float beta = 90.f, float gamma = 90.f) {
+    if (mode != 0 && mode != 1 && mode != 2 && mode != 3 && mode != 4 && mode != 6 && mode != 16) return -1;
+    const size_t n = (size_t)nx * ny * nz * mrc_channels(mode);
+    // the stored values, as doubles for the header statistics
+    std::vector<double> q(n);
+    std::vector<unsigned char> data;
+    auto put = [&](const void* p, size_t bytes) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        data.insert(data.end(), c, c + bytes);
+    };
+    data.reserve(n * 4);
+    for (size_t i = 0; i < n; i++) {
+        if (mode == 0 || mode == 16) {
+            const unsigned char x = detail::mrc_int<unsigned char>(v[i]);
+            put(&x, 1);
+            q[i] = x;
+        } else if (mode == 1 || mode == 3) {
+            const int16_t x = detail::mrc_int<int16_t>(v[i]);
+            put(&x, 2);
+            q[i] = x;
+        } else if (mode == 6) {
+            const uint16_t x = detail::mrc_int<uint16_t>(v[i]);
+            put(&x, 2);
+            q[i] = x;
+        } else {
+            const float x = (float)v[i];
+            put(&x, 4);
+            q[i] = x;
+        }
+    }
     FILE* f = std::fopen(path, "wb");
     if (!f) return -1;
     unsigned char h[1024];
     std::memset(h, 0, sizeof h);
     auto put_i = [&](int word, int32_t x) { std::memcpy(h + 4 * word, &x, 4); };
     auto put_f = [&](int word, float x) { std::memcpy(h + 4 * word, &x, 4); };
-    const size_t n = (size_t)nx * ny * nz;
-    double mn = n ? v[0] : 0, mx = mn, sum = 0;
+    double mn = n ? q[0] : 0, mx = mn, sum = 0;
     for (size_t i = 0; i < n; i++) {
-        mn = std::min(mn, v[i]);
-        mx = std::max(mx, v[i]);
-        sum += v[i];
+        mn = std::min(mn, q[i]);
+        mx = std::max(mx, q[i]);
+        sum += q[i];
     }
     const double mean = n ? sum / n : 0;
     double var = 0;
-    for (size_t i = 0; i < n; i++) var += (v[i] - mean) * (v[i] - mean);
-    put_i(0, nx); put_i(1, ny); put_i(2, nz); put_i(3, 2);
+    for (size_t i = 0; i < n; i++) var += (q[i] - mean) * (q[i] - mean);
+    put_i(0, nx); put_i(1, ny); put_i(2, nz); put_i(3, mode);
     put_i(7, nx); put_i(8, ny); put_i(9, nz);
     put_f(10, xlen); put_f(11, ylen); put_f(12, zlen);
-    put_f(13, 90.f); put_f(14, 90.f); put_f(15, 90.f);
+    put_f(13, alpha); put_f(14, beta); put_f(15, gamma);
     put_i(16, 1); put_i(17, 2); put_i(18, 3);
     put_f(19, (float)mn); put_f(20, (float)mx); put_f(21, (float)mean);
     std::memcpy(h + 208, "MAP ", 4);
     h[212] = 0x44; h[213] = 0x44;  // little-endian machine stamp
     put_f(54, (float)std::sqrt(n ? var / n : 0));
     bool ok = std::fwrite(h, 1, 1024, f) == 1024;
-    std::vector<float> buf(n);
-    for (size_t i = 0; i < n; i++) buf[i] = (float)v[i];
-    ok = ok && std::fwrite(buf.data(), sizeof(float), n, f) == n;
+    ok = ok && (data.empty() || std::fwrite(data.data(), 1, data.size(), f) == data.size());
     return (std::fclose(f) == 0 && ok) ? 0 : -1;
 }
 
 template <class ModelT>
-int Image::saveMRC(const char* filename, const ModelT* model, int w, int h, int thick) const {
-    const size_t n = (size_t)w * h * thick;
-    if (n > (size_t)width * height * thickness) return -1;
-    std::vector<double> v(n);
-    for (size_t i = 0; i < n; i++) v[i] = store[(size_t)nChannels * i];
+int Image::saveMRC(const char* filename, const ModelT* model, int w, int h, int thick, int mode) const {
+    // MRC::save reads the image's interleaved values in order, mrc_channels(mode)
+    // per pixel (MRC.cpp:308-333 over pData = imageData, Image.cu:141-158): for
+    // an image with as many channels as the mode, pixel by pixel
+    const size_t n = (size_t)w * h * thick * mrc_channels(mode);
+    if (w < 0 || h < 0 || thick < 0 || n > (size_t)width * height * thickness * nChannels) return -1;
+    const std::vector<double> v(store.begin(), store.begin() + (long)n);
     const std::string p = std::string(filename) + ".mrc";
     return write_mrc(p.c_str(), v.data(), w, h, thick, model ? model->getA() : 0.f, model ? model->getB() : 0.f,
-                     model ? model->getC() : 0.f);
+                     model ? model->getC() : 0.f, mode, model ? model->getAlpha() : 90.f,
+                     model ? model->getBeta() : 90.f, model ? model->getGamma() : 90.f);
 }
 
 // ----------------------------------------------------------------- Microscope

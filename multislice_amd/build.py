@@ -55,7 +55,7 @@ def build(jobs=None, verbose=True, defines=(), out=None, bdir=None):
         units.append((os.path.join(CSRC, "msx_launch.hip"), os.path.join(bdir, f"launch_{n}.o"),
                       [f"-DMSX_N={n}"]))
     for name in ("msx_potential", "msx_api", "msx_unet", "msx_unet_kernels", "msx_specimen", "msx_formats",
-                 "msx_binning", "msx_lms"):
+                 "msx_binning", "msx_lms", "msx_series"):
         units.append((os.path.join(CSRC, name + ".hip"), os.path.join(bdir, name + ".o"), []))
     jobs = jobs or min(8, os.cpu_count() or 4)
     with cf.ThreadPoolExecutor(jobs) as ex:

@@ -209,7 +209,7 @@ struct msx_ctx_s {
         return MSX_OK;
     }
     std::map<int, DevBuf> tw32, tw64;
-    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab, binbuf;
+    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab, binbuf, series;
     // host side of the potential's atom binning, kept across calls (no page
     // faults on reuse); the sorted records go to pinned memory for the upload
     std::vector<long long> h_cell;
@@ -296,7 +296,7 @@ struct msx_ctx_s {
         for (auto& kv : tw64) kv.second.release();
         drop_graphs();
         for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf, &radtab,
-                           &binbuf})
+                           &binbuf, &series})
             b->release();
         for (Lane& l : lanes) {
             l.wave.release();
@@ -536,6 +536,56 @@ struct DetPlan {
     long long nprobe = 0, probe0 = 0;
 };
 
+// Device side of a thickness series (msx_multislice_series): what is
+// recorded after each slice, read from the batch between the column sweep
+// and the next row sweep.
+template <typename C> struct SeriesPlan {
+    const int32_t* at = nullptr;  // host: slice counts of the kept waves, ascending
+    int nat = 0, iat = 0;         // count, next one to keep
+    C* waves = nullptr;           // [nat][nb][nx][ny]
+    const int4* bm = nullptr;     // [nbeams] (h, k, in band, 0)
+    int nbeams = 0;
+    C* beams = nullptr;           // [nslice][nb][nbeams]
+    double* inten = nullptr;      // [nslice][nb], or nullptr
+    double* partial = nullptr;    // [nb][nparts]
+    int nparts = 1;
+};
+
+// The records of slice s (0-based): psi holds IFFT_x(P FFT_x FFT_y(t psi_s)),
+// nx times the y transform of the exit wave of slice s.
+template <typename C>
+int series_records(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, C* psi, int nb, int s, const C* twy,
+                   bool obl, SeriesPlan<C>* ser) {
+    using Re = typename real_of<C>::type;
+    const int nx = st->nx, ny = st->ny;
+    const long long img = (long long)nx * ny;
+    const double dnx = nx, dny = ny;
+    if (ser->nbeams > 0)
+        HIPCHK(launch_series_beams<C>(psi, nb, nx, ny, ser->bm, ser->nbeams, 1.0 / (dnx * dnx * dny),
+                                      ser->beams + (long long)s * nb * ser->nbeams, ctx->stream));
+    if (ser->inten) {
+        // unwritten columns (ky2 >= k2max) are skipped unless the cell is
+        // oblique (then every column was written)
+        HIPCHK(launch_series_norm<C>(psi, nb, img, ny, obl ? nullptr : tb.ky2, tb.k2max, ser->nparts, ser->partial,
+                                     ctx->stream));
+        HIPCHK(launch_det_finish(ser->partial, 1, nb, ser->nparts, 1.0 / (dnx * dnx * dnx * dny * dny),
+                                 ser->inten + (long long)s * nb, 0, 0, ctx->stream));
+    }
+    if (ser->iat < ser->nat && ser->at[ser->iat] == s + 1) {
+        // the loop's final inverse row sweep, into the kept wave instead of psi
+        RowArgs<C> r = row_args<C>(ser->waves + (long long)ser->iat * nb * img, nx, nb, twy);
+        r.src = psi;
+        r.src_bstride = img;
+        r.scale = (Re)1 / ((Re)nx * (Re)ny);
+        r.ky2 = tb.ky2;
+        r.k2max = tb.k2max;
+        r.prune = obl ? 0 : 1;
+        LAUNCH_ROW(ny, kRowInv, r);
+        ser->iat++;
+    }
+    return MSX_OK;
+}
+
 // Advances nb waves through the stack.  start: 0 = psi holds real-space waves
 // (or src is a shared incident wave), 1 = psi holds k-space probes already
 // through the column inverse (the first row sweep starts with IFFT_y).
@@ -543,10 +593,12 @@ struct DetPlan {
 // it ends with the final inverse row sweep (exit waves in real space), or,
 // with mixed_out, just before it: psi then holds IFFT_x(P FFT_x FFT_y(t psi))
 // of the last slice, i.e. FFT_y of the exit wave times nx, in the columns
-// ky2 < k2max (the others are not written).
+// ky2 < k2max (the others are not written).  ser: thickness-series records
+// after every slice (no graph capture: never set on the keyed path).
 template <typename C>
 int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, const C* trans, C* psi, int nb,
-                    const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false) {
+                    const C* src, long long src_bstride, int start, DetPlan* det, bool mixed_out = false,
+                    SeriesPlan<C>* ser = nullptr) {
     using Re = typename real_of<C>::type;
     const int nx = st->nx, ny = st->ny;
     const long long img = (long long)nx * ny;
@@ -637,6 +689,7 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
             ick++;
         }
         if (last) return MSX_OK;
+        if (ser) CHK(series_records<C>(ctx, st, tb, psi, nb, s, twy, obl, ser));
     }
     if (!det && !mixed_out) {
         RowArgs<C> r = row_args<C>(psi, nx, nb, twy);
@@ -1005,6 +1058,88 @@ int multislice_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const v
     CHK(build_tables<C>(ctx, st, true, &tb));
     CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nwave, s_dev, 0, 0, nullptr));
     if (where == MSX_HOST) HIPCHK(hipMemcpyAsync(psi, w_dev, wbytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return MSX_OK;
+}
+
+// msx_multislice with the records of a thickness series.  One device block
+// (ctx->series) holds the beam table, the intensity partial sums and, for
+// host outputs, the device copies of the kept waves, beams and intensities.
+template <typename C>
+int multislice_series_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const void* psi0, void* psi,
+                           int nwave, int where, const msx_series* se) {
+    const int nx = st->nx, ny = st->ny;
+    const long long img = (long long)nx * ny;
+    const size_t tbytes = sizeof(C) * img * st->nlayer, wbytes = sizeof(C) * img * nwave;
+    const bool obl = is_oblique(st);
+    // beams: mulslice.cpp:693-700 (wrap a negative index once, then clamp),
+    // flagged out of band with the column sweep's float test; an oblique
+    // cell's band is the FP64 metric disk: every beam is transformed there
+    const std::vector<float> kx2 = k2_table(nx, st->ax), ky2 = k2_table(ny, st->by);
+    const float k2max = k2max_of(nx, ny, st->ax, st->by, st->bw);
+    std::vector<int4> bm((size_t)std::max(se->nbeams, 1));
+    for (int i = 0; i < se->nbeams; i++) {
+        int h = se->hk[2 * i], k = se->hk[2 * i + 1];
+        if (h < 0) h += nx;
+        if (k < 0) k += ny;
+        h = std::min(std::max(h, 0), nx - 1);
+        k = std::min(std::max(k, 0), ny - 1);
+        bm[i] = make_int4(h, k, (obl || kx2[h] + ky2[k] < k2max) ? 1 : 0, 0);
+    }
+    const int nparts =
+        se->intensity ? (int)std::max(1LL, std::min<long long>(std::min(1024 / nwave, 1024), img / 4096)) : 1;
+    const bool host = where == MSX_HOST;
+    const size_t o_bm = 0, o_part = al256(sizeof(int4) * bm.size()),
+                 o_wav = o_part + al256(sizeof(double) * nwave * nparts),
+                 o_bea = o_wav + (host ? al256(sizeof(C) * img * nwave * se->nat) : 0),
+                 o_int = o_bea + (host ? al256(sizeof(C) * (size_t)st->nslice * nwave * se->nbeams) : 0),
+                 tot = o_int + (host && se->intensity ? al256(sizeof(double) * st->nslice * nwave) : 0);
+    CHK(ctx->series.ensure(tot));
+    char* base = static_cast<char*>(ctx->series.p);
+    HIPCHK(hipMemcpyAsync(base + o_bm, bm.data(), sizeof(int4) * bm.size(), hipMemcpyHostToDevice, ctx->stream));
+    SeriesPlan<C> sp;
+    sp.at = se->at;
+    sp.nat = se->nat;
+    sp.bm = reinterpret_cast<const int4*>(base + o_bm);
+    sp.nbeams = se->nbeams;
+    sp.partial = reinterpret_cast<double*>(base + o_part);
+    sp.nparts = nparts;
+    sp.waves = static_cast<C*>(host ? (void*)(base + o_wav) : se->waves);
+    sp.beams = static_cast<C*>(host ? (void*)(base + o_bea) : se->beams);
+    sp.inten = se->intensity ? (host ? reinterpret_cast<double*>(base + o_int) : se->intensity) : nullptr;
+
+    const C* t_dev = static_cast<const C*>(trans);
+    C* w_dev = static_cast<C*>(psi);
+    const C* s_dev = static_cast<const C*>(psi0);
+    if (host) {
+        CHK(ctx->trans.ensure(tbytes));
+        CHK(ctx->wave.ensure(wbytes));
+        HIPCHK(hipMemcpyAsync(ctx->trans.p, trans, tbytes, hipMemcpyHostToDevice, ctx->stream));
+        t_dev = static_cast<const C*>(ctx->trans.p);
+        w_dev = static_cast<C*>(ctx->wave.p);
+        if (psi0) {
+            CHK(ctx->aux.ensure(sizeof(C) * img));
+            HIPCHK(hipMemcpyAsync(ctx->aux.p, psi0, sizeof(C) * img, hipMemcpyHostToDevice, ctx->stream));
+            s_dev = static_cast<const C*>(ctx->aux.p);
+        } else {
+            HIPCHK(hipMemcpyAsync(w_dev, psi, wbytes, hipMemcpyHostToDevice, ctx->stream));
+        }
+    }
+    SliceTables<C> tb;
+    CHK(build_tables<C>(ctx, st, true, &tb));  // waits for the copies above (bm outlives them)
+    CHK(slice_loop_body<C>(ctx, st, tb, t_dev, w_dev, nwave, s_dev, 0, 0, nullptr, false, &sp));
+    if (host) {
+        HIPCHK(hipMemcpyAsync(psi, w_dev, wbytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (se->nat > 0)
+            HIPCHK(hipMemcpyAsync(se->waves, sp.waves, sizeof(C) * img * nwave * se->nat, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        if (se->nbeams > 0)
+            HIPCHK(hipMemcpyAsync(se->beams, sp.beams, sizeof(C) * (size_t)st->nslice * nwave * se->nbeams,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        if (se->intensity)
+            HIPCHK(hipMemcpyAsync(se->intensity, sp.inten, sizeof(double) * st->nslice * nwave,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return MSX_OK;
 }
@@ -2298,6 +2433,34 @@ MSX_API int msx_multislice(msx_ctx ctx, const msx_stack* stack, const void* tran
     HIPCHK(hipSetDevice(ctx->device));
     if (prec == MSX_F64) return multislice_impl<double2>(ctx, stack, trans, psi0, psi, nwave, where);
     return multislice_impl<float2>(ctx, stack, trans, psi0, psi, nwave, where);
+}
+
+MSX_API int msx_multislice_series(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,
+                                  void* psi, int nwave, int prec, int where, const msx_series* series) {
+    if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
+    CHK(check_prec_where(prec, where));
+    CHK(check_stack(stack));
+    if (!trans || !psi || nwave <= 0) return fail(MSX_EINVAL, "bad trans/psi/nwave");
+    if (nwave > 65535) return fail(MSX_EINVAL, "msx_multislice_series: at most 65535 waves (%d)", nwave);
+    if (!series) return fail(MSX_EINVAL, "msx_multislice_series: series is NULL");
+    const msx_series* se = series;
+    if (se->nat < 0 || se->nbeams < 0 || se->nbeams > 4096)
+        return fail(MSX_EINVAL, "msx_multislice_series: nat %d, nbeams %d (0..4096)", se->nat, se->nbeams);
+    if (se->nat > 0) {
+        if (!se->at || !se->waves) return fail(MSX_EINVAL, "msx_multislice_series: nat > 0 needs at and waves");
+        for (int i = 0; i < se->nat; i++) {
+            const int a = se->at[i];
+            if (a < 1 || a > stack->nslice || (i > 0 && a <= se->at[i - 1]))
+                return fail(MSX_EINVAL,
+                            "msx_multislice_series: at[%d] = %d: slice counts must ascend strictly within 1..%d", i,
+                            a, stack->nslice);
+        }
+    }
+    if (se->nbeams > 0 && (!se->hk || !se->beams))
+        return fail(MSX_EINVAL, "msx_multislice_series: nbeams > 0 needs hk and beams");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (prec == MSX_F64) return multislice_series_impl<double2>(ctx, stack, trans, psi0, psi, nwave, where, se);
+    return multislice_series_impl<float2>(ctx, stack, trans, psi0, psi, nwave, where, se);
 }
 
 MSX_API int msx_multislice_noalloc(msx_ctx ctx, const msx_stack* stack, const void* trans, const void* psi0,

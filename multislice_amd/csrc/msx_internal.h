@@ -242,6 +242,18 @@ hipError_t launch_tilt_waves(C* psi, int nb, int nx, int ny, const int2* iq, hip
 template <typename Re>
 hipError_t launch_wave_sum(const Re* inten, int nb, long long img, double scale, double* image, hipStream_t s);
 
+// thickness-series records of the mixed-domain batch m after a column sweep
+// (msx_series.hip): beams out[b][i] = scale * sum_x m[b][x][k_i]
+// exp(+2 pi i h_i x / nx) for bm[i] = (h_i, k_i, in band, 0) (0 when out of
+// band); per-wave partial sums of |m|^2 (columns with ky2 >= k2max skipped
+// when ky2 is set) into partial[nb][nparts]
+template <typename C>
+hipError_t launch_series_beams(const C* m, int nb, int nx, int ny, const int4* bm, int nbeams, double scale, C* out,
+                               hipStream_t s);
+template <typename C>
+hipError_t launch_series_norm(const C* m, int nb, long long img, int ny, const float* ky2, float k2max, int nparts,
+                              double* partial, hipStream_t s);
+
 // error reporting shared by the C ABI units: records the message for
 // msx_last_error() and returns `code`
 int fail(int code, const char* fmt, ...);

@@ -58,6 +58,11 @@ class Stack(C.Structure):
                 ("layer", C.POINTER(C.c_int32)), ("gam", C.c_double)]
 
 
+class Series(C.Structure):
+    _fields_ = [("at", C.c_void_p), ("nat", C.c_int32), ("waves", C.c_void_p), ("hk", C.c_void_p),
+                ("nbeams", C.c_int32), ("beams", C.c_void_p), ("intensity", C.c_void_p)]
+
+
 class Probe(C.Structure):
     _fields_ = [("alpha_max", C.c_double), ("alpha_min", C.c_double), ("cs", C.c_double), ("df", C.c_double)]
 
@@ -93,7 +98,7 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_wait_stream", "ms
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
            "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read",
            "msx_musli_stem_write", "msx_musli_wave_write", "msx_potential_device_atoms", "msx_multislice_noalloc",
-           "msx_create_pots_rec_dt", "msx_create_pots_rs_dt"]
+           "msx_create_pots_rec_dt", "msx_create_pots_rs_dt", "msx_multislice_series"]
 
 _lib = None
 _lock = threading.Lock()
@@ -147,6 +152,7 @@ def load():
         "msx_transmission": (i, [vp, vp, i, i, i, d, d, d, d, vp, i, i]),
         "msx_multislice": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i, i]),
         "msx_multislice_noalloc": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i]),
+        "msx_multislice_series": (i, [vp, C.POINTER(Stack), vp, vp, vp, i, i, i, C.POINTER(Series)]),
         "msx_create_pots_rec_dt": (i, [i, i, d, vp, vp, d, vp]),
         "msx_create_pots_rs_dt": (i, [i, i, d, vp, vp, vp]),
         "msx_stem": (i, [vp, C.POINTER(Stack), vp, C.POINTER(Probe), vp, i, C.POINTER(Detector), i, vp, i, d, i,
@@ -502,6 +508,58 @@ class Context:
         self._order(trans, psi, psi0)
         _check(load().msx_multislice(self.h, C.byref(st), pt[0], p0, pw[0], nwave, prec, where))
         return psi
+
+    def multislice_series(self, trans, cz, layer, psi, ax, by, kev, at=(), beams=(), intensity=False,
+                          tilt=(0.0, 0.0), bw=2.0 / 3.0, psi0=None, gam=0.0):
+        """`multislice` (psi in place, same exit waves) recording a thickness
+        series (msx_multislice_series; mulslice.cpp:682-737): returns
+        (waves, beam_values, intensity) with waves (len(at), nwave, nx, ny) the
+        exit waves after the slice counts `at`, beam_values (nslice, nwave,
+        len(beams)) the k-space values at the (h, k) of `beams` over nx*ny
+        after every slice, intensity (nslice, nwave) sum |psi|^2/(nx*ny) after
+        every slice (None unless asked).  Outputs are numpy arrays for host
+        psi, CUDA tensors for device psi."""
+        prec = _prec_of(trans)
+        nl, nx, ny = _check_trans(trans)
+        _check_layers(nl, cz, layer)
+        _check_waves("psi", psi, nx, ny)
+        _need(psi0 is None or _numel(psi0) == nx * ny, f"psi0 must be one ({nx}, {ny}) image")
+        at = np.ascontiguousarray(np.asarray(at, np.int32).reshape(-1))
+        hk = np.ascontiguousarray(np.asarray(beams, np.int32).reshape(-1, 2))
+        ns = len(layer)
+        _need(np.all((at >= 1) & (at <= ns)) and np.all(np.diff(at) > 0),
+              f"at must ascend strictly within 1..{ns}")
+        _need(len(hk) <= 4096, "at most 4096 beams")
+        nwave = int(np.prod(psi.shape[:-2])) if len(psi.shape) > 2 else 1
+        st, keep = self._stack(nx, ny, ax, by, kev, cz, layer, tilt, bw, gam)
+        pt = _buf(trans, prec)
+        pw = _buf(psi, prec, writable=True)
+        bufs = [pt, pw]
+        p0 = None
+        if psi0 is not None:
+            b0 = _buf(psi0, prec)
+            bufs.append(b0)
+            p0 = b0[0]
+        where = _ptr_where(bufs)
+        cdt = np.complex64 if prec == F32 else np.complex128
+        if where == DEVICE:
+            import torch
+            dev = psi.device
+            tdt = torch.complex64 if prec == F32 else torch.complex128
+            waves = torch.empty((len(at), nwave, nx, ny), dtype=tdt, device=dev)
+            bv = torch.empty((ns, nwave, len(hk)), dtype=tdt, device=dev)
+            inten = torch.empty((ns, nwave), dtype=torch.float64, device=dev) if intensity else None
+            ptr = lambda x: x.data_ptr() if x is not None and x.numel() else None
+        else:
+            waves = np.empty((len(at), nwave, nx, ny), cdt)
+            bv = np.empty((ns, nwave, len(hk)), cdt)
+            inten = np.empty((ns, nwave), np.float64) if intensity else None
+            ptr = lambda x: x.ctypes.data if x is not None and x.size else None
+        se = Series(at.ctypes.data if len(at) else None, len(at), ptr(waves), hk.ctypes.data if len(hk) else None,
+                    len(hk), ptr(bv), ptr(inten))
+        self._order(trans, psi, psi0)
+        _check(load().msx_multislice_series(self.h, C.byref(st), pt[0], p0, pw[0], nwave, prec, where, C.byref(se)))
+        return waves, bv, inten
 
     def multislice_batch(self, jobs):
         """Uneven HRTEM jobs in shared launches (msx_multislice_batch).  jobs:

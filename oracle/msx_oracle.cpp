@@ -594,9 +594,13 @@ ORC_API int orc_multislice_oblique_f64(const double *trans, int nlayer, const do
     return 0;
 }
 
-ORC_API int orc_multislice_f64(const double *trans, int nlayer, const double *cz, const int *layer,
-                               int nslice, int nx, int ny, double ax, double by, double kev,
-                               double ctiltx, double ctilty, double bw, double *wave, double *intensity) {
+/* FP64 slice loop with optional beam records (mulslice.cpp:682-724): after
+ * propagate, before the inverse transform, beams[is][ib] = (1/(nx ny)) *
+ * wave(h, k) for the beam indices hk[ib] = (h, k), wrapped once when negative
+ * and clamped to the grid as mulslice.cpp:693-700 does. */
+static int multislice_f64(const double *trans, int nlayer, const double *cz, const int *layer, int nslice, int nx,
+                          int ny, double ax, double by, double kev, double ctiltx, double ctilty, double bw,
+                          double *wave, double *intensity, const int *hk, int nbeams, double *beams) {
     Freq f(nx, ny, ax, by);
     float k2max = k2max_of(nx, ny, (float)ax, (float)by, (float)bw);
     double wavlen = wavelength(kev);
@@ -618,6 +622,16 @@ ORC_API int orc_multislice_f64(const double *trans, int nlayer, const double *cz
         for (int iy = 0; iy < ny; iy++) fxp.run(w + iy, ny);
         propagate(w, &pxr[(size_t)l * nx], &pxi[(size_t)l * nx], &pyr[(size_t)l * ny], &pyi[(size_t)l * ny],
                   f.kx2.data(), f.ky2.data(), k2max, nx, ny);
+        for (int ib = 0; ib < nbeams; ib++) {
+            int h = hk[2 * ib], k = hk[2 * ib + 1];
+            if (h < 0) h = nx + h;
+            if (k < 0) k = ny + k;
+            h = std::min(std::max(h, 0), nx - 1);
+            k = std::min(std::max(k, 0), ny - 1);
+            const std::complex<double> v = w[(size_t)h * ny + k] / ((double)nx * ny);
+            beams[2 * ((size_t)is * nbeams + ib)] = v.real();
+            beams[2 * ((size_t)is * nbeams + ib) + 1] = v.imag();
+        }
         for (int ix = 0; ix < nx; ix++) fym.run(w + (size_t)ix * ny, 1);
         for (int iy = 0; iy < ny; iy++) fxm.run(w + iy, ny);
         scale_inv(w, nx, ny);
@@ -628,6 +642,24 @@ ORC_API int orc_multislice_f64(const double *trans, int nlayer, const double *cz
         }
     }
     return 0;
+}
+
+ORC_API int orc_multislice_f64(const double *trans, int nlayer, const double *cz, const int *layer,
+                               int nslice, int nx, int ny, double ax, double by, double kev,
+                               double ctiltx, double ctilty, double bw, double *wave, double *intensity) {
+    return multislice_f64(trans, nlayer, cz, layer, nslice, nx, ny, ax, by, kev, ctiltx, ctilty, bw, wave, intensity,
+                          nullptr, 0, nullptr);
+}
+
+/* The thickness series of the coherent loop (mulslice.cpp:682-737): beams
+ * [nslice][nbeams] complex as above, intensity[nslice] = sum |wave|^2 (the
+ * reference prints it times 1/(nx ny), :735). */
+ORC_API int orc_multislice_series_f64(const double *trans, int nlayer, const double *cz, const int *layer,
+                                      int nslice, int nx, int ny, double ax, double by, double kev, double ctiltx,
+                                      double ctilty, double bw, double *wave, double *intensity, const int *hk,
+                                      int nbeams, double *beams) {
+    return multislice_f64(trans, nlayer, cz, layer, nslice, nx, ny, ax, by, kev, ctiltx, ctilty, bw, wave, intensity,
+                          hk, nbeams, beams);
 }
 
 /* STEM probe, after Musli ConvergedBeam + Xi (MSProject.cs:458-514) in

@@ -45,6 +45,8 @@ def lib():
             ("orc_multislice_f32", i, [_fp, i, _fp, _ip, i, i, i, d, d, d, d, d, d, _fp, C.c_void_p]),
             ("orc_multislice_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, _dp, C.c_void_p]),
             ("orc_multislice_oblique_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, d, _dp]),
+            ("orc_multislice_series_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, _dp, C.c_void_p, _ip, i,
+                                              _dp]),
             ("orc_probe_f64", i, [i, i, d, d, d, d, d, d, d, d, d, _dp]),
             ("orc_stem_f64", i, [_dp, i, _dp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
             ("orc_stem_f32", i, [_fp, i, _fp, _ip, i, i, i, d, d, d, d, d, d, d, d, _dp, i, _dp, i, _ip, i, _dp, i]),
@@ -228,6 +230,27 @@ class Oracle:
         if rc != 0:
             raise ValueError("orc_multislice failed %d" % rc)
         return (w, inten) if want_intensity else w
+
+    @staticmethod
+    def multislice_series(trans, cz, layer, wave, ax, by, kev, hk, tilt=(0.0, 0.0), bw=2.0 / 3.0):
+        """FP64 slice loop recording, after every slice, the beams hk [(h, k)]
+        (mulslice.cpp:682-724) and sum |wave|^2; returns (exit wave, beams
+        [nslice][nbeams] complex, intensity [nslice])."""
+        trans = np.asarray(trans)
+        nl, nx, ny = trans.shape
+        layer = np.ascontiguousarray(layer, np.int32)
+        hk = np.ascontiguousarray(np.asarray(hk, np.int32).reshape(-1, 2))
+        beams = np.zeros((len(layer), len(hk)), np.complex128)
+        inten = np.zeros(len(layer), np.float64)
+        w = _c128(wave).copy()
+        rc = lib().orc_multislice_series_f64(_c128(trans).view(np.float64).reshape(-1), nl,
+                                             np.ascontiguousarray(cz, np.float64), layer, len(layer),
+                                             nx, ny, ax, by, kev, tilt[0], tilt[1], bw,
+                                             w.view(np.float64).reshape(-1), inten.ctypes.data, hk.reshape(-1),
+                                             len(hk), beams.view(np.float64).reshape(-1))
+        if rc != 0:
+            raise ValueError("orc_multislice_series failed %d" % rc)
+        return w, beams, inten
 
     @staticmethod
     def multislice_oblique(trans, cz, layer, wave, ax, by, gam, kev, tilt=(0.0, 0.0), bw=2.0 / 3.0):

@@ -42,12 +42,39 @@ static int surface() {
     return 0;
 }
 
+// Writes <prefix>_m<mode>.mrc for every MRC mode (MRC.cpp:206-345) from
+// images of the mode's channel count (1, 2 or 3) whose values cover
+// fractions, negatives, values beyond each integer type and large floats;
+// tests/test_lms_surface.py reads them back.  Needs no GPU.
+static int mrc_modes(const std::string& out) {
+    msx::lms::amodel::Model model(7.5f, 6.25f, 3.5f);
+    model.setAlpha(90.f);
+    model.setBeta(95.5f);
+    model.setGamma(120.f);
+    const int modes[] = {0, 1, 2, 3, 4, 6, 16};
+    for (int mode : modes) {
+        const int ch = msx::lms::mrc_channels(mode);
+        msx::lms::Image img(5, 3, 2, 8, ch);
+        double* d = img.data();
+        for (int i = 0; i < 5 * 3 * 2 * ch; i++) d[i] = (i % 7 - 3) * 37.25 + (i % 5) * 1000.5 * ((i % 3) - 1) + 0.75;
+        const std::string p = out + "_m" + std::to_string(mode);
+        if (img.saveMRC(p.c_str(), &model, 5, 3, 2, mode) != 0) return 1;
+    }
+    // an unknown mode and an image too small for its mode are refused
+    msx::lms::Image one(5, 3, 2, 8, 1);
+    if (one.saveMRC((out + "_bad").c_str(), &model, 5, 3, 2, 5) != -1) return 1;
+    if (one.saveMRC((out + "_bad").c_str(), &model, 5, 3, 2, 4) != -1) return 1;
+    std::printf("mrc modes written\n");
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) {
         std::fprintf(stderr, "usage: lms_check <output prefix> | --surface\n");
         return 2;
     }
     if (std::string(argv[1]) == "--surface") return surface();
+    if (std::string(argv[1]) == "--mrc" && argc > 2) return mrc_modes(argv[2]);
     const std::string out = argv[1];
     // Si [001], 2x2x1 conventional cells, fractional coordinates, planes at slice centres
     const float a = 5.431f;

@@ -82,7 +82,7 @@ def test_structs_match_header_layout(tmp_path):
     """The ctypes structs of the binding have the sizes and field offsets the
     C compiler gives the header's structs."""
     import subprocess
-    from multislice_amd.engine import Stack, Probe, Detector, Tuning, Job, Lens
+    from multislice_amd.engine import Stack, Probe, Detector, Tuning, Job, Lens, Series
     assert msx.ATOM_DTYPE.itemsize == 32
     fields = {"msx_stack": (Stack, ["nx", "ny", "ax", "by", "kev", "ctilt_x", "ctilt_y", "bw", "nlayer", "cz",
                                      "nslice", "layer", "gam"]),
@@ -90,7 +90,8 @@ def test_structs_match_header_layout(tmp_path):
               "msx_probe": (Probe, ["alpha_max", "alpha_min", "cs", "df"]),
               "msx_detector": (Detector, ["k_inner", "k_outer"]),
               "msx_job": (Job, ["stack", "trans", "psi0", "psi"]),
-              "msx_lens": (Lens, ["cs", "df", "aobj", "dfa2", "dfa2phi", "dfa3", "dfa3phi"])}
+              "msx_lens": (Lens, ["cs", "df", "aobj", "dfa2", "dfa2phi", "dfa3", "dfa3phi"]),
+              "msx_series": (Series, ["at", "nat", "waves", "hk", "nbeams", "beams", "intensity"])}
     src = ["#include <stdio.h>", "#include <stddef.h>", "#include \"msx.h\"", "int main(void) {"]
     for st, (_, names) in fields.items():
         src.append(f'printf("{st} %zu\\n", sizeof({st}));')
