@@ -148,16 +148,30 @@ def traffic_from_profile(batch, tuning):
     return d["hbm_bytes_per_launch"]
 
 
+C3_DETECTOR = "70-200mrad"  # the ADF annulus of this workload, as the evaluation report names it
+
+
 def surrogate_rate(msx, torch, ctx, value):
     """Converged ADF images/hr of this workload (128x128 scan, 100 slices):
     plain 64-config multislice vs the surrogate (4 configs + the U-Net passes
     per image that the evaluated surrogate makes, 4 levels / 32 channels, one
-    pass timed here on a 128x128 image).  The surrogate's accuracy and pass
-    count come from the evaluation report committed under profiles/ (its own
-    scaled-down specimen set, named there)."""
+    pass timed here on a 128x128 image of the evaluated network's input
+    channels).  The surrogate's accuracy, input channels and pass count come
+    from the newest evaluation report committed under profiles/ (its own
+    scaled-down specimen set, named there), for this workload's ADF detector
+    (70-200 mrad, clipped by the 2/3 band limit) where the report has it."""
     import glob
-    net = msx.UNet(ctx, 4, 32, seed=0)
-    x = torch.rand((1, SCAN, SCAN), device="cuda") + 0.5
+    reps_ = sorted(glob.glob(os.path.join(ROOT, "profiles", "surrogate_eval_*.json")), key=os.path.getmtime)
+    ev = None
+    for path in reversed(reps_):
+        with open(path) as f:
+            d = json.load(f)
+        if "detectors" in d:
+            ev, src = d, path
+            break
+    cin = ev["network"].get("in_channels", 1) if ev else 1
+    net = msx.UNet(ctx, 4, 32, seed=0, in_channels=cin)
+    x = torch.rand((1, cin, SCAN, SCAN) if cin > 1 else (1, SCAN, SCAN), device="cuda") + 0.5
     for _ in range(3):
         net.forward(x)
     torch.cuda.synchronize()
@@ -169,32 +183,34 @@ def surrogate_rate(msx, torch, ctx, value):
     t_unet = (time.perf_counter() - t0) / reps
     net.close()
     per_config = SCAN * SCAN * NSLICE / value
+    passes = ev["network"].get("forward_passes_per_image", 1) if ev else 1
+    t_all = 4 * per_config + passes * t_unet
     out = {"plain_64config": round(3600.0 / (64 * per_config), 3),
-           "surrogate_4config": round(3600.0 / (4 * per_config + t_unet), 3),
-           "gpu_time_ratio": round(64 * per_config / (4 * per_config + t_unet), 2),
-           "unet_forward_ms_128px": round(1e3 * t_unet, 3)}
-    reps_ = sorted(glob.glob(os.path.join(ROOT, "profiles", "surrogate_eval_*.json")))
-    if reps_:
-        with open(reps_[-1]) as f:
-            ev = json.load(f)
-        passes = ev["network"].get("forward_passes_per_image", 1)
-        if passes != 1:  # the evaluated surrogate averages several network passes per image
-            t_all = 4 * per_config + passes * t_unet
-            out.update({"surrogate_4config": round(3600.0 / t_all, 3),
-                        "gpu_time_ratio": round(64 * per_config / t_all, 2), "unet_passes_per_image": passes})
-        met = ev["eval"]["surrogate_rel_rms_mean"] <= 0.02
-        out["meets_2pct_target"] = bool(met)
-        if not met:
-            # the metric counts images converged to 2%: this rate is not one
-            out["surrogate_4config_unconverged"] = out.pop("surrogate_4config")
-            out["surrogate_4config"] = None
-        out["accuracy"] = {"source": os.path.relpath(reps_[-1], ROOT),
-                           "held_out_specimens": ev["eval_specimens"],
-                           "surrogate_rel_rms_mean": round(ev["eval"]["surrogate_rel_rms_mean"], 4),
-                           "input_4config_rel_rms_mean": round(ev["eval"]["input_rel_rms_mean"], 4),
-                           "fraction_within_2pct": round(ev["eval"]["fraction_within_2pct_surrogate"], 3),
-                           "setup": "SrTiO3/Si/Au-on-aC, 5-40 nm, 128^2 grid, 32x32 ADF scans (40 mrad to the "
-                                    "band limit), measured against plain 64-config multislice"}
+           "surrogate_4config": round(3600.0 / t_all, 3),
+           "gpu_time_ratio": round(64 * per_config / t_all, 2),
+           "unet_forward_ms_128px": round(1e3 * t_unet, 3), "unet_passes_per_image": passes}
+    if ev is None:
+        out["meets_2pct_target"] = None
+        return out
+    det = ev["detectors"].get(C3_DETECTOR)
+    name = C3_DETECTOR if det else sorted(ev["detectors"])[0]
+    det = det or ev["detectors"][name]
+    met = det["surrogate_rel_rms_mean"] <= 0.02
+    out["meets_2pct_target"] = bool(met)
+    if not met:
+        # the metric counts images converged to 2%: this rate is not one
+        out["surrogate_4config_unconverged"] = out.pop("surrogate_4config")
+        out["surrogate_4config"] = None
+    out["accuracy"] = {"source": os.path.relpath(src, ROOT), "detector": name,
+                       "held_out_specimens": ev["eval_specimens"],
+                       "surrogate_rel_rms_mean": round(det["surrogate_rel_rms_mean"], 4),
+                       "input_4config_rel_rms_mean": round(det["input_rel_rms_mean"], 4),
+                       "fraction_within_2pct": round(det["fraction_within_2pct_surrogate"], 3),
+                       "floor_64config_vs_256config_rel_rms_mean": round(det["floor_rel_rms_mean"], 4),
+                       "floor_fraction_within_2pct": round(det["fraction_floor_within_2pct"], 3),
+                       "setup": "SrTiO3/Si/Au-on-aC, 5-40 nm, 128^2 grid, 32x32 ADF scans, "
+                                f"{ev['network'].get('in_channels', 1)} input channels, measured against plain "
+                                "64-config multislice"}
     return out
 
 
@@ -210,7 +226,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch slice loops directly (no HIP graph replay)")
     ap.add_argument("--no-tune", action="store_true", help="built-in launch shapes instead of the autotuner's")
     ap.add_argument("--shapes", default="",
-                    help="ROW_WPB,COL_WPB,BATCH[,COL_TILE]: fixed launch shapes instead of the autotuner's")
+                    help="ROW_WPB,COL_WPB,BATCH[,COL_TILE[,ROW_PLAN,COL_PLAN]]: fixed launch shapes "
+                         "instead of the autotuner's")
     ap.add_argument("--tile-probe", action="store_true",
                     help="bandwidth of a contiguous copy and of the column sweep's access pattern, then exit")
     ap.add_argument("--copy-probe", action="store_true",
@@ -235,6 +252,8 @@ def main():
     if a.shapes:
         v = [int(x) for x in a.shapes.split(",")]
         tuning = dict(row_wpb=v[0], col_wpb=v[1], batch=v[2], ms_per_slice=0.0, col_tile=v[3] if len(v) > 3 else 0)
+        if len(v) > 5:
+            tuning.update(row_plan=v[4], col_plan=v[5])
         ctx.set_tuning(N, prec=msx.F32, tuning=tuning)
     elif not a.no_tune:
         tuning = ctx.autotune(N, prec=msx.F32, max_batch=a.batch or 512)

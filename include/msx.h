@@ -94,13 +94,17 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
  * is that schedule with one work-group per slot), the
  * column-sweep tile width (columns per work-group; 0 = built-in width, else
  * the built-in one or its alternative for this line length) and the STEM
- * batch (smallest within 3% of the best rate, at most max_batch; 0: 512).
- * ms_per_slice is the time of one wave through one slice at that batch.
- * msx_set_tuning installs a saved result (NULL clears it). */
+ * batch (smallest within 3% of the best rate, at most max_batch; 0: 512);
+ * then, at that batch, the FFT plan of each sweep (0 = the built-in radix
+ * sequence, 1 = the alternative where the line length has one: 512 = 16 x 16
+ * x 2 against the built-in 8 x 8 x 8).  ms_per_slice is the time of one wave
+ * through one slice with the chosen shapes.  msx_set_tuning installs a saved
+ * result (NULL clears it). */
 typedef struct msx_tuning {
     int32_t row_wpb, col_wpb, batch;
     double ms_per_slice;
     int32_t col_tile;
+    int32_t row_plan, col_plan;
 } msx_tuning;
 MSX_API int msx_autotune(msx_ctx ctx, int nx, int ny, int prec, int max_batch, msx_tuning* best);
 MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuning* t);
@@ -427,11 +431,13 @@ MSX_API int msx_multislice_noalloc(msx_ctx ctx, const msx_stack* stack, const vo
 /* Many independent HRTEM jobs of uneven sizes and thicknesses in as few
  * launches as possible (the reference ran one job per process,
  * statistics/getStatistics.py:34 -> LMultiSlice main).  Each job is a full
- * msx_multislice of one wave.  Jobs with the same grid, cell, kV, tilt, band
- * limit and slice thicknesses (over their common slices) advance together:
- * one row and one column sweep per slice for all of their waves still
- * running, each wave with its own transmission.  Device buffers only
- * (where = MSX_DEVICE); psi0 may be NULL (psi holds the incident wave). */
+ * msx_multislice of one wave, with the same exit wave bit for bit.  Jobs
+ * with the same grid, cell and band limit advance together whatever their
+ * kV, tilt and slice thicknesses: one row and one column sweep per slice for
+ * all of their waves still running, each wave with its own transmission and
+ * propagator, the column sweep's persistent work-groups taking (wave, column
+ * group) units from a device-side work counter.  Device buffers only (where
+ * = MSX_DEVICE); psi0 may be NULL (psi holds the incident wave). */
 typedef struct msx_job {
     const msx_stack* stack;
     const void* trans;  /* [nlayer][nx][ny] complex */
@@ -515,13 +521,19 @@ MSX_API int msx_partial_coherence(msx_ctx ctx, const msx_stack* stack, const voi
  * library's own BF16 matrix-core convolution kernels (FP32 accumulation).
  * Architecture: `levels` resolutions with base_channels << l channels, two
  * 3x3 conv+ReLU per level, 2x2 average pooling, nearest upsampling with skip
- * concatenation, a 1x1 head and a residual connection (output = input + head).
- * Images are FP32, n x h x w (one channel), h and w multiples of
- * 2^(levels-1).  Parameters are one packed FP32 vector: conv by conv (encoder
+ * concatenation, a 1x1 head and a residual connection (output = input + head;
+ * with several input channels, their mean + head).  Input images are FP32,
+ * n x in_channels x h x w (one channel per frozen-phonon configuration, or one
+ * mean image: msx_unet_create), outputs and targets n x h x w; h and w
+ * multiples of 2^(levels-1).  Parameters are one packed FP32 vector: conv by conv (encoder
  * levels 0..L-1 two each, decoder levels L-2..0 two each, head), each as
  * W[Cout][Cin][k*k] followed by bias[Cout].  Loss: mean squared error. */
 typedef struct msx_unet_s* msx_unet;
 MSX_API int msx_unet_create(msx_ctx ctx, int levels, int base_channels, uint64_t seed, msx_unet* out);
+/* a network whose first convolution takes in_channels (1..64) images */
+MSX_API int msx_unet_create_in(msx_ctx ctx, int levels, int base_channels, int in_channels, uint64_t seed,
+                               msx_unet* out);
+MSX_API int msx_unet_in_channels(msx_unet net);
 MSX_API int msx_unet_destroy(msx_unet net);
 MSX_API int64_t msx_unet_param_count(msx_unet net);
 MSX_API int msx_unet_params(msx_unet net, float* dst, int where);

@@ -351,6 +351,19 @@ template <typename C> struct HostTables {
 
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Propagator factors along one axis of n points, cell length a, slice
+// thickness cz, wavelength lam, tt = 2 tan(tilt): mulslice.cpp:494-513,
+// t = pi*cz*(k^2*lambda - k*2tan(tilt)), P = (cos t, -sin t), evaluated in
+// FP64 with k = index/a.
+template <typename C> void prop_factors(int n, double a, double cz, double lam, double tt, C* out) {
+    for (int i = 0; i < n; i++) {
+        const double k = (double)fidx(i, n) / a;
+        const double t = M_PI * cz * (k * k * lam - k * tt);
+        out[i].x = std::cos(t);
+        out[i].y = -std::sin(t);
+    }
+}
+
 template <typename C> void host_tables(const msx_stack* st, bool with_prop, HostTables<C>* h) {
     const int nx = st->nx, ny = st->ny, nl = with_prop ? st->nlayer : 0;
     h->kx2 = k2_table(nx, st->ax);
@@ -360,21 +373,8 @@ template <typename C> void host_tables(const msx_stack* st, bool with_prop, Host
     const double lam = msx_wavelength(st->kev);
     const double tx = 2.0 * std::tan(st->ctilt_x), ty = 2.0 * std::tan(st->ctilt_y);
     for (int l = 0; l < nl; l++) {
-        const double cz = st->cz[l];
-        // mulslice.cpp:494-513: t = pi*cz*(k^2*lambda - k*2tan(tilt)), P = (cos t, -sin t),
-        // evaluated in FP64 with k = index/a
-        for (int i = 0; i < nx; i++) {
-            const double k = (double)fidx(i, nx) / st->ax;
-            const double t = M_PI * cz * (k * k * lam - k * tx);
-            h->px[(size_t)l * nx + i].x = std::cos(t);
-            h->px[(size_t)l * nx + i].y = -std::sin(t);
-        }
-        for (int i = 0; i < ny; i++) {
-            const double k = (double)fidx(i, ny) / st->by;
-            const double t = M_PI * cz * (k * k * lam - k * ty);
-            h->py[(size_t)l * ny + i].x = std::cos(t);
-            h->py[(size_t)l * ny + i].y = -std::sin(t);
-        }
+        prop_factors<C>(nx, st->ax, st->cz[l], lam, tx, &h->px[(size_t)l * nx]);
+        prop_factors<C>(ny, st->by, st->cz[l], lam, ty, &h->py[(size_t)l * ny]);
     }
     h->o_ky = al256(sizeof(float) * nx);
     h->o_px = h->o_ky + al256(sizeof(float) * ny);
@@ -626,7 +626,7 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
         r.prune = obl ? 0 : ((s == 0) ? 2 : 3);
-        if (tu) r.wpb = tu->row_wpb;
+        if (tu) r.wpb = tu->row_wpb, r.plan = tu->row_plan;
         if (s == 0 && start == 0) {
             r.src = src;
             r.src_bstride = src_bstride;
@@ -643,6 +643,7 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         c.px = tb.px + (long long)l * nx;
         c.py = tb.py + (long long)l * ny;
         c.ops = kColProp | kColStore;
+        if (tu) c.plan = tu->col_plan;
         if (obl) {
             // Propagator.cpp:65-72 with this slice's thickness
             const double lam = msx_wavelength(st->kev), dz = st->cz[l];
@@ -697,6 +698,7 @@ int slice_loop_body(msx_ctx ctx, const msx_stack* st, const SliceTables<C>& tb, 
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
         r.prune = obl ? 0 : 1;
+        if (tu) r.plan = tu->row_plan;
         LAUNCH_ROW(ny, kRowInv, r);
     }
     return MSX_OK;
@@ -794,8 +796,9 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     // times (ms per launch) of the row and column sweeps for one shape, on
     // the STEM loop's sequence (waves start in k-space, no final inverse row
     // sweep)
-    auto trial = [&](int rw, int cw, int tile, int nb, double* row_ms, double* col_ms) -> int {
-        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0, tile};
+    auto trial = [&](int rw, int cw, int tile, int nb, double* row_ms, double* col_ms, int rp = 0,
+                     int cp = 0) -> int {
+        ctx->tune[key] = msx_tuning{rw, cw, 0, 0.0, tile, rp, cp};
         ctx->prof = true;
         for (int rep = 0; rep < 2; rep++) {  // the first pass warms up
             for (int k = 0; k < MSX_PROF_KINDS; k++) ctx->prof_n[k] = 0, ctx->prof_ms[k] = 0, ctx->prof_bytes[k] = 0;
@@ -811,7 +814,7 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
     std::vector<int> tiles{tdef};
     if (talt != tdef) tiles.push_back(talt);
     int rc = MSX_OK;
-    msx_tuning best{1, 1, 0, 0.0, 0};
+    msx_tuning best{1, 1, 0, 0.0, 0, 0, 0};
     do {
         // per batch size: the fastest waves-per-work-group of the row sweep,
         // the fastest (tile, waves-per-work-group) of the column sweep, and
@@ -837,9 +840,33 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
         if (rc != MSX_OK) break;
         for (int i = 0; i < kNB && batches[i] <= bmax; i++)
             if (rate[i] >= 0.97 * top) {
-                best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i], tile_of[i]};
+                best = msx_tuning{rw_of[i], cw_of[i], batches[i], 1.0 / rate[i], tile_of[i], 0, 0};
                 break;
             }
+        // at that batch and with those shapes, each sweep's FFT plan (row and
+        // column sweeps are timed separately, so one trial per variant times
+        // both)
+        // (row lines have length ny, column lines nx)
+        int ralt = 0, calt = 0, unused = 0;
+        plan_alts(ny, &ralt, &unused);
+        plan_alts(nx, &unused, &calt);
+        const int nplan = (ralt || calt) ? 2 : 1;
+        if (nplan > 1) {
+            double brow = 1e30, bcol = 1e30;
+            int rbest = 0, cbest = 0;
+            for (int v = 0; v < nplan && rc == MSX_OK; v++) {
+                double rm, cm;
+                if ((rc = trial(best.row_wpb, best.col_wpb, best.col_tile, best.batch, &rm, &cm, ralt ? v : 0,
+                                calt ? v : 0)) != MSX_OK)
+                    break;
+                if (rm < brow) brow = rm, rbest = ralt ? v : 0;
+                if (cm < bcol) bcol = cm, cbest = calt ? v : 0;
+            }
+            if (rc != MSX_OK) break;
+            best.row_plan = rbest;
+            best.col_plan = cbest;
+            best.ms_per_slice = (brow + bcol) / best.batch;
+        }
     } while (0);
     ctx->prof = prof0;
     ctx->graphs = graphs0;
@@ -854,58 +881,80 @@ template <typename C> int autotune_impl(msx_ctx ctx, int nx, int ny, int max_bat
 }
 
 // ---------------------------------------------------------------- job batches
-// Uneven HRTEM jobs (msx_multislice_batch).  Jobs whose propagator tables
-// agree (grid, cell, kV, tilt, band limit, and slice thicknesses over the
-// common slices) form one group; within a group the waves are ordered by
-// slice count (longest first), so the waves still running at slice s are a
-// prefix: every row and column sweep advances all of them in one launch, the
-// row sweep reading each wave's transmission through a per-wave pointer table.
-// A job's final inverse row sweep runs right after its last slice.
+// Uneven HRTEM jobs (msx_multislice_batch).  Jobs on the same grid (size,
+// cell and band limit: the tables of the band pruning) form one class,
+// whatever their kV, tilt and slice thicknesses.  Within a class the waves
+// are ordered by slice count (longest first), so the waves still running at
+// slice s are a prefix, and each slice step is one row sweep (each wave's
+// transmission read through a per-wave pointer table) and one column sweep
+// whose persistent work-groups pull (wave, column group) units from a
+// device-side counter, each wave's propagator factors read through a second
+// pointer table (one factor array per distinct kV, tilt and thickness, the
+// same FP64-evaluated values msx_multislice uses).  A job's final inverse
+// row sweep runs right after its last slice.
 struct BatchItem {
     int job;
     const msx_stack* st;
 };
 
 template <typename C>
-int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g,
-                std::vector<std::vector<const void*>>& keep, bool sync) {
+int batch_class(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g, std::vector<std::vector<char>>& keep,
+                bool sync) {
     using Re = typename real_of<C>::type;
     std::stable_sort(g.begin(), g.end(),
                      [](const BatchItem& a, const BatchItem& b) { return a.st->nslice > b.st->nslice; });
-    const msx_stack* ref = g[0].st;  // the longest: its thicknesses cover every member's slices
+    const msx_stack* ref = g[0].st;  // the longest
     const int nx = ref->nx, ny = ref->ny, nb = (int)g.size(), S = ref->nslice;
     const long long img = (long long)nx * ny;
-    // synthetic stack: one layer per slice of the longest job (its thicknesses)
-    std::vector<double> cz(S);
-    std::vector<int32_t> lay(S);
-    for (int s = 0; s < S; s++) {
-        cz[s] = ref->cz[ref->layer[s]];
-        lay[s] = s;
-    }
-    msx_stack st = *ref;
-    st.nlayer = S;
-    st.cz = cz.data();
-    st.nslice = S;
-    st.layer = lay.data();
     SliceTables<C> tb;
-    CHK(build_tables<C>(ctx, &st, true, &tb));
-    // wave slots and per-slice transmission pointers
-    CHK(ctx->wave.ensure(sizeof(C) * img * nb));
-    C* w = static_cast<C*>(ctx->wave.p);
+    CHK(build_tables<C>(ctx, ref, false, &tb));  // the class's k^2 tables and band limit
     std::vector<int> active(S + 1, 0);
     for (int s = 0; s <= S; s++)
         for (int i = 0; i < nb; i++) active[s] += g[i].st->nslice > s;
-    // host copy kept by the caller until the stream has consumed it
-    keep.emplace_back((size_t)S * nb, nullptr);
-    std::vector<const void*>& tab = keep.back();
+    // distinct propagators (kV, tilt, thickness) of the class's slices
+    std::map<std::tuple<double, double, double, double>, int> pid;
+    std::vector<int> pidx((size_t)S * nb, 0);
     for (int s = 0; s < S; s++)
         for (int i = 0; i < active[s]; i++) {
             const msx_stack* js = g[i].st;
-            tab[(size_t)s * nb + i] = static_cast<const C*>(jobs[g[i].job].trans) + (long long)js->layer[s] * img;
+            const auto key = std::make_tuple(js->kev, js->ctilt_x, js->ctilt_y, js->cz[js->layer[s]]);
+            auto it = pid.find(key);
+            if (it == pid.end()) it = pid.emplace(key, (int)pid.size()).first;
+            pidx[(size_t)s * nb + i] = it->second;
         }
-    CHK(ctx->aux.ensure(sizeof(C*) * tab.size()));
-    const C* const* dtab = static_cast<const C* const*>(ctx->aux.p);
-    HIPCHK(hipMemcpyAsync(ctx->aux.p, tab.data(), sizeof(C*) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+    const int U = (int)pid.size();
+    // one device block: transmission pointers [S][nb], propagator pointers
+    // [S][nb], the slices' work counters [S] (zero) and the factors [U][nx + ny]
+    const size_t o_tt = 0, o_pt = al256(sizeof(void*) * S * nb), o_q = o_pt + al256(sizeof(void*) * S * nb),
+                 o_pf = o_q + al256(sizeof(unsigned) * S), tot = o_pf + sizeof(C) * (size_t)U * (nx + ny);
+    CHK(ctx->aux.ensure(tot));
+    char* dbase = static_cast<char*>(ctx->aux.p);
+    keep.emplace_back(tot, 0);  // host image, kept by the caller until the stream has consumed it
+    char* hbase = keep.back().data();
+    const C** htt = reinterpret_cast<const C**>(hbase + o_tt);
+    const C** hpt = reinterpret_cast<const C**>(hbase + o_pt);
+    C* hpf = reinterpret_cast<C*>(hbase + o_pf);
+    for (const auto& kv : pid) {
+        const double kev = std::get<0>(kv.first), tx = std::get<1>(kv.first), ty = std::get<2>(kv.first);
+        const double dz = std::get<3>(kv.first), lam = msx_wavelength(kev);
+        C* f = hpf + (size_t)kv.second * (nx + ny);
+        prop_factors<C>(nx, ref->ax, dz, lam, 2.0 * std::tan(tx), f);
+        prop_factors<C>(ny, ref->by, dz, lam, 2.0 * std::tan(ty), f + nx);
+    }
+    const C* dpf = reinterpret_cast<const C*>(dbase + o_pf);
+    for (int s = 0; s < S; s++)
+        for (int i = 0; i < active[s]; i++) {
+            const msx_stack* js = g[i].st;
+            htt[(size_t)s * nb + i] = static_cast<const C*>(jobs[g[i].job].trans) + (long long)js->layer[s] * img;
+            hpt[(size_t)s * nb + i] = dpf + (size_t)pidx[(size_t)s * nb + i] * (nx + ny);
+        }
+    HIPCHK(hipMemcpyAsync(dbase, hbase, tot, hipMemcpyHostToDevice, ctx->stream));
+    const C* const* dtt = reinterpret_cast<const C* const*>(dbase + o_tt);
+    const C* const* dpt = reinterpret_cast<const C* const*>(dbase + o_pt);
+    unsigned* dq = reinterpret_cast<unsigned*>(dbase + o_q);
+    // wave slots
+    CHK(ctx->wave.ensure(sizeof(C) * img * nb));
+    C* w = static_cast<C*>(ctx->wave.p);
     for (int i = 0; i < nb; i++) {
         const msx_job& j = jobs[g[i].job];
         HIPCHK(hipMemcpyAsync(w + i * img, j.psi0 ? j.psi0 : j.psi, sizeof(C) * img, hipMemcpyDeviceToDevice,
@@ -915,13 +964,15 @@ int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g,
     CHK(ctx->twiddles<C>(nx, &twx));
     CHK(ctx->twiddles<C>(ny, &twy));
     const Re inv = (Re)1 / ((Re)nx * (Re)ny);
+    const msx_tuning* tu = ctx->tuning(nx, ny, sizeof(Re) == 8 ? MSX_F64 : MSX_F32);
     for (int s = 0; s < S; s++) {
         const int na = active[s];
         RowArgs<C> r = row_args<C>(w, nx, na, twy);
-        r.trans_tab = dtab + (size_t)s * nb;
+        r.trans_tab = dtt + (size_t)s * nb;
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
         r.prune = (s == 0) ? 2 : 3;
+        if (tu) r.wpb = tu->row_wpb;
         if (s == 0) {
             LAUNCH_ROW(ny, kRowFwd, r);
         } else {
@@ -933,9 +984,11 @@ int batch_group(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g,
         c.kx2 = tb.kx2;
         c.ky2 = tb.ky2;
         c.k2max = tb.k2max;
-        c.px = tb.px + (long long)s * nx;
-        c.py = tb.py + (long long)s * ny;
-        c.ops = kColProp | kColStore;
+        c.ops = kColProp | kColJobs | kColStore;
+        c.prop_tab = dpt + (size_t)s * nb;
+        c.queue = dq + s;
+        c.wpb = (tu && tu->col_wpb < 0) ? tu->col_wpb : -1;
+        c.tile = tu ? tu->col_tile : 0;
         LAUNCH_COL(nx, kColSlice, c);
         const int done = na - active[s + 1];  // jobs whose last slice this was
         if (done > 0) {
@@ -958,15 +1011,8 @@ int check_stack(const msx_stack* st);
 template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs, int njobs) {
     std::vector<bool> used(njobs, false);
     for (int i = 0; i < njobs; i++) CHK(check_stack(jobs[i].stack));
-    auto same_optics = [](const msx_stack* a, const msx_stack* b) {
-        return a->nx == b->nx && a->ny == b->ny && a->ax == b->ax && a->by == b->by && a->kev == b->kev &&
-               a->ctilt_x == b->ctilt_x && a->ctilt_y == b->ctilt_y && a->bw == b->bw;
-    };
-    auto same_slices = [](const msx_stack* a, const msx_stack* b) {
-        const int n = std::min(a->nslice, b->nslice);
-        for (int s = 0; s < n; s++)
-            if (a->cz[a->layer[s]] != b->cz[b->layer[s]]) return false;
-        return true;
+    auto same_grid = [](const msx_stack* a, const msx_stack* b) {
+        return a->nx == b->nx && a->ny == b->ny && a->ax == b->ax && a->by == b->by && a->bw == b->bw;
     };
     std::vector<std::vector<BatchItem>> groups;
     for (int i = 0; i < njobs; i++) {
@@ -974,23 +1020,20 @@ template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs
         std::vector<BatchItem> g{{i, jobs[i].stack}};
         used[i] = true;
         for (int k = i + 1; k < njobs; k++) {
-            if (used[k] || !same_optics(jobs[i].stack, jobs[k].stack)) continue;
-            bool ok = true;
-            for (const BatchItem& m : g) ok = ok && same_slices(m.st, jobs[k].stack);
-            if (!ok) continue;
+            if (used[k] || !same_grid(jobs[i].stack, jobs[k].stack)) continue;
             g.push_back({k, jobs[k].stack});
             used[k] = true;
         }
         groups.push_back(std::move(g));
     }
-    std::vector<std::vector<const void*>> keep;
+    std::vector<std::vector<char>> keep;
     // Groups are independent: up to kLanes of them run at once, each on its
     // own stream (groups of small grids alone leave most of the GPU idle).
     // Profiling times launches one at a time, so it keeps one stream.
     constexpr int kLanes = 4;
     const int nl = ctx->prof ? 1 : std::min<int>(kLanes, (int)groups.size());
     if (nl <= 1) {
-        for (auto& g : groups) CHK(batch_group<C>(ctx, jobs, g, keep, true));
+        for (auto& g : groups) CHK(batch_class<C>(ctx, jobs, g, keep, true));
         return MSX_OK;
     }
     CHK(ctx->ensure_lanes(nl));
@@ -1006,7 +1049,7 @@ template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs
     for (size_t gi = 0; gi < groups.size() && rc == MSX_OK; gi++) {
         const int l = (int)(gi % nl);
         ctx->swap_lane(l);
-        rc = batch_group<C>(ctx, jobs, groups[gi], keep, false);
+        rc = batch_class<C>(ctx, jobs, groups[gi], keep, false);
         ctx->swap_lane(l);
     }
     // the context stream waits for every lane; the call returns when all are done
@@ -1261,6 +1304,7 @@ int stem_impl(msx_ctx ctx, const msx_stack* st, const void* trans, const msx_pro
         g.kap_lo = kap_lo;
         g.kap_hi = kap_hi;
         g.amp = (Re)std::sqrt((double)img / (double)nap);
+        if (tu) g.plan = tu->col_plan;
         LAUNCH_COL(nx, kColGenInv, g);
         CHK(slice_loop<C>(ctx, st, tb, t_dev, w_dev, nb, nullptr, 0, 1, &dp));
         // this batch's [nck][ndet][nb] block into columns p0.. of the results
@@ -2077,6 +2121,12 @@ MSX_API int msx_set_tuning(msx_ctx ctx, int nx, int ny, int prec, const msx_tuni
     else col_tiles<float2>(nx, &tdef, &talt);
     if (t->col_tile != 0 && t->col_tile != tdef && t->col_tile != talt)
         return fail(MSX_EINVAL, "column tile %d: this grid takes 0, %d or %d", t->col_tile, tdef, talt);
+    int ralt = 0, calt = 0, unused = 0;
+    plan_alts(ny, &ralt, &unused);
+    plan_alts(nx, &unused, &calt);
+    if (t->row_plan < 0 || t->row_plan > ralt || t->col_plan < 0 || t->col_plan > calt)
+        return fail(MSX_EINVAL, "FFT plans %d/%d: this grid takes row plan 0%s and column plan 0%s", t->row_plan,
+                    t->col_plan, ralt ? " or 1" : "", calt ? " or 1" : "");
     ctx->tune[std::make_tuple(nx, ny, prec)] = *t;
     return MSX_OK;
 }

@@ -249,6 +249,28 @@ template <> struct Plan<2048> : PlanT<2048, 128, 16, 16, 8> {};
 template <int N> struct ColPlan : Plan<N> {};
 template <> struct ColPlan<512> : PlanT<512, MSX_COL512_T, 8, 8, 8> {};
 
+// Plan variants the autotuner chooses between (msx_tuning row_plan /
+// col_plan): variant 0 is the plan above; variant 1 exists where `alt` is
+// true and uses another radix sequence with the same exchange buffers.
+template <int N, int V> struct RowPlanSel {
+    using type = Plan<N>;
+    static constexpr bool alt = false;
+};
+template <int N, int V> struct ColPlanSel {
+    using type = ColPlan<N>;
+    static constexpr bool alt = false;
+};
+// 512 = 16 x 16 x 2 on 32 threads per line: radix-16 butterflies (one
+// twiddle stage of 15 factors instead of two of 7) and 16 elements per thread
+template <> struct RowPlanSel<512, 1> {
+    using type = PlanT<512, 32, 16, 16, 2>;
+    static constexpr bool alt = true;
+};
+template <> struct ColPlanSel<512, 1> {
+    using type = PlanT<512, MSX_COL512_T, 16, 16, 2>;
+    static constexpr bool alt = true;
+};
+
 // ---------------------------------------------------------------- LDS views
 // Element m of the line owned by this thread, in LDS.
 //  Contiguous lines: line-major, one padding slot every 16 elements.

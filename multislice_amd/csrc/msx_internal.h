@@ -31,7 +31,8 @@ enum ColMode : int {
     template <typename C> hipError_t launch_row_##NN(int mode, const RowArgs<C>& a, hipStream_t s); \
     template <typename C> hipError_t launch_col_##NN(int mode, const ColArgs<C>& a, hipStream_t s); \
     template <typename C> int col_groups_##NN(int ny, int tile);                                 \
-    template <typename C> int col_tiles_##NN(int* def, int* alt);
+    template <typename C> int col_tiles_##NN(int* def, int* alt);                               \
+    int plan_alts_##NN(int* row, int* col);
 MSX_FOR_EACH_N(MSX_DECL_N)
 #undef MSX_DECL_N
 
@@ -65,6 +66,15 @@ template <typename C> int col_groups(int nx, int ny, int tile = 0) {
     MSX_FOR_EACH_N(MSX_CASE)
 #undef MSX_CASE
     return 0;
+}
+// whether an n-long line has an alternative FFT plan for the row sweep
+// (RowPlanSel<n, 1>) and for the column sweep (ColPlanSel<n, 1>)
+inline void plan_alts(int n, int* row, int* col) {
+    *row = *col = 0;
+#define MSX_CASE(NN) \
+    if (n == NN) plan_alts_##NN(row, col);
+    MSX_FOR_EACH_N(MSX_CASE)
+#undef MSX_CASE
 }
 // the built-in column tile width of an nx-long line and the alternative the
 // autotuner may pick (equal when there is none)

@@ -18,11 +18,16 @@ namespace msx {
 
 namespace {
 constexpr int kN = MSX_N;
-constexpr int kT = Plan<kN>::T;      // threads per row line
-constexpr int kTc = ColPlan<kN>::T;  // threads per column line
+// threads per row / column line of plan variant PV
+template <int PV> constexpr int kT = RowPlanSel<kN, PV>::type::T;
+template <int PV> constexpr int kTc = ColPlanSel<kN, PV>::type::T;
+constexpr bool kRowAlt = RowPlanSel<kN, 1>::alt, kColAlt = ColPlanSel<kN, 1>::alt;
+// the column tile widths below are those of variant 0; variant 1 keeps its
+// threads per line, so every width fits both
+static_assert(!kColAlt || kTc<1> == kTc<0>, "column plan variants share the thread count");
 
 // lines per work-group of the row sweep: 256 threads (4 wavefronts)
-constexpr int row_lpw() { return kT >= 256 ? 1 : 256 / kT; }
+template <int PV> constexpr int row_lpw() { return kT<PV> >= 256 ? 1 : 256 / kT<PV>; }
 
 // columns per work-group of the column sweep (coalescing width), bounded by
 // 1024 threads and by the LDS tile N*(TW+1) complex values
@@ -45,7 +50,7 @@ constexpr int row_lpw() { return kT >= 256 ? 1 : 256 / kT; }
 #define MSX_COL_THREADS 512
 #endif
 template <typename C> constexpr int fit_tw(int tw) {
-    while (tw > 1 && (kTc * tw > MSX_COL_THREADS || (long)kN * (tw + 1) * (long)sizeof(C) > 80 * 1024)) tw /= 2;
+    while (tw > 1 && (kTc<0> * tw > MSX_COL_THREADS || (long)kN * (tw + 1) * (long)sizeof(C) > 80 * 1024)) tw /= 2;
     return tw;
 }
 template <typename C> constexpr int col_tw() { return fit_tw<C>(kN == 512 ? MSX_COL_TW512 : MSX_COL_TW); }
@@ -73,19 +78,19 @@ inline int pick_wpb(int requested, long long units, long long target_blocks) {
     return (int)(w < 1 ? 1 : (w > 8 ? 8 : w));
 }
 
-template <typename C, int IN, int OUT, bool VREAL = false, bool INTEN = false>
+template <typename C, int IN, int OUT, bool VREAL = false, bool INTEN = false, int PV = 0>
 hipError_t row_go(const RowArgs<C>& a0, hipStream_t s) {
-    constexpr int LPW = row_lpw();
+    constexpr int LPW = row_lpw<PV>();
     RowArgs<C> a = a0;
     const int nxg = (a.nx + LPW - 1) / LPW;
     a.wpb = pick_wpb(a.wpb, (long long)nxg * a.nb, 4096);
     const long long nchunk = (a.nb + a.wpb - 1) / a.wpb;
     const unsigned grid = (unsigned)(nxg * nchunk);
     const size_t smem = (size_t)LPW * (kN + kN / 16) * sizeof(C);
-    auto k = row_sweep<C, kN, IN, OUT, LPW, VREAL, INTEN>;
+    auto k = row_sweep<C, kN, IN, OUT, LPW, VREAL, INTEN, PV>;
     hipError_t e = set_smem(k, smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kT * LPW), smem, s, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kT<PV> * LPW), smem, s, a);
     return hipGetLastError();
 }
 
@@ -115,18 +120,20 @@ hipError_t resident_groups(K k, int threads, size_t smem, std::atomic<long long>
     return hipSuccess;
 }
 
-template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(const ColArgs<C>& a0, hipStream_t s) {
+template <typename C, int IN, int OUT, int OPS, int TW, int PV>
+hipError_t col_go_tw(const ColArgs<C>& a0, hipStream_t s) {
     ColArgs<C> a = a0;
     const int groups = (a.ny + TW - 1) / TW;
     constexpr size_t tile = (size_t)kN * (TW + 1) * sizeof(C);
     // double-buffer the exchanges when two tiles fit in the work-group LDS
     // limit (a launch asking for the full 160 KiB is refused)
     constexpr bool DB = MSX_COL_DB && 2 * tile <= 150 * 1024;
-    auto k = col_sweep<C, kN, IN, OUT, TW, OPS, DB>;
+    auto k = col_sweep<C, kN, IN, OUT, TW, OPS, DB, PV>;
     size_t smem = col_list_offset<C, kN, TW, DB>();  // exchange tiles, detector reduction scratch
     unsigned grid;
-    if (a.wpb <= 0) {
-        // balanced schedule (the built-in one): -wpb work-groups per resident
+    if (a.wpb <= 0 || (OPS & kColJobs) != 0) {
+        // balanced schedule (the built-in one, and the only one of uneven
+        // jobs): -wpb work-groups per resident
         // slot (0: one), each with its list of (group, live mask) pairs after
         // the exchange tiles
         a.wpb = a.wpb < 0 ? a.wpb : -1;
@@ -135,7 +142,7 @@ template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(con
         if (e != hipSuccess) return e;
         static std::atomic<long long> cache[kDev];
         long long slots = 0;
-        e = resident_groups(k, kTc * TW, smem, cache, &slots);
+        e = resident_groups(k, kTc<PV> * TW, smem, cache, &slots);
         if (e != hipSuccess) return e;
         const long long g = std::min(slots * std::min(-a.wpb, 8), (long long)groups * a.nb);
         grid = (unsigned)std::max(1LL, g);
@@ -145,16 +152,32 @@ template <typename C, int IN, int OUT, int OPS, int TW> hipError_t col_go_tw(con
         hipError_t e = set_smem(k, smem);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kTc * TW), smem, s, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kTc<PV> * TW), smem, s, a);
     return hipGetLastError();
 }
 // dispatch on the tile width chosen in the arguments
-template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<C>& a, hipStream_t s) {
+template <typename C, int IN, int OUT, int OPS, int PV = 0> hipError_t col_go_pv(const ColArgs<C>& a, hipStream_t s) {
     constexpr int D = col_tw<C>(), A = col_tw_alt<C>();
     if constexpr (A != D) {
-        if (a.tile == A) return col_go_tw<C, IN, OUT, OPS, A>(a, s);
+        if (a.tile == A) return col_go_tw<C, IN, OUT, OPS, A, PV>(a, s);
     }
-    return col_go_tw<C, IN, OUT, OPS, D>(a, s);
+    return col_go_tw<C, IN, OUT, OPS, D, PV>(a, s);
+}
+template <typename C, int IN, int OUT, int OPS> hipError_t col_go(const ColArgs<C>& a, hipStream_t s) {
+    return col_go_pv<C, IN, OUT, OPS, 0>(a, s);
+}
+// the slice loop's sweeps also dispatch on the plan variant in the arguments
+template <typename C, int IN, int OUT, int OPS> hipError_t col_go_plan(const ColArgs<C>& a, hipStream_t s) {
+    if constexpr (kColAlt) {
+        if (a.plan == 1) return col_go_pv<C, IN, OUT, OPS, 1>(a, s);
+    }
+    return col_go_pv<C, IN, OUT, OPS, 0>(a, s);
+}
+template <typename C, int IN, int OUT> hipError_t row_go_plan(const RowArgs<C>& a, hipStream_t s) {
+    if constexpr (kRowAlt) {
+        if (a.plan == 1) return row_go<C, IN, OUT, false, false, 1>(a, s);
+    }
+    return row_go<C, IN, OUT>(a, s);
 }
 }  // namespace
 
@@ -171,9 +194,9 @@ template <typename C> hipError_t MSX_CAT(launch_row_, MSX_N)(int mode, const Row
         }
     }
     switch (mode) {
-        case kRowFwd: return row_go<C, 0, 1>(a, s);
-        case kRowSlice: return row_go<C, -1, 1>(a, s);
-        case kRowInv: return row_go<C, -1, 0>(a, s);
+        case kRowFwd: return row_go_plan<C, 0, 1>(a, s);
+        case kRowSlice: return row_go_plan<C, -1, 1>(a, s);
+        case kRowInv: return row_go_plan<C, -1, 0>(a, s);
         case kRowInv0: return row_go<C, 0, -1>(a, s);
         case kRowNone: return row_go<C, 0, 0>(a, s);
         default: return hipErrorInvalidValue;
@@ -185,19 +208,23 @@ template <typename C> hipError_t MSX_CAT(launch_col_, MSX_N)(int mode, const Col
     constexpr int kPD = kColProp | kColDetect;
     switch (mode) {
         case kColSlice:
-            if (ops == kColProp) return col_go<C, 1, -1, kColProp>(a, s);
+            if (ops == kColProp) return col_go_plan<C, 1, -1, kColProp>(a, s);
             if (ops == (kColProp | kColObl)) return col_go<C, 1, -1, kColProp | kColObl>(a, s);
-            if (ops == kPD) return col_go<C, 1, -1, kPD>(a, s);
+            if (ops == (kColProp | kColJobs)) {
+                if (!a.prop_tab || !a.queue) return hipErrorInvalidValue;
+                return col_go<C, 1, -1, kColProp | kColJobs>(a, s);
+            }
+            if (ops == kPD) return col_go_plan<C, 1, -1, kPD>(a, s);
             if (ops == kColMask) return col_go<C, 1, -1, kColMask>(a, s);
             if (ops == kColCtf) return col_go<C, 1, -1, kColCtf>(a, s);
             if (ops == 0) return col_go<C, 1, -1, 0>(a, s);
             break;
         case kColFwd:
-            if (ops == kPD) return col_go<C, 1, 0, kPD>(a, s);
+            if (ops == kPD) return col_go_plan<C, 1, 0, kPD>(a, s);
             if (ops == 0) return col_go<C, 1, 0, 0>(a, s);
             break;
         case kColGenInv:
-            if (ops == kColGen) return col_go<C, 0, -1, kColGen>(a, s);
+            if (ops == kColGen) return col_go_plan<C, 0, -1, kColGen>(a, s);
             break;
         case kColInv:
             if (ops == 0) return col_go<C, -1, 0, 0>(a, s);
@@ -222,6 +249,11 @@ template hipError_t MSX_CAT(launch_row_, MSX_N)<float2>(int, const RowArgs<float
 template hipError_t MSX_CAT(launch_row_, MSX_N)<double2>(int, const RowArgs<double2>&, hipStream_t);
 template hipError_t MSX_CAT(launch_col_, MSX_N)<float2>(int, const ColArgs<float2>&, hipStream_t);
 template hipError_t MSX_CAT(launch_col_, MSX_N)<double2>(int, const ColArgs<double2>&, hipStream_t);
+int MSX_CAT(plan_alts_, MSX_N)(int* row, int* col) {
+    *row = kRowAlt ? 1 : 0;
+    *col = kColAlt ? 1 : 0;
+    return 0;
+}
 template int MSX_CAT(col_groups_, MSX_N)<float2>(int, int);
 template int MSX_CAT(col_groups_, MSX_N)<double2>(int, int);
 template int MSX_CAT(col_tiles_, MSX_N)<float2>(int*, int*);

@@ -112,6 +112,7 @@ template <typename C> struct RowArgs {
     int iacc;
     Re iw;
     int nostore;  // 1: only the intensity is written, not the wave
+    int plan;  // FFT plan variant (RowPlanSel)
 };
 
 // IN/OUT: 0 none, +1 forward (exp +i), -1 inverse (exp -i, unscaled)
@@ -122,10 +123,10 @@ template <typename C> struct RowArgs {
 // transmission build); a compile-time switch so the slice kernels carry none
 // of its sincos code.  INTEN: the |psi|^2 output (p.inten) is written; also
 // compile-time, so the slice kernels hold no registers for it.
-template <typename C, int N, int IN, int OUT, int LPW, bool VREAL, bool INTEN = false>
-__global__ __launch_bounds__(Plan<N>::T* LPW) void row_sweep(RowArgs<C> p) {
+template <typename C, int N, int IN, int OUT, int LPW, bool VREAL, bool INTEN = false, int PV = 0>
+__global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(RowArgs<C> p) {
     using Re = typename real_of<C>::type;
-    using PL = Plan<N>;
+    using PL = typename RowPlanSel<N, PV>::type;
     constexpr int T = PL::T, E = N / T;
     constexpr bool WAVE = (64 % T) == 0;  // a line never straddles two wavefronts
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -319,6 +320,8 @@ enum ColOp : int {
     kColCtf = 16,    // objective lens exp(-i chi) inside the aperture, 0 outside
     kColStore = 32,  // write the result back
     kColObl = 64,    // with kColProp: the oblique-cell propagator (FP64 metric per element)
+    kColJobs = 128,  // with kColProp: uneven jobs in one launch: per-wave propagator factors
+                     // (prop_tab) and units pulled from a device-side work counter (queue)
 };
 
 template <typename C> struct ColArgs {
@@ -356,6 +359,13 @@ template <typename C> struct ColArgs {
     // b sin gam), s = x (x - obl_2c y) + y^2 (obl_2c = 2 cos gam), kept where
     // s < obl_smax with the factor exp(-i (obl_pl s - obl_tx x - obl_ty y))
     double obl_asg, obl_bsg, obl_2c, obl_smax, obl_pl, obl_tx, obl_ty;
+    int plan;  // FFT plan variant (ColPlanSel)
+    // kColJobs: prop_tab[b] = the factors of wave b's slice, px (N values)
+    // then py (ny values), in place of px/py; queue = a zeroed counter from
+    // which the balanced schedule's work-groups claim their units after the
+    // first (unit = gridDim.x + the counter's previous value)
+    const C* const* prop_tab;
+    unsigned* queue;
 };
 
 __device__ __forceinline__ int freq_index(int i, int n) {
@@ -418,10 +428,10 @@ template <typename C, int N, int TW, bool DB> constexpr size_t col_list_offset()
 //        of one XCD hold consecutive groups of one wave, as in the chunked
 //        order.
 
-template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB>
-__global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(ColArgs<C> p) {
+template <typename C, int N, int IN, int OUT, int TW, int OPS, bool DB, int PV = 0>
+__global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) void col_sweep(ColArgs<C> p) {
     using Re = typename real_of<C>::type;
-    using PL = ColPlan<N>;
+    using PL = typename ColPlanSel<N, PV>::type;
     constexpr int T = PL::T, E = N / T;
     constexpr bool WAVE = false;  // a column is spread over several wavefronts
     static_assert(TW <= 32, "live-column masks are 32 bits");
@@ -448,7 +458,8 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         }
         return m;
     };
-    const bool bal = p.wpb < 0;
+    constexpr bool jobs = (OPS & kColJobs) != 0;
+    const bool bal = jobs || p.wpb < 0;
     int2* glist = reinterpret_cast<int2*>(smem_raw + col_list_offset<C, N, TW, DB>());
     int ngl = 0, gfix = 0, b0 = 0, b1 = 0;
     unsigned mfix = 0;
@@ -507,9 +518,20 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
     }
     // unit i of this work-group: (group, live mask, wave); false past the end
     const long long units = (long long)ngl * p.nb;
+    __shared__ unsigned qslot;
     auto unit = [&](int i, int& g, unsigned& m, int& b) -> bool {
         if (bal) {
-            const long long u = bid + (long long)i * gridDim.x;
+            long long u = bid + (long long)i * gridDim.x;
+            if constexpr (jobs) {
+                // device-side work queue: after its first unit a work-group
+                // claims the next free one (the claims end when the counter
+                // passes the unit count, so every work-group exits)
+                if (i > 0) {
+                    if (threadIdx.x == 0) qslot = gridDim.x + atomicAdd(p.queue, 1u);
+                    lds_barrier();
+                    u = (long long)__builtin_amdgcn_readfirstlane(qslot);
+                }
+            }
             if (u >= units) return false;
             const int2 e = glist[(int)(u % ngl)];
             g = __builtin_amdgcn_readfirstlane(e.x);
@@ -536,7 +558,7 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
 #pragma unroll
             for (int e = 0; e < E; ++e) kx2c[e] = p.kx2[t + e * T];
         }
-        if constexpr ((OPS & kColProp) != 0) {
+        if constexpr ((OPS & kColProp) != 0 && !jobs) {
 #pragma unroll
             for (int e = 0; e < E; ++e) pxc[e] = p.px[t + e * T];
         }
@@ -544,8 +566,11 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
     auto KX2 = [&](int e) -> float {
         if constexpr (MSX_COL_CONST) return kx2c[e]; else return p.kx2[t + e * T];
     };
+    // propagator factors of the current unit's wave (kColJobs: per wave)
+    const C* pxp = p.px;
+    const C* pyp = p.py;
     auto PX = [&](int e) -> C {
-        if constexpr (MSX_COL_CONST) return pxc[e]; else return p.px[t + e * T];
+        if constexpr (MSX_COL_CONST) return pxc[e]; else return pxp[t + e * T];
     };
     // per-column constants along y of a unit's group, loaded before the
     // prefetch of the next unit (so that waiting for them never waits for it)
@@ -554,7 +579,7 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         if constexpr (point || gen) return p.ky2[col_y(g)]; else return 0.f;
     };
     auto load_py = [&](int g) -> C {
-        if constexpr ((OPS & kColProp) != 0) return p.py[col_y(g)]; else return mk<C>((Re)1, (Re)0);
+        if constexpr ((OPS & kColProp) != 0) return pyp[col_y(g)]; else return mk<C>((Re)1, (Re)0);
     };
 
     // strided accesses: buffer descriptor of the wave's image (uniform) +
@@ -588,6 +613,19 @@ __global__ __launch_bounds__(ColPlan<N>::T* TW, MSX_COL_MINW) void col_sweep(Col
         unsigned m2 = m;
         const bool more = unit(i + 1, g2, m2, b2);
         if (!more) g2 = g, m2 = m, b2 = b;
+        if constexpr (jobs) {
+            // this unit's wave's factors, loaded before the prefetch of the
+            // next unit (using them then never waits for it)
+            const unsigned long long pa = (unsigned long long)p.prop_tab[b];
+            pxp = reinterpret_cast<const C*>(
+                ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(pa >> 32)) << 32) |
+                (unsigned)__builtin_amdgcn_readfirstlane((unsigned)pa));
+            pyp = pxp + N;
+            if constexpr (MSX_COL_CONST) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) pxc[e] = pxp[t + e * T];
+            }
+        }
         const float ky2 = load_ky2(g);
         const C py = load_py(g);
         if constexpr (MSX_COL_PREFETCH) {

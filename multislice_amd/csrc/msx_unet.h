@@ -35,6 +35,8 @@ hipError_t launch_copy_channels(const float* src, int Cs, int s0, float* dst, in
 hipError_t launch_bias_grad(const float* g, const float* y, int relu, int N, int C, int HW, float* db, hipStream_t s);
 hipError_t launch_residual_mse(const float* in, const float* z, const float* t, float* out, float* g, long long n,
                                float scale, hipStream_t s);
+// y[n][p] = (1/C) sum_c x[n][c][p] (FP32, channels in order)
+hipError_t launch_channel_mean(const float* x, float* y, int N, int C, long long HW, hipStream_t s);
 hipError_t launch_sq_diff(const float* a, const float* b, long long n, double* partial, int nblocks, hipStream_t s);
 hipError_t launch_adam(float* w, const float* gw, float* m, float* v, long long n, float lr, float b1, float b2,
                        float eps, float c1, float c2, hipStream_t s);

@@ -5,10 +5,13 @@
 //
 // Architecture (levels L, base channels B, C_l = B << l):
 //   encoder l = 0..L-1 : conv3x3(C_{l-1} -> C_l)+ReLU, conv3x3(C_l -> C_l)+ReLU,
-//                        2x2 average pool between levels (input has 1 channel)
+//                        2x2 average pool between levels (the input has
+//                        C_{-1} = in_channels channels: one image per
+//                        frozen-phonon configuration, or their mean)
 //   decoder l = L-2..0 : nearest x2 upsample, concat [up, skip_l],
 //                        conv3x3(C_{l+1}+C_l -> C_l)+ReLU, conv3x3(C_l -> C_l)+ReLU
-//   head               : conv1x1(C_0 -> 1), output = input + head (residual)
+//   head               : conv1x1(C_0 -> 1), output = channel mean of the input
+//                        + head (residual)
 // Loss: mean squared error over all pixels; optimiser: Adam.
 // Parameters are one packed FP32 vector, conv by conv in the order above, each
 // conv as W[Cout][Cin][k*k] then bias[Cout].
@@ -51,7 +54,7 @@ constexpr int kSqBlocks = 256;
 struct msx_unet_s {
     msx_ctx ctx = nullptr;  // must outlive every call except msx_unet_destroy
     int device = 0;
-    int levels = 0, base = 0;
+    int levels = 0, base = 0, cin0 = 1;
     std::vector<ConvSpec> convs;
     size_t nparam = 0;
     float* pbuf = nullptr;  // params | grads | adam m | adam v
@@ -63,6 +66,7 @@ struct msx_unet_s {
     double* partial = nullptr;
     std::vector<float*> ea, eb, pl, cat, da, db, g_ea, g_eb, g_pl, g_cat, g_da, g_db;
     float *xin = nullptr, *tin = nullptr, *z = nullptr, *out = nullptr, *gz = nullptr;
+    float* xbase = nullptr;  // channel mean of xin (xin itself with one channel)
     float* grad_begin = nullptr;
     size_t grad_bytes = 0;
 
@@ -108,7 +112,8 @@ struct msx_unet_s {
         g_da.assign(L, nullptr);
         g_db.assign(L, nullptr);
         const size_t img = (size_t)n * h * w;
-        slots.push_back({&xin, img});
+        slots.push_back({&xin, img * cin0});
+        if (cin0 > 1) slots.push_back({&xbase, img});
         slots.push_back({&tin, img});
         slots.push_back({&z, img});
         slots.push_back({&out, img});
@@ -149,6 +154,7 @@ struct msx_unet_s {
             ws_bytes = bytes;
         }
         for (size_t i = 0; i < slots.size(); i++) *slots[i].dst = ws + offs[i];
+        if (cin0 == 1) xbase = xin;
         grad_begin = ws + offs[grad_first];
         grad_bytes = (total - offs[grad_first]) * sizeof(float);
         wn = n;
@@ -226,7 +232,7 @@ struct msx_unet_s {
         // clear the accumulated gradients first: gz is the first tensor of the region
         UHIP(hipMemsetAsync(grads(), 0, nparam * sizeof(float), stream()));
         UHIP(hipMemsetAsync(grad_begin, 0, grad_bytes, stream()));
-        UHIP(launch_residual_mse(xin, z, tin, out, gz, cnt, 2.0f / (float)cnt, stream()));
+        UHIP(launch_residual_mse(xbase, z, tin, out, gz, cnt, 2.0f / (float)cnt, stream()));
         UHIP(launch_sq_diff(out, tin, cnt, partial, kSqBlocks, stream()));
         const float* prev = L > 1 ? db[0] : eb[0];
         float* g_prev = L > 1 ? g_db[0] : g_eb[0];
@@ -273,11 +279,15 @@ int check_shape(msx_unet net, int n, int h, int w, int where) {
     if ((long long)n * h * w > (1LL << 26)) return fail(MSX_EINVAL, "batch of %lld pixels too large", (long long)n * h * w);
     return MSX_OK;
 }
-// stage the inputs (x, optional target) into the workspace
+// stage the inputs (x [n][cin0][h][w], optional target [n][h][w]) into the
+// workspace, with the channel mean of x for the residual
 int stage_inputs(msx_unet net, const float* x, const float* t, int where) {
-    const size_t bytes = sizeof(float) * (size_t)net->wn * net->wh * net->ww;
+    const size_t npix = (size_t)net->wn * net->wh * net->ww, bytes = sizeof(float) * npix;
     const hipMemcpyKind k = where == MSX_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
-    UHIP(hipMemcpyAsync(net->xin, x, bytes, k, net->stream()));
+    UHIP(hipMemcpyAsync(net->xin, x, bytes * net->cin0, k, net->stream()));
+    if (net->cin0 > 1)
+        UHIP(launch_channel_mean(net->xin, net->xbase, net->wn, net->cin0, (long long)net->wh * net->ww,
+                                 net->stream()));
     if (t) UHIP(hipMemcpyAsync(net->tin, t, bytes, k, net->stream()));
     return MSX_OK;
 }
@@ -287,11 +297,13 @@ int stage_inputs(msx_unet net, const float* x, const float* t, int where) {
 // ============================================================== C ABI
 extern "C" {
 
-MSX_API int msx_unet_create(msx_ctx ctx, int levels, int base_channels, uint64_t seed, msx_unet* out) {
+MSX_API int msx_unet_create_in(msx_ctx ctx, int levels, int base_channels, int in_channels, uint64_t seed,
+                               msx_unet* out) {
     if (!out) return fail(MSX_EINVAL, "out is NULL");
     *out = nullptr;
     if (!ctx) return fail(MSX_ESTATE, "ctx is NULL");
     if (levels < 1 || levels > 6) return fail(MSX_EINVAL, "levels %d outside 1..6", levels);
+    if (in_channels < 1 || in_channels > 64) return fail(MSX_EINVAL, "in_channels %d outside 1..64", in_channels);
     if (base_channels < 1 || (base_channels << (levels - 1)) > 1024)
         return fail(MSX_EINVAL, "base_channels %d: widest level must be <= 1024 channels", base_channels);
     UHIP(hipSetDevice(msx_ctx_device(ctx)));
@@ -300,6 +312,7 @@ MSX_API int msx_unet_create(msx_ctx ctx, int levels, int base_channels, uint64_t
     net->device = msx_ctx_device(ctx);
     net->levels = levels;
     net->base = base_channels;
+    net->cin0 = in_channels;
     size_t off = 0;
     auto add = [&](int cin, int cout, int ks, int relu) {
         ConvSpec c{cin, cout, ks, relu, off, off + (size_t)cout * cin * ks * ks};
@@ -307,7 +320,7 @@ MSX_API int msx_unet_create(msx_ctx ctx, int levels, int base_channels, uint64_t
         net->convs.push_back(c);
     };
     for (int l = 0; l < levels; l++) {
-        add(l == 0 ? 1 : net->C(l - 1), net->C(l), 3, 1);
+        add(l == 0 ? in_channels : net->C(l - 1), net->C(l), 3, 1);
         add(net->C(l), net->C(l), 3, 1);
     }
     for (int l = levels - 2; l >= 0; l--) {
@@ -338,6 +351,12 @@ MSX_API int msx_unet_create(msx_ctx ctx, int levels, int base_channels, uint64_t
     *out = net;
     return MSX_OK;
 }
+
+MSX_API int msx_unet_create(msx_ctx ctx, int levels, int base_channels, uint64_t seed, msx_unet* out) {
+    return msx_unet_create_in(ctx, levels, base_channels, 1, seed, out);
+}
+
+MSX_API int msx_unet_in_channels(msx_unet net) { return net ? net->cin0 : 0; }
 
 MSX_API int msx_unet_destroy(msx_unet net) {
     if (!net) return MSX_OK;
@@ -414,7 +433,7 @@ MSX_API int msx_unet_forward(msx_unet net, const float* x, float* y, int n, int 
     UCHK(stage_inputs(net, x, nullptr, where));
     UCHK(net->forward());
     const long long cnt = (long long)n * h * w;
-    UHIP(launch_residual_mse(net->xin, net->z, nullptr, net->out, nullptr, cnt, 0.f, net->stream()));
+    UHIP(launch_residual_mse(net->xbase, net->z, nullptr, net->out, nullptr, cnt, 0.f, net->stream()));
     UHIP(hipMemcpyAsync(y, net->out, sizeof(float) * cnt,
                         where == MSX_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, net->stream()));
     UHIP(hipStreamSynchronize(net->stream()));

@@ -296,6 +296,15 @@ __global__ void residual_mse(const float* in, const float* z, const float* t, fl
     out[i] = o;
     if (g) g[i] = scale * (o - t[i]);
 }
+__global__ void channel_mean(const float* x, float* y, int C, long long HW, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const long long b = i / HW, p = i % HW;
+    const float* q = x + b * C * HW + p;
+    float s = 0.f;
+    for (int c = 0; c < C; ++c) s += q[c * HW];
+    y[i] = s * (1.0f / (float)C);
+}
 // per-block partial sums of (a - b)^2 in FP64 (fixed order: deterministic loss)
 __global__ __launch_bounds__(256) void sq_diff(const float* a, const float* b, long long n, double* partial) {
     __shared__ double red[256];
@@ -354,6 +363,11 @@ hipError_t launch_bias_grad(const float* g, const float* y, int relu, int N, int
 hipError_t launch_residual_mse(const float* in, const float* z, const float* t, float* out, float* g, long long n,
                                float scale, hipStream_t s) {
     hipLaunchKernelGGL(residual_mse, MSX_GRID(n), in, z, t, out, g, n, scale);
+    return hipGetLastError();
+}
+hipError_t launch_channel_mean(const float* x, float* y, int N, int C, long long HW, hipStream_t s) {
+    const long long n = (long long)N * HW;
+    hipLaunchKernelGGL(channel_mean, MSX_GRID(n), x, y, C, HW, n);
     return hipGetLastError();
 }
 hipError_t launch_sq_diff(const float* a, const float* b, long long n, double* partial, int nblocks, hipStream_t s) {

@@ -69,7 +69,7 @@ class Probe(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("row_wpb", C.c_int32), ("col_wpb", C.c_int32), ("batch", C.c_int32), ("ms_per_slice", C.c_double),
-                ("col_tile", C.c_int32)]
+                ("col_tile", C.c_int32), ("row_plan", C.c_int32), ("col_plan", C.c_int32)]
 
 
 class Job(C.Structure):
@@ -98,7 +98,8 @@ EXPORTS = ["msx_create", "msx_destroy", "msx_set_stream", "msx_wait_stream", "ms
            "msx_multislice_batch", "msx_potential_rec_curve", "msx_potential_rs_curve",
            "msx_partial_coherence", "msx_amorphous", "msx_parlay", "msx_tiff_write", "msx_tiff_read",
            "msx_musli_stem_write", "msx_musli_wave_write", "msx_potential_device_atoms", "msx_multislice_noalloc",
-           "msx_create_pots_rec_dt", "msx_create_pots_rs_dt", "msx_multislice_series"]
+           "msx_create_pots_rec_dt", "msx_create_pots_rs_dt", "msx_multislice_series", "msx_unet_create_in",
+           "msx_unet_in_channels"]
 
 _lib = None
 _lock = threading.Lock()
@@ -167,6 +168,7 @@ def load():
         "msx_musli_stem_write": (i, [C.c_char_p, vp, i, i, d, d, d, d, d]),
         "msx_musli_wave_write": (i, [C.c_char_p, vp, i, i, d, d, d, d, d]),
         "msx_unet_create": (i, [vp, i, i, C.c_uint64, C.POINTER(vp)]), "msx_unet_destroy": (i, [vp]),
+        "msx_unet_create_in": (i, [vp, i, i, i, C.c_uint64, C.POINTER(vp)]), "msx_unet_in_channels": (i, [vp]),
         "msx_unet_param_count": (C.c_int64, [vp]), "msx_unet_params": (i, [vp, vp, i]),
         "msx_unet_set_params": (i, [vp, vp, i]), "msx_unet_forward": (i, [vp, vp, vp, i, i, i, i]),
         "msx_unet_gradient": (i, [vp, vp, vp, i, i, i, C.POINTER(d), vp, i]),
@@ -310,12 +312,15 @@ class Context:
         t = Tuning()
         _check(load().msx_autotune(self.h, nx, ny or nx, prec, max_batch, C.byref(t)))
         return dict(row_wpb=t.row_wpb, col_wpb=t.col_wpb, batch=t.batch, ms_per_slice=t.ms_per_slice,
-                    col_tile=t.col_tile)
+                    col_tile=t.col_tile, row_plan=t.row_plan, col_plan=t.col_plan)
 
     def set_tuning(self, nx, ny=None, prec=F32, tuning=None):
-        """Install a saved autotuner result (dict as returned by autotune), or clear it (None)."""
+        """Install a saved autotuner result (dict as returned by autotune), or clear it (None).
+        Missing keys: col_tile and the row / column FFT plans 0 (built-in)."""
+        ny = ny or nx
         t = None if tuning is None else C.byref(Tuning(tuning["row_wpb"], tuning["col_wpb"], tuning["batch"],
-                                                       tuning.get("ms_per_slice", 0.0), tuning.get("col_tile", 0)))
+                                                       tuning.get("ms_per_slice", 0.0), tuning.get("col_tile", 0),
+                                                       tuning.get("row_plan", 0), tuning.get("col_plan", 0)))
         _check(load().msx_set_tuning(self.h, nx, ny or nx, prec, t))
 
     def set_graphs(self, enable=True):
@@ -701,13 +706,15 @@ def unet_conv(ctx, mode, ks, out, w, x=None, bias=None, g=None, y=None, relu=Fal
 
 class UNet:
     """The frozen-phonon surrogate network (msx_unet_*), resident on the
-    context's GPU.  Images are (n, h, w) float32: numpy (host) or torch CUDA
+    context's GPU.  Inputs are (n, h, w) float32 images for a one-channel
+    network, (n, in_channels, h, w) otherwise (one image per frozen-phonon
+    configuration); outputs and targets (n, h, w).  numpy (host) or torch CUDA
     tensors (device); h, w multiples of 2**(levels-1)."""
 
-    def __init__(self, ctx, levels=4, base=32, seed=0):
-        self.ctx, self.levels, self.base = ctx, levels, base
+    def __init__(self, ctx, levels=4, base=32, seed=0, in_channels=1):
+        self.ctx, self.levels, self.base, self.in_channels = ctx, levels, base, int(in_channels)
         h = C.c_void_p()
-        _check(load().msx_unet_create(ctx.h, levels, base, seed, C.byref(h)))
+        _check(load().msx_unet_create_in(ctx.h, levels, base, self.in_channels, seed, C.byref(h)))
         self.h = h
 
     def close(self):
@@ -736,16 +743,25 @@ class UNet:
             raise ValueError(f"{p.size} parameters given, the network has {self.nparam}")
         _check(load().msx_unet_set_params(self.h, C.c_void_p(p.ctypes.data), HOST))
 
-    @staticmethod
-    def _img(x):
+    def _in(self, x):
+        """Input buffer and its (n, h, w)."""
         b = _buf(x, F32, complex_=False)
-        if len(b[2].shape) != 3:
-            raise ValueError("images must be (n, h, w)")
+        shp = tuple(b[2].shape)
+        if self.in_channels == 1 and len(shp) == 3:
+            return b, shp
+        if len(shp) != 4 or shp[1] != self.in_channels:
+            raise ValueError(f"inputs must be (n, {self.in_channels}, h, w)"
+                             + (" or (n, h, w)" if self.in_channels == 1 else "") + f", got {shp}")
+        return b, (shp[0], shp[2], shp[3])
+
+    @staticmethod
+    def _img(x, shp, name="target"):
+        b = _buf(x, F32, complex_=False)
+        _need(tuple(b[2].shape) == shp, f"{name} must be {shp}, got {tuple(b[2].shape)}")
         return b
 
     def forward(self, x, out=None):
-        bx = self._img(x)
-        n, hh, ww = bx[2].shape
+        bx, (n, hh, ww) = self._in(x)
         if out is None:
             out = np.empty((n, hh, ww), np.float32) if bx[1] == HOST else x.new_empty((n, hh, ww))
         _need(tuple(out.shape) == (n, hh, ww), f"out must be ({n}, {hh}, {ww})")
@@ -757,9 +773,8 @@ class UNet:
 
     def gradient(self, x, target):
         """(loss, packed gradient as numpy float32)."""
-        bx, bt = self._img(x), self._img(target)
-        n, hh, ww = bx[2].shape
-        _need(tuple(bt[2].shape) == (n, hh, ww), "target must have the shape of x")
+        bx, (n, hh, ww) = self._in(x)
+        bt = self._img(target, (n, hh, ww))
         g = np.empty(self.nparam, np.float32)
         loss = C.c_double()
         where = _ptr_where([bx, bt])
@@ -776,9 +791,8 @@ class UNet:
         return loss.value, g
 
     def train_step(self, x, target, lr):
-        bx, bt = self._img(x), self._img(target)
-        n, hh, ww = bx[2].shape
-        _need(tuple(bt[2].shape) == (n, hh, ww), "target must have the shape of x")
+        bx, (n, hh, ww) = self._in(x)
+        bt = self._img(target, (n, hh, ww))
         loss = C.c_double()
         self.ctx._order(x, target)
         _check(load().msx_unet_train_step(self.h, bx[0], bt[0], n, hh, ww, float(lr), C.byref(loss),

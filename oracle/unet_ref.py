@@ -24,12 +24,12 @@ def bf16_round(a):
     return (r & 0xFFFFFFFF).astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def conv_specs(levels, base):
-    """[(cin, cout, ks, relu)] in packed order."""
+def conv_specs(levels, base, cin=1):
+    """[(cin, cout, ks, relu)] in packed order (cin: input channels)."""
     C = [base << l for l in range(levels)]
     s = []
     for l in range(levels):
-        s.append((1 if l == 0 else C[l - 1], C[l], 3, True))
+        s.append((cin if l == 0 else C[l - 1], C[l], 3, True))
         s.append((C[l], C[l], 3, True))
     for l in range(levels - 2, -1, -1):
         s.append((C[l + 1] + C[l], C[l], 3, True))
@@ -38,13 +38,13 @@ def conv_specs(levels, base):
     return s
 
 
-def param_count(levels, base):
-    return sum(co * ci * k * k + co for ci, co, k, _ in conv_specs(levels, base))
+def param_count(levels, base, cin=1):
+    return sum(co * ci * k * k + co for ci, co, k, _ in conv_specs(levels, base, cin))
 
 
-def unpack(p, levels, base):
+def unpack(p, levels, base, cin=1):
     out, o = [], 0
-    for ci, co, k, relu in conv_specs(levels, base):
+    for ci, co, k, relu in conv_specs(levels, base, cin):
         w = np.asarray(p[o:o + co * ci * k * k], np.float64).reshape(co, ci * k * k)
         o += co * ci * k * k
         b = np.asarray(p[o:o + co], np.float64)
@@ -100,9 +100,9 @@ def conv_direct(x, w, b, ks):
 
 
 class UNetRef:
-    def __init__(self, levels, base, params, bf16=False):
-        self.L, self.B, self.bf16 = levels, base, bf16
-        self.convs = unpack(params, levels, base)
+    def __init__(self, levels, base, params, bf16=False, cin=1):
+        self.L, self.B, self.bf16, self.cin = levels, base, bf16, cin
+        self.convs = unpack(params, levels, base, cin)
         self.rnd = bf16_round if bf16 else (lambda a: a)
 
     def _conv(self, i, x):
@@ -116,9 +116,12 @@ class UNetRef:
         return y, cols
 
     def forward(self, x):
-        """x [N, H, W] -> output [N, H, W]; keeps the tape for backward."""
+        """x [N, H, W] (one channel) or [N, cin, H, W] -> output [N, H, W]:
+        the channel mean of x plus the head; keeps the tape for backward."""
         L = self.L
-        x4 = np.asarray(x, np.float64)[:, None]
+        x = np.asarray(x, np.float64)
+        x4 = x[:, None] if x.ndim == 3 else x
+        assert x4.shape[1] == self.cin
         tape = {}
         h = x4
         for l in range(L):
@@ -141,7 +144,7 @@ class UNetRef:
         z, cz = self._conv(i, prev)
         tape["head"] = (prev, cz, z, i)
         self.tape = tape
-        return x4[:, 0] + z[:, 0]
+        return x4.mean(axis=1) + z[:, 0]
 
     def _conv_bwd(self, i, xin, cols, y, g, grads):
         w, b, ks, relu = self.convs[i]

@@ -86,7 +86,7 @@ def test_structs_match_header_layout(tmp_path):
     assert msx.ATOM_DTYPE.itemsize == 32
     fields = {"msx_stack": (Stack, ["nx", "ny", "ax", "by", "kev", "ctilt_x", "ctilt_y", "bw", "nlayer", "cz",
                                      "nslice", "layer", "gam"]),
-              "msx_tuning": (Tuning, ["row_wpb", "col_wpb", "batch", "ms_per_slice", "col_tile"]),
+              "msx_tuning": (Tuning, ["row_wpb", "col_wpb", "batch", "ms_per_slice", "col_tile", "row_plan", "col_plan"]),
               "msx_probe": (Probe, ["alpha_max", "alpha_min", "cs", "df"]),
               "msx_detector": (Detector, ["k_inner", "k_outer"]),
               "msx_job": (Job, ["stack", "trans", "psi0", "psi"]),

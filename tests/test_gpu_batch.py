@@ -59,6 +59,33 @@ def test_batch_equals_single_jobs(ctx):
     assert np.linalg.norm(got[0] - ref) / np.linalg.norm(ref) < 1e-5
 
 
+def test_batch_mixed_optics_on_one_grid(ctx):
+    """Jobs on one grid with different voltages, tilts, slice thicknesses and
+    slice counts share each slice step's launches (per-wave propagator
+    tables, the column sweep's work counter): each exit wave equals its
+    msx_multislice run bit for bit, in FP32 and FP64."""
+    import torch
+    rng = np.random.default_rng(12)
+    for cdt, rdt in ((torch.complex64, np.float32), (torch.complex128, np.float64)):
+        jobs = []
+        for k in range(9):
+            n, L = 256, 25.6
+            ns = int(rng.integers(4, 30))
+            nl = 3
+            V = (np.abs(rng.standard_normal((nl, n, n))).cumsum(axis=2) % 20.0).astype(rdt)
+            kev = float(rng.choice([100.0, 200.0, 300.0]))
+            cz = rng.uniform(1.5, 2.5, nl)
+            tilt = tuple(rng.uniform(-0.01, 0.01, 2)) if k % 3 else (0.0, 0.0)
+            t = ctx.transmission(torch.from_numpy(V).cuda(), L, L, kev)
+            jobs.append(dict(trans=t, ax=L, by=L, kev=kev, cz=cz, layer=rng.integers(0, nl, ns), tilt=tilt,
+                             psi=torch.ones((n, n), dtype=cdt, device="cuda")))
+        ctx.multislice_batch(jobs)
+        for j in jobs:
+            psi = torch.ones_like(j["psi"])
+            ctx.multislice(j["trans"], j["cz"], j["layer"], psi, j["ax"], j["by"], j["kev"], tilt=j["tilt"])
+            assert torch.equal(j["psi"], psi)
+
+
 def test_batch_fp32_vs_fp64_per_job(ctx):
     import torch
     jobs32 = make_jobs(np.random.default_rng(7), 10, [128, 256, 384], "f32")

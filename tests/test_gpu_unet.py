@@ -31,10 +31,10 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
 
 
-def rand_params(levels, base, seed):
+def rand_params(levels, base, seed, cin=1):
     rng = np.random.default_rng(seed)
     p = []
-    for ci, co, k, _ in conv_specs(levels, base):
+    for ci, co, k, _ in conv_specs(levels, base, cin):
         p.append(rng.standard_normal(co * ci * k * k) * np.sqrt(2.0 / (ci * k * k)))
         p.append(rng.standard_normal(co) * 0.1)
     return np.concatenate(p).astype(np.float32)
@@ -189,4 +189,36 @@ def test_rejects_bad_shapes(ctx):
         net.forward(np.zeros((1, 30, 32), np.float32))  # 30 is not a multiple of 4
     with pytest.raises(msx.MsxError):
         msx.UNet(ctx, 7, 8)
+    net.close()
+
+
+@pytest.mark.parametrize("levels,base,cin", [(3, 8, 4), (4, 32, 4)])
+def test_multichannel_input_matches_oracle(ctx, levels, base, cin):
+    """Several input channels (one image per frozen-phonon configuration):
+    forward and gradient against the oracle, the residual being the channel
+    mean; a one-channel input shape is refused."""
+    import torch
+    net = msx.UNet(ctx, levels, base, seed=1, in_channels=cin)
+    p = rand_params(levels, base, 13, cin)
+    assert net.nparam == p.size
+    net.set_params(p)
+    hw = 32 if levels < 4 else 64
+    x = np.stack([images(2, hw, 20 + c) for c in range(cin)], axis=1)
+    t = images(2, hw, 30)
+    y = net.forward(x)
+    ref = UNetRef(levels, base, p.astype(np.float64), bf16=True, cin=cin)
+    base_img = x.astype(np.float64).mean(axis=1)
+    e_fwd = rel(y.astype(np.float64) - base_img, ref.forward(x) - base_img)
+    loss, g = net.gradient(x, t)
+    l_bf, _, g_bf = ref.loss_and_grad(x, t)
+    e_g = rel(g, g_bf)
+    print(f"levels {levels} base {base} in_channels {cin}: forward vs bf16 oracle {e_fwd:.2e}, "
+          f"loss {loss:.6g} (oracle {l_bf:.6g}), gradient {e_g:.2e}")
+    assert e_fwd < FWD_TOL[levels]
+    assert abs(loss - l_bf) <= 2e-3 * l_bf
+    assert e_g < GRAD_TOL[levels]
+    yd = net.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    np.testing.assert_array_equal(y, yd)
+    with pytest.raises(ValueError, match="inputs must be"):
+        net.forward(x[:, 0])
     net.close()

@@ -5,7 +5,10 @@
 - the im2col convolution against the nested-loop definition;
 - the analytic gradient of the loss against central finite differences, for
   one-, two- and three-level networks (pooling, upsampling, concatenation and
-  the residual head all on the path).
+  the residual head all on the path), with one input channel and with
+  several (the residual is then the channel mean);
+- a network whose input channels are copies of one image equals the
+  one-channel network whose first-layer weights are their sums.
 """
 import numpy as np
 import pytest
@@ -52,29 +55,29 @@ def test_param_layout():
     assert param_count(4, 32) == sum(sizes) == 1_946_305
 
 
-def _rand_params(levels, base, rng):
+def _rand_params(levels, base, rng, cin=1):
     p = []
-    for ci, co, k, _ in conv_specs(levels, base):
+    for ci, co, k, _ in conv_specs(levels, base, cin):
         p.append(rng.standard_normal(co * ci * k * k) * np.sqrt(2.0 / (ci * k * k)))
         p.append(rng.standard_normal(co) * 0.1)
     return np.concatenate(p)
 
 
-@pytest.mark.parametrize("levels,base,hw", [(1, 3, 6), (2, 2, 8), (3, 2, 8)])
-def test_gradient_matches_finite_differences(levels, base, hw):
-    rng = np.random.default_rng(10 * levels + base)
-    p = _rand_params(levels, base, rng)
-    x = rng.standard_normal((2, hw, hw))
+@pytest.mark.parametrize("levels,base,hw,cin", [(1, 3, 6, 1), (2, 2, 8, 1), (3, 2, 8, 1), (2, 2, 8, 3)])
+def test_gradient_matches_finite_differences(levels, base, hw, cin):
+    rng = np.random.default_rng(10 * levels + base + cin)
+    p = _rand_params(levels, base, rng, cin)
+    x = rng.standard_normal((2, hw, hw) if cin == 1 else (2, cin, hw, hw))
     t = rng.standard_normal((2, hw, hw))
-    loss, _, g = UNetRef(levels, base, p).loss_and_grad(x, t)
+    loss, _, g = UNetRef(levels, base, p, cin=cin).loss_and_grad(x, t)
     idx = rng.choice(p.size, 40, replace=False)
     h = 1e-6
     for i in idx:
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        lp = UNetRef(levels, base, pp).loss_and_grad(x, t)[0]
-        lm = UNetRef(levels, base, pm).loss_and_grad(x, t)[0]
+        lp = UNetRef(levels, base, pp, cin=cin).loss_and_grad(x, t)[0]
+        lm = UNetRef(levels, base, pm, cin=cin).loss_and_grad(x, t)[0]
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
@@ -88,3 +91,17 @@ def test_untrained_identity_head():
     p[-(co * ci * k * k + co):] = 0.0
     x = rng.standard_normal((1, 8, 8))
     np.testing.assert_array_equal(UNetRef(levels, base, p).forward(x), x)
+
+
+def test_copied_channels_equal_one_channel_network():
+    levels, base, cin = 2, 3, 4
+    rng = np.random.default_rng(3)
+    p4 = _rand_params(levels, base, rng, cin)
+    w0 = base * cin * 9
+    # one-channel parameters: first-layer weights summed over the input channels
+    p1 = np.concatenate([p4[:w0].reshape(base, cin, 9).sum(axis=1).ravel(), p4[w0:]])
+    assert p1.size == param_count(levels, base)
+    x = rng.standard_normal((2, 8, 8))
+    y4 = UNetRef(levels, base, p4, cin=cin).forward(np.repeat(x[:, None], cin, axis=1))
+    y1 = UNetRef(levels, base, p1).forward(x)
+    np.testing.assert_allclose(y4, y1, rtol=1e-12, atol=1e-12)

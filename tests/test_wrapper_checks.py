@@ -99,11 +99,14 @@ def test_unet_and_displace_reject(ctx):
     with pytest.raises(ValueError, match="mode 0 needs"):
         engine.unet_conv(ctx, 2, 3, torch.zeros((4, 18)), w, x=x)
     net = engine.UNet.__new__(engine.UNet)
-    net.h, net.ctx = None, ctx
+    net.h, net.ctx, net.in_channels = None, ctx, 1
     with pytest.raises(ValueError, match="out must be"):
         net.forward(np.zeros((1, 8, 8), np.float32), out=np.zeros((1, 8, 4), np.float32))
     with pytest.raises(ValueError, match="target"):
         net.train_step(np.zeros((1, 8, 8), np.float32), np.zeros((2, 8, 8), np.float32), 1e-3)
+    net.in_channels = 4
+    with pytest.raises(ValueError, match="inputs must be"):
+        net.forward(np.zeros((1, 3, 8, 8), np.float32))
     at = np.zeros(4, engine.ATOM_DTYPE)
     with pytest.raises(ValueError, match="out must be"):
         msx.displace(at, {6: 0.1}, 1, 0, out=np.zeros(3, engine.ATOM_DTYPE))
