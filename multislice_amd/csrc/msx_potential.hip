@@ -136,13 +136,26 @@ hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
 
 // TAB: V_Z from the radial tables (FP32 output); otherwise the exact FP64
 // formula per pair (separate instantiations: the table kernel carries none of
-// the Bessel code)
+// the Bessel code).
+// A work-group is a 16 x 16 pixel tile of one slice.  It walks the atom bins
+// the tile can reach in the same order as a per-pixel loop would (bin column
+// bx, then bin rows by, atoms in cell order), and for each run of contiguous
+// cells loads up to 256 atom records at once, one per thread, keeps those
+// whose minimum-image distance to the tile centre allows a pixel of the tile
+// within the cutoff (a conservative bound: every pixel-atom pair it drops
+// fails the per-pixel test), and compacts them in LDS in that order
+// (wavefront ballots).  Every thread then sums over the list: the same pairs
+// in the same order as testing every atom of the bins, so the potential is
+// bit-identical, with the atom loop reading LDS instead of per-wavefront
+// scalar loads and branches.
 template <bool TAB>
 __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
+    __shared__ AtomRec cand[256];
+    __shared__ int wcnt[4];
     // each wavefront covers an 8 x 8 quarter of the 16 x 16 tile: a compact
     // footprint leaves fewer lanes idle outside the radius of an atom the
     // wavefront evaluates
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int iy = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
     const int ix = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
     const int kz = blockIdx.z;
@@ -157,25 +170,55 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     int bya = (int)floor(ylo / bwy), byb = (int)floor(yhi / bwy);
     if (bxb - bxa + 1 >= p.nbx) { bxa = 0; bxb = p.nbx - 1; }
     if (byb - bya + 1 >= p.nby) { bya = 0; byb = p.nby - 1; }
+    // the tile filter: centre and reach (half extent + cutoff, plus a margin
+    // far above rounding) per axis; no filtering along an axis where the
+    // reach covers half the period
+    const double cx = (double)(blockIdx.y * 16) * dx + 7.5 * dx, cy = (double)(blockIdx.x * 16) * dy + 7.5 * dy;
+    const double hx = 7.5 * dx + p.radius + 1e-6, hy = 7.5 * dy + p.radius + 1e-6;
+    const bool fx = hx < 0.5 * p.a, fy = hy < 0.5 * p.b;
     double v = 0.0;
     for (int bx = bxa; bx <= bxb; ++bx) {
         const int bxw = ((bx % p.nbx) + p.nbx) % p.nbx;
-        for (int by = bya; by <= byb; ++by) {
+        for (int by = bya; by <= byb;) {
+            // the run of bins by..byr whose wrapped rows are contiguous cells
             const int byw = ((by % p.nby) + p.nby) % p.nby;
+            const int byr = min(byb, by + (p.nby - 1 - byw));
             const int cell = (kz * p.nbx + bxw) * p.nby + byw;
-            const int i0 = p.cell_start[cell], i1 = p.cell_start[cell + 1];
-            for (int i = i0; i < i1; ++i) {
-                const AtomRec at = p.atoms[i];
-                double dX = fabs(at.x - xp);
-                if (dX >= p.a / 2.0) dX -= p.a;
-                double dY = fabs(at.y - yp);
-                if (dY >= p.b / 2.0) dY -= p.b;
-                const double r2 = dX * dX + dY * dY;
-                if (r2 < r2max) {
-                    double vz;
-                    if constexpr (TAB) vz = radial_interp(p, at.slot, r2);
-                    else vz = d_vz(at.Z, sqrt(r2), p.table, p.formula);
-                    v += (double)at.occ * vz;
+            const int i0 = p.cell_start[cell], i1 = p.cell_start[cell + (byr - by) + 1];
+            by = byr + 1;
+            for (int base = i0; base < i1; base += 256) {
+                __syncthreads();  // the previous list has been read
+                const int i = base + tid;
+                AtomRec at{};
+                bool keep = false;
+                if (i < i1) {
+                    at = p.atoms[i];
+                    double ex = fabs(at.x - cx), ey = fabs(at.y - cy);
+                    if (ex > 0.5 * p.a) ex = p.a - ex;
+                    if (ey > 0.5 * p.b) ey = p.b - ey;
+                    keep = (!fx || ex <= hx) && (!fy || ey <= hy);
+                }
+                const unsigned long long bal = __ballot(keep);
+                if (lane == 0) wcnt[wv] = __popcll(bal);
+                __syncthreads();
+                int off = 0;
+                for (int w = 0; w < wv; ++w) off += wcnt[w];
+                if (keep) cand[off + __popcll(bal & ((1ull << lane) - 1ull))] = at;
+                const int n = __builtin_amdgcn_readfirstlane(wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3]);
+                __syncthreads();
+                for (int k = 0; k < n; ++k) {
+                    const AtomRec c = cand[k];
+                    double dX = fabs(c.x - xp);
+                    if (dX >= p.a / 2.0) dX -= p.a;
+                    double dY = fabs(c.y - yp);
+                    if (dY >= p.b / 2.0) dY -= p.b;
+                    const double r2 = dX * dX + dY * dY;
+                    if (r2 < r2max) {
+                        double vz;
+                        if constexpr (TAB) vz = radial_interp(p, c.slot, r2);
+                        else vz = d_vz(c.Z, sqrt(r2), p.table, p.formula);
+                        v += (double)c.occ * vz;
+                    }
                 }
             }
         }
