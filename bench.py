@@ -135,20 +135,19 @@ def tile_probe():
 def traffic_from_profile(batch, tuning):
     """HBM bytes per launch of the column sweep on this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
-    probe) of the same launch shape; None when there is none."""
-    path = os.path.join(ROOT, "profiles", f"col_sweep_stem512_b{batch}_traffic.json")
+    probe) of the same launch shapes, scaled per wave to this batch (the
+    sweep loads the same tiles for every wave); None when there is none."""
+    path = os.path.join(ROOT, "profiles", "col_sweep_stem512_b128_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         d = json.load(f)
     # the built-in column tile at 512^2 is 8 columns (tile 0 means that width)
     if tuning and (d.get("col_wpb") not in (None, tuning.get("col_wpb"))
-                   or d.get("col_tile", 8) != (tuning.get("col_tile") or 8)):
+                   or d.get("col_tile", 8) != (tuning.get("col_tile") or 8)
+                   or tuning.get("col_plan", 0) != 0):
         return None
-    return d["hbm_bytes_per_launch"]
-
-
-C3_DETECTOR = "70-200mrad"  # the ADF annulus of this workload, as the evaluation report names it
+    return int(round(d["hbm_bytes_per_launch"] * batch / d["batch"]))
 
 
 def surrogate_rate(msx, torch, ctx, value):

@@ -94,7 +94,9 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
  * is that schedule with one work-group per slot), the
  * column-sweep tile width (columns per work-group; 0 = built-in width, else
  * the built-in one or its alternative for this line length) and the STEM
- * batch (smallest within 3% of the best rate, at most max_batch; 0: 512);
+ * batch (at most max_batch, 0: 512; of the batches within 10% of the best
+ * per-sweep rate, the one whose slice loop runs fastest back to back, the
+ * smaller within 1%);
  * then, at that batch, the FFT plan of each sweep (0 = the built-in radix
  * sequence, 1 = the alternative where the line length has one: 512 = 16 x 16
  * x 2 against the built-in 8 x 8 x 8).  ms_per_slice is the time of one wave

@@ -9,6 +9,7 @@ all configurations advance through one batched slice loop
 (msx_multislice_batch: one wave per configuration, each with its own
 transmission), and the objective-lens images are averaged.
 
+    python -m multislice_amd.hrtem c1      # Si 4x4x20, 256^2, 80 slices, one plane wave (ms/slice)
     python -m multislice_amd.hrtem c2      # SrTiO3 52x52x50, 2048^2, 100 slices, 16 configurations
     python -m multislice_amd.hrtem c4      # 256 uneven jobs, batched vs one by one
     python -m multislice_amd.hrtem pc      # partial-coherence images, Si 256^2 and 512^2, 80 slices
@@ -63,6 +64,40 @@ def frozen_phonon_image(ctx, atoms, cell, n, nslices, kev, u_rms, nconf, seed, c
     img /= nconf
     tm["ctf_average"] = _sync() - t2
     return img, tm
+
+
+def c1(ctx, reps=20):
+    """The small-HRTEM configuration: Si [001] 4x4x20 cells (2560 atoms at
+    exact sites), 256^2, 80 slices of a/4, 200 kV, one plane wave: the
+    potential and transmission build, then the slice loop of one wave
+    (launch-latency bound), timed with direct launches and replayed from a
+    cached HIP graph (mean of reps runs after a warm-up)."""
+    import torch
+    at, cell = specimen.si_001((4, 4, 20))
+    n, ns, kev = 256, 80, 200.0
+    V = torch.empty((ns, n, n), dtype=torch.float32, device="cuda")
+    t0 = _sync()
+    ctx.potential(at, cell, n, n, ns, 3.0, prec=engine.F32, out=V)
+    t = ctx.transmission(V, cell[0], cell[1], kev)
+    t_build = _sync() - t0
+    cz, layer = np.full(ns, cell[2] / ns), np.arange(ns)
+    psi = torch.empty((n, n), dtype=torch.complex64, device="cuda")
+    out = {}
+    for graphs in (False, True):
+        ctx.set_graphs(graphs)
+        psi.fill_(1)
+        ctx.multislice(t, cz, layer, psi, cell[0], cell[1], kev)
+        t0 = _sync()
+        for _ in range(reps):
+            psi.fill_(1)
+            ctx.multislice(t, cz, layer, psi, cell[0], cell[1], kev)
+        dt = (_sync() - t0) / reps
+        out["graphs" if graphs else "direct"] = {"ms_per_run": round(1e3 * dt, 3),
+                                                 "ms_per_slice": round(1e3 * dt / ns, 4)}
+    ctx.set_graphs(False)
+    return {"workload": "C1 Si [001] 4x4x20 cells (2560 atoms), 256^2, 80 slices of 1.35775 A, 200 kV, one "
+                        "plane wave", "build_ms": round(1e3 * t_build, 2), "slice_loop": out,
+            "mean_exit_intensity": float((psi.abs() ** 2).mean())}
 
 
 def c2(ctx):
@@ -217,10 +252,10 @@ def pc(ctx, reps=5):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m multislice_amd.hrtem")
-    ap.add_argument("workload", choices=["c2", "c4", "pc"])
+    ap.add_argument("workload", choices=["c1", "c2", "c4", "pc"])
     a = ap.parse_args(argv)
     ctx = engine.Context(0)
-    run = {"c2": c2, "c4": c4, "pc": pc}[a.workload]
+    run = {"c1": c1, "c2": c2, "c4": c4, "pc": pc}[a.workload]
     print(json.dumps(run(ctx)), flush=True)
 
 
