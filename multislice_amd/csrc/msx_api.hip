@@ -513,15 +513,26 @@ int transmission_dev(msx_ctx ctx, const typename real_of<C>::type* V, int nl, in
     st.bw = bw;
     SliceTables<C> tb;
     CHK(build_tables<C>(ctx, &st, false, &tb));
+    // Band pruning, exact as in the slice loop: the mask zeroes every y
+    // column with ky2 >= k2max, so the forward row sweep does not store
+    // those columns, the column sweep skips them, and the inverse row sweep
+    // takes them as zeros without loading them (and stores the full image).
+    r.ky2 = tb.ky2;
+    r.k2max = tb.k2max;
+    r.prune = 2;
     LAUNCH_ROW(ny, kRowFwd, r);
     ColArgs<C> c = col_args<C>(t, ny, nl, twx);
     c.kx2 = tb.kx2;
     c.ky2 = tb.ky2;
     c.k2max = tb.k2max;
+    c.prune = 1;
     c.ops = kColMask | kColStore;
     LAUNCH_COL(nx, kColSlice, c);
     RowArgs<C> r2 = row_args<C>(t, nx, nl, twy);
     r2.scale = (Re)1 / ((Re)nx * (Re)ny);
+    r2.ky2 = tb.ky2;
+    r2.k2max = tb.k2max;
+    r2.prune = 1;
     LAUNCH_ROW(ny, kRowInv, r2);
     return MSX_OK;
 }
