@@ -228,19 +228,22 @@ template <int NN, int TT, int... RR> struct PlanT {
     static constexpr int R[] = {RR...};
     static constexpr int P = sizeof...(RR);
 };
-// Row-sweep plans (contiguous lines; T divides 64 where possible so a line
-// stays inside one wavefront).
+// Row-sweep plans (contiguous lines).  Up to 512 T divides 64, so a line
+// stays inside one wavefront and its exchanges need only wavefront barriers;
+// at 1024 and 2048 a line spans 2 and 4 wavefronts (work-group barriers),
+// which with 8 elements per thread still beats 16 per thread in one
+// wavefront (three waves per SIMD instead of two).
 template <int N> struct Plan;
 template <> struct Plan<16> : PlanT<16, 4, 4, 4> {};
 template <> struct Plan<32> : PlanT<32, 4, 4, 8> {};
 template <> struct Plan<64> : PlanT<64, 8, 8, 8> {};
 template <> struct Plan<128> : PlanT<128, 8, 16, 8> {};
 template <> struct Plan<256> : PlanT<256, 16, 16, 16> {};
-template <> struct Plan<384> : PlanT<384, 16, 6, 8, 8> {};
+template <> struct Plan<384> : PlanT<384, 32, 6, 4, 4, 4> {};
 template <> struct Plan<512> : PlanT<512, 64, 8, 8, 8> {};
-template <> struct Plan<768> : PlanT<768, 32, 12, 8, 8> {};
-template <> struct Plan<1024> : PlanT<1024, 64, 16, 8, 8> {};
-template <> struct Plan<2048> : PlanT<2048, 128, 16, 16, 8> {};
+template <> struct Plan<768> : PlanT<768, 64, 12, 4, 4, 4> {};
+template <> struct Plan<1024> : PlanT<1024, 128, 8, 8, 4, 4> {};
+template <> struct Plan<2048> : PlanT<2048, 256, 8, 8, 8, 4> {};
 // Column-sweep plans: fewer threads per line (more elements per thread) so a
 // work-group of TW columns stays within 512 threads and its register budget.
 #ifndef MSX_COL512_T
@@ -248,6 +251,8 @@ template <> struct Plan<2048> : PlanT<2048, 128, 16, 16, 8> {};
 #endif
 template <int N> struct ColPlan : Plan<N> {};
 template <> struct ColPlan<512> : PlanT<512, MSX_COL512_T, 8, 8, 8> {};
+template <> struct ColPlan<1024> : PlanT<1024, 64, 16, 8, 8> {};
+template <> struct ColPlan<2048> : PlanT<2048, 128, 16, 16, 8> {};
 
 // Plan variants the autotuner chooses between (msx_tuning row_plan /
 // col_plan): variant 0 is the plan above; variant 1 exists where `alt` is
@@ -268,6 +273,34 @@ template <> struct RowPlanSel<512, 1> {
 };
 template <> struct ColPlanSel<512, 1> {
     using type = PlanT<512, MSX_COL512_T, 16, 16, 2>;
+    static constexpr bool alt = true;
+};
+// 384 and 768 with half the threads per line and one pass less (6 x 8 x 8
+// and 12 x 8 x 8: 24 elements per thread, whose registers allow one wave per
+// SIMD; the built-in plans hold 12 and run two)
+template <> struct RowPlanSel<384, 1> {
+    using type = PlanT<384, 16, 6, 8, 8>;
+    static constexpr bool alt = true;
+};
+template <> struct RowPlanSel<768, 1> {
+    using type = PlanT<768, 32, 12, 8, 8>;
+    static constexpr bool alt = true;
+};
+// 256 on twice the threads per line (8 elements per thread, three passes):
+// the faster row sweep for a few waves per launch (more work-groups per wave)
+template <> struct RowPlanSel<256, 1> {
+    using type = PlanT<256, 32, 8, 8, 4>;
+    static constexpr bool alt = true;
+};
+// 1024 and 2048 on half the threads per line and one pass less (16
+// elements per thread, two waves per SIMD; the built-in plans hold 8 and run
+// three, their lines spanning 2 and 4 wavefronts)
+template <> struct RowPlanSel<1024, 1> {
+    using type = PlanT<1024, 64, 16, 8, 8>;
+    static constexpr bool alt = true;
+};
+template <> struct RowPlanSel<2048, 1> {
+    using type = PlanT<2048, 128, 16, 16, 8>;
     static constexpr bool alt = true;
 };
 

@@ -4,6 +4,9 @@
   built-in 8 x 8 x 8) is another factorisation of the same transform: the
   slice loop and STEM agree with the oracle's FP64 loop to the FP32 bars of
   the built-in plan, in every combination of row and column plan;
+* the row sweep's second plan at 256, 384, 768, 1024 and 2048 (other thread
+  counts per line and radix sequences) meets the same bar against the
+  oracle's FP64 loop as the built-in plan;
 * msx_set_tuning refuses a plan the grid does not have.
 """
 import numpy as np
@@ -67,6 +70,28 @@ def test_alternative_plans_match_oracle(ctx, rp, cp):
     ea, eb = np.abs(a - o).max() / np.abs(o).max(), np.abs(b - o).max() / np.abs(o).max()
     print(f"STEM plans row {rp} col {cp}: max rel vs oracle {ea:.2e} (built-in plans {eb:.2e})")
     assert ea < 1e-5 and eb < 1e-5
+
+
+@pytest.mark.parametrize("n,nsl", [(256, 5), (384, 5), (768, 4), (1024, 3), (2048, 2)])
+def test_alternative_row_plans_match_oracle(ctx, n, nsl):
+    rng = np.random.default_rng(n)
+    t = np.exp(1j * 0.5 * rng.random((2, n, n)))
+    cz = rng.uniform(1.8, 2.1, 2)
+    layer = [s % 2 for s in range(nsl)]
+    a = 0.09 * n
+    w0 = np.ones((n, n), np.complex128)
+    ref = O.multislice(t, cz, layer, w0, a, a, KEV)
+    res = {}
+    for rp in (0, 1):
+        ctx.set_tuning(n, prec=msx.F32, tuning=dict(row_wpb=0, col_wpb=0, batch=0, row_plan=rp))
+        try:
+            psi = np.zeros((2, n, n), np.complex64)
+            ctx.multislice(t.astype(np.complex64), cz, layer, psi, a, a, KEV, psi0=w0.astype(np.complex64))
+        finally:
+            ctx.set_tuning(n, prec=msx.F32, tuning=None)
+        res[rp] = max(rel_l2(psi[0], ref), rel_l2(psi[1], ref))
+    print(f"{n}^2 row plans 0 / 1: rel-L2 vs oracle {res[0]:.2e} / {res[1]:.2e}")
+    assert res[1] < 1e-6 and res[1] < 2.0 * res[0] + 1e-7
 
 
 def test_set_tuning_refuses_what_the_grid_lacks(ctx):
