@@ -518,19 +518,17 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
     }
     // unit i of this work-group: (group, live mask, wave); false past the end
     const long long units = (long long)ngl * p.nb;
-    __shared__ unsigned qslot;
+    // device-side work queue (kColJobs): after its first unit a work-group
+    // takes the units it claims from the counter, one claim ahead: unit i+2
+    // is claimed while unit i is transformed and published in qslot[i & 1]
+    // at the end of that iteration (the claims end when the counter passes
+    // the unit count, so every work-group exits)
+    __shared__ unsigned qslot[2];
     auto unit = [&](int i, int& g, unsigned& m, int& b) -> bool {
         if (bal) {
             long long u = bid + (long long)i * gridDim.x;
             if constexpr (jobs) {
-                // device-side work queue: after its first unit a work-group
-                // claims the next free one (the claims end when the counter
-                // passes the unit count, so every work-group exits)
-                if (i > 0) {
-                    if (threadIdx.x == 0) qslot = gridDim.x + atomicAdd(p.queue, 1u);
-                    lds_barrier();
-                    u = (long long)__builtin_amdgcn_readfirstlane(qslot);
-                }
+                if (i > 0) u = (long long)__builtin_amdgcn_readfirstlane(qslot[i & 1]);
             }
             if (u >= units) return false;
             const int2 e = glist[(int)(u % ngl)];
@@ -601,6 +599,10 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
     int g, b;
     unsigned m;
     unit(0, g, m, b);  // every work-group that gets here has a unit
+    if constexpr (jobs) {
+        if (threadIdx.x == 0) qslot[1] = gridDim.x + atomicAdd(p.queue, 1u);
+        lds_barrier();
+    }
     C a[E], nxt[MSX_COL_PREFETCH ? E : 1];
     if (MSX_COL_PREFETCH && !gen) load(g, m, b, true, a);
     // Drain the constant loads (and the first tile) once: the wait counters
@@ -613,6 +615,12 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
         unsigned m2 = m;
         const bool more = unit(i + 1, g2, m2, b2);
         if (!more) g2 = g, m2 = m, b2 = b;
+        // the claim of unit i+2, issued before the prefetch so that reading
+        // its result at the end of this unit never waits for the prefetch
+        unsigned claim = 0;
+        if constexpr (jobs) {
+            if (threadIdx.x == 0 && more) claim = atomicAdd(p.queue, 1u);
+        }
         if constexpr (jobs) {
             // this unit's wave's factors, loaded before the prefetch of the
             // next unit (using them then never waits for it)
@@ -752,6 +760,10 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
             for (int e = 0; e < E; ++e) buf_store(a[e], rs, so + (unsigned)e * ostep * (unsigned)sizeof(C));
         }
         if (!more) break;
+        if constexpr (jobs) {
+            if (threadIdx.x == 0) qslot[i & 1] = gridDim.x + claim;
+            lds_barrier();
+        }
         if constexpr (MSX_COL_PREFETCH) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = nxt[e];
