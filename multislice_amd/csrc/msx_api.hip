@@ -965,9 +965,11 @@ int batch_class(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g, std
         }
     const int U = (int)pid.size();
     // one device block: transmission pointers [S][nb], propagator pointers
-    // [S][nb], the slices' work counters [S] (zero) and the factors [U][nx + ny]
+    // [S][nb], the slices' work counters [S][kQueueSlots] (zero: one per XCD,
+    // kQueueStride apart) and the factors [U][nx + ny]
     const size_t o_tt = 0, o_pt = al256(sizeof(void*) * S * nb), o_q = o_pt + al256(sizeof(void*) * S * nb),
-                 o_pf = o_q + al256(sizeof(unsigned) * S), tot = o_pf + sizeof(C) * (size_t)U * (nx + ny);
+                 o_pf = o_q + al256(sizeof(unsigned) * kQueueSlots * S),
+                 tot = o_pf + sizeof(C) * (size_t)U * (nx + ny);
     CHK(ctx->aux.ensure(tot));
     char* dbase = static_cast<char*>(ctx->aux.p);
     keep.emplace_back(tot, 0);  // host image, kept by the caller until the stream has consumed it
@@ -1039,7 +1041,7 @@ int batch_class(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g, std
         } else {
             c.ops = kColProp | kColJobs | kColStore;
             c.prop_tab = dpt + (size_t)s * nb;
-            c.queue = dq + s;
+            c.queue = dq + (size_t)s * kQueueSlots;
             c.wpb = (tu && tu->col_wpb < 0) ? tu->col_wpb : -1;
         }
         LAUNCH_COL(nx, kColSlice, c);

@@ -321,7 +321,8 @@ enum ColOp : int {
     kColStore = 32,  // write the result back
     kColObl = 64,    // with kColProp: the oblique-cell propagator (FP64 metric per element)
     kColJobs = 128,  // with kColProp: uneven jobs in one launch: per-wave propagator factors
-                     // (prop_tab) and units pulled from a device-side work counter (queue)
+                     // (prop_tab) and units pulled from device-side work counters (queue:
+                     // kXcdCount counters kQueueStride apart, zero at launch)
 };
 
 template <typename C> struct ColArgs {
@@ -402,8 +403,13 @@ template <typename C> __device__ __forceinline__ C expi_reduced(double ph) {
 #endif
 // Bijection hardware block id -> logical id such that the blocks one XCD
 // receives (ids congruent mod 8) cover a contiguous logical range.
+constexpr int kXcdCount = 8;
+// distance between the per-XCD work counters of one kColJobs launch
+// (unsigned; 128 B, one cache line each)
+constexpr int kQueueStride = 32;
+constexpr int kQueueSlots = kXcdCount * kQueueStride;  // unsigned per launch
 __device__ __forceinline__ int xcd_remap(int bid, int nblk) {
-    constexpr int kXcd = 8;
+    constexpr int kXcd = kXcdCount;
     const int per = nblk / kXcd, rem = nblk % kXcd;
     const int xcd = bid % kXcd, idx = bid / kXcd;
     return xcd * per + min(xcd, rem) + idx;
@@ -518,12 +524,33 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
     }
     // unit i of this work-group: (group, live mask, wave); false past the end
     const long long units = (long long)ngl * p.nb;
-    // device-side work queue (kColJobs): after its first unit a work-group
-    // takes the units it claims from the counter, one claim ahead: unit i+2
-    // is claimed while unit i is transformed and published in qslot[i & 1]
-    // at the end of that iteration (the claims end when the counter passes
-    // the unit count, so every work-group exits)
+    // device-side work queue (kColJobs), one counter per XCD: the units of
+    // the balanced schedule's later rounds that fall to an XCD's work-groups
+    // (consecutive column groups, whose shared cache lines then meet in that
+    // XCD's L2) are handed out by that XCD's counter in round order; after its
+    // first unit a work-group takes the units it claims, one claim ahead:
+    // unit i+2 is claimed while unit i is transformed and published in
+    // qslot[i & 1] at the end of that iteration.  The k-th claim on an XCD
+    // is unit (1 + k / W) G + base + k % W (W work-groups from logical index
+    // base on that XCD, G in all), which grows with k: the claims end when it
+    // passes the unit count, so every work-group exits.
     __shared__ unsigned qslot[2];
+    int q_w = 1, q_base = 0;
+    unsigned* q_ctr = p.queue;
+    if constexpr (jobs) {
+        if (MSX_XCD_REMAP && gridDim.x >= kXcdCount) {
+            const int x = (int)blockIdx.x % kXcdCount, per = (int)gridDim.x / kXcdCount,
+                      rem = (int)gridDim.x % kXcdCount;
+            q_w = per + (x < rem ? 1 : 0);
+            q_base = x * per + min(x, rem);
+            q_ctr = p.queue + x * kQueueStride;
+        } else {
+            q_w = (int)gridDim.x;
+        }
+    }
+    auto claim_unit = [&](unsigned k) -> unsigned {
+        return (1u + k / (unsigned)q_w) * gridDim.x + (unsigned)q_base + k % (unsigned)q_w;
+    };
     auto unit = [&](int i, int& g, unsigned& m, int& b) -> bool {
         if (bal) {
             long long u = bid + (long long)i * gridDim.x;
@@ -600,10 +627,15 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
     unsigned m;
     unit(0, g, m, b);  // every work-group that gets here has a unit
     if constexpr (jobs) {
-        if (threadIdx.x == 0) qslot[1] = gridDim.x + atomicAdd(p.queue, 1u);
+        if (threadIdx.x == 0) qslot[1] = claim_unit(atomicAdd(q_ctr, 1u));
         lds_barrier();
     }
     C a[E], nxt[MSX_COL_PREFETCH ? E : 1];
+    // kColJobs: the address of a unit's propagator factors is read one unit
+    // ahead (issued before the prefetch and the stores, so that using it at
+    // the next unit never waits for them)
+    const C* pnext = nullptr;
+    if constexpr (jobs) pnext = p.prop_tab[b];
     if (MSX_COL_PREFETCH && !gen) load(g, m, b, true, a);
     // Drain the constant loads (and the first tile) once: the wait counters
     // count in issue order, and a load still pending at the loop entry makes
@@ -619,12 +651,12 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
         // its result at the end of this unit never waits for the prefetch
         unsigned claim = 0;
         if constexpr (jobs) {
-            if (threadIdx.x == 0 && more) claim = atomicAdd(p.queue, 1u);
+            if (threadIdx.x == 0 && more) claim = atomicAdd(q_ctr, 1u);
         }
         if constexpr (jobs) {
             // this unit's wave's factors, loaded before the prefetch of the
             // next unit (using them then never waits for it)
-            const unsigned long long pa = (unsigned long long)p.prop_tab[b];
+            const unsigned long long pa = (unsigned long long)pnext;
             pxp = reinterpret_cast<const C*>(
                 ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(pa >> 32)) << 32) |
                 (unsigned)__builtin_amdgcn_readfirstlane((unsigned)pa));
@@ -633,6 +665,7 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
 #pragma unroll
                 for (int e = 0; e < E; ++e) pxc[e] = pxp[t + e * T];
             }
+            pnext = p.prop_tab[b2];
         }
         const float ky2 = load_ky2(g);
         const C py = load_py(g);
@@ -761,7 +794,7 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
         }
         if (!more) break;
         if constexpr (jobs) {
-            if (threadIdx.x == 0) qslot[i & 1] = gridDim.x + claim;
+            if (threadIdx.x == 0) qslot[i & 1] = claim_unit(claim);
             lds_barrier();
         }
         if constexpr (MSX_COL_PREFETCH) {
