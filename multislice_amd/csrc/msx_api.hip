@@ -1100,11 +1100,22 @@ template <typename C> int multislice_batch_impl(msx_ctx ctx, const msx_job* jobs
     }
     HIPCHK(hipEventRecord(ctx->lanes_go, ctx->stream));
     for (int l = 0; l < nl; l++) HIPCHK(hipStreamWaitEvent(ctx->lanes[l].stream, ctx->lanes_go, 0));
+    // longest work first, each class to the lane with the least work so far
+    // (work: pixel-slices of the class's jobs)
+    std::vector<std::pair<double, size_t>> order;
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        double w = 0.0;
+        for (const auto& it : groups[gi]) w += (double)it.st->nslice * it.st->nx * it.st->ny;
+        order.emplace_back(w, gi);
+    }
+    std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<double> load(nl, 0.0);
     int rc = MSX_OK;
-    for (size_t gi = 0; gi < groups.size() && rc == MSX_OK; gi++) {
-        const int l = (int)(gi % nl);
+    for (size_t k = 0; k < order.size() && rc == MSX_OK; k++) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[l] += order[k].first;
         ctx->swap_lane(l);
-        rc = batch_class<C>(ctx, jobs, groups[gi], keep, false);
+        rc = batch_class<C>(ctx, jobs, groups[order[k].second], keep, false);
         ctx->swap_lane(l);
     }
     // the context stream waits for every lane; the call returns when all are done
