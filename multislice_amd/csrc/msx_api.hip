@@ -209,7 +209,8 @@ struct msx_ctx_s {
         return MSX_OK;
     }
     std::map<int, DevBuf> tw32, tw64;
-    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab, binbuf, series;
+    DevBuf wave, wave2, trans, aux, tabs, partial, outbuf, atoms, cells, vtmp, curvebuf, radtab, radcoef, binbuf,
+        series;
     // host side of the potential's atom binning, kept across calls (no page
     // faults on reuse); the sorted records go to pinned memory for the upload
     std::vector<long long> h_cell;
@@ -296,7 +297,7 @@ struct msx_ctx_s {
         for (auto& kv : tw64) kv.second.release();
         drop_graphs();
         for (DevBuf* b : {&wave, &wave2, &trans, &aux, &tabs, &partial, &outbuf, &atoms, &cells, &vtmp, &curvebuf, &radtab,
-                           &binbuf, &series})
+                           &radcoef, &binbuf, &series})
             b->release();
         for (Lane& l : lanes) {
             l.wave.release();
@@ -1653,6 +1654,7 @@ int splat_binned(msx_ctx ctx, RadArgs& ra, const BinGeom& g, int nx, int ny, dou
     sa.table = table;
     sa.formula = formula;
     sa.rtab = nullptr;
+    sa.rcoef = nullptr;
     sa.rt_noct = 0;
     if (prec == MSX_F32 && ra.nsp > 0) {
         // FP32 output: V_Z(r^2) from radial tables (below FP32 resolution of
@@ -1663,8 +1665,11 @@ int splat_binned(msx_ctx ctx, RadArgs& ra, const BinGeom& g, int nx, int ny, dou
         ra.formula = formula;
         CHK(ctx->radtab.ensure(sizeof(double) * (size_t)ra.nsp * ((size_t)ra.noct * (kRadPts + 3) + 1)));
         ra.tab = static_cast<double*>(ctx->radtab.p);
+        CHK(ctx->radcoef.ensure(2 * sizeof(double2) * (size_t)ra.nsp * ra.noct * kRadPts));
+        ra.coef = static_cast<double2*>(ctx->radcoef.p);
         HIPCHK(launch_radial_tables(ra, ctx->stream));
         sa.rtab = ra.tab;
+        sa.rcoef = ra.coef;
         sa.rt_noct = ra.noct;
     }
     CHK(ctx->timed(MSX_PROF_SPLAT, (double)vbytes, [&] { return launch_splat(sa, ctx->stream); }));

@@ -99,6 +99,9 @@ struct AtomRec {
 // intervals), evaluated by the same FP64 formula as the exact splat without
 // its clamp r >= 1e-10 (so the tabulated function stays smooth), followed by
 // one entry holding the clamped value V_Z(1e-10) for s < 1e-20.
+// The splat reads the cubic through the four points around each interval as
+// power coefficients in t (position in the interval, 0..1): per species
+// [noct][kRadPts] intervals of (c0, c1) (c2, c3), built from the points.
 constexpr int kRadPts = 256;
 // lowest octave: s = r^2 from 2^-67 A^2, below 1e-20 = 2^-66.4
 constexpr int kRadOmin = -67;
@@ -110,10 +113,12 @@ struct SplatArgs {
     double a, b, radius;
     int nx, ny, nslices, nbx, nby, table, formula;
     const double* rtab;      // radial tables [nsp][rt_noct * (kRadPts + 3) + 1] (FP32 output), or nullptr: exact
+    const double2* rcoef;    // their interval coefficients [nsp][rt_noct * kRadPts][2]
     int rt_noct;
 };
 struct RadArgs {
-    double* tab;  // [nsp][noct * (kRadPts + 3) + 1]
+    double* tab;    // [nsp][noct * (kRadPts + 3) + 1]
+    double2* coef;  // [nsp][noct * kRadPts][2]
     int zs[104];  // Z of each species slot
     int nsp, noct, table, formula;
 };

@@ -88,27 +88,25 @@ __device__ __forceinline__ double d_vz(int Z, double r, int table, int formula) 
 
 // V_Z(s = r^2) from the radial table of species `slot` (s in [2^kRadOmin,
 // 2^(kRadOmin + noct))): the octave from the exponent of s, the position in
-// it from the mantissa (uniform spacing within an octave), cubic Lagrange
-// interpolation through the four surrounding points.  Relative error below
-// 3e-9 (99.9th percentile), up to 3e-8 next to the branch switches of the
-// reference's Bessel polynomials: below FP32 resolution.  Below s = 1e-20
-// (r < 1e-10, where the formula clamps r) the table's last entry holds the
-// clamped value.
+// it from the mantissa (uniform spacing within an octave), the cubic through
+// the four surrounding points (Lagrange interpolation, evaluated as
+// c0 + t (c1 + t (c2 + t c3)) from the interval's coefficients).  Relative
+// error below 3e-9 (99.9th percentile), up to 3e-8 next to the branch
+// switches of the reference's Bessel polynomials: below FP32 resolution.
+// Below s = 1e-20 (r < 1e-10, where the formula clamps r) the point table's
+// last entry holds the clamped value.
 __device__ __forceinline__ double radial_interp(const SplatArgs& p, int slot, double s) {
-    const long long per = (long long)p.rt_noct * (kRadPts + 3);
-    const double* base = p.rtab + (long long)slot * (per + 1);
-    if (s < 1.0e-20) return base[per];
+    if (s < 1.0e-20) return p.rtab[(long long)slot * ((long long)p.rt_noct * (kRadPts + 3) + 1) +
+                                   (long long)p.rt_noct * (kRadPts + 3)];
     int e;
     const double m = frexp(s, &e);  // s = m 2^e, m in [0.5, 1)
     const int o = e - 1 - kRadOmin;
     const double u = (2.0 * m - 1.0) * kRadPts;
     const int k = (int)u;
     const double t = u - k;
-    const double* T = base + (long long)o * (kRadPts + 3) + k;
-    const double p0 = T[0], p1 = T[1], p2 = T[2], p3 = T[3];
-    const double tm1 = t - 1.0, tm2 = t - 2.0, tp1 = t + 1.0;
-    return (-t * tm1 * tm2 * (1.0 / 6.0)) * p0 + (tp1 * tm1 * tm2 * 0.5) * p1 - (tp1 * t * tm2 * 0.5) * p2 +
-           (tp1 * t * tm1 * (1.0 / 6.0)) * p3;
+    const double2* c = p.rcoef + 2 * (((long long)slot * p.rt_noct + o) * kRadPts + k);
+    const double2 c01 = c[0], c23 = c[1];
+    return c01.x + t * (c01.y + t * (c23.x + t * c23.y));
 }
 
 // per species: noct * (kRadPts + 3) grid values of the unclamped formula,
@@ -128,9 +126,26 @@ __global__ __launch_bounds__(256) void radial_table_kernel(RadArgs p) {
     p.tab[i] = d_vz<false>(p.zs[sp], sqrt(s), p.table, p.formula);
 }
 
+// power coefficients of the cubic through the points at t = -1, 0, 1, 2
+// around each interval (the point table's entries k .. k + 3)
+__global__ __launch_bounds__(256) void radial_coef_kernel(RadArgs p) {
+    const long long per = (long long)p.noct * kRadPts;
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= per * p.nsp) return;
+    const int sp = (int)(i / per);
+    const long long rem = i % per;
+    const int o = (int)(rem / kRadPts), k = (int)(rem % kRadPts);
+    const double* q = p.tab + (long long)sp * ((long long)p.noct * (kRadPts + 3) + 1) + (long long)o * (kRadPts + 3) + k;
+    const double p0 = q[0], p1 = q[1], p2 = q[2], p3 = q[3];
+    p.coef[2 * i] = make_double2(p1, (-2.0 * p0 - 3.0 * p1 + 6.0 * p2 - p3) * (1.0 / 6.0));
+    p.coef[2 * i + 1] = make_double2((p0 - 2.0 * p1 + p2) * 0.5, (-p0 + 3.0 * p1 - 3.0 * p2 + p3) * (1.0 / 6.0));
+}
+
 hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
     const long long n = (long long)a.nsp * (a.noct * (kRadPts + 3) + 1);
     hipLaunchKernelGGL(radial_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+    const long long nc = (long long)a.nsp * a.noct * kRadPts;
+    hipLaunchKernelGGL(radial_coef_kernel, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
