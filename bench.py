@@ -150,6 +150,50 @@ def traffic_from_profile(batch, tuning):
     return int(round(d["hbm_bytes_per_launch"] * batch / d["batch"]))
 
 
+C3_DETECTOR = "70-200mrad"  # the ADF annulus of this workload, as the evaluation report names it
+
+
+def surrogate_report():
+    """The surrogate evaluation report committed under profiles/ that holds
+    per-detector results (the newest such report by name when there are
+    several), and its path; (None, None) when there is none."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "surrogate_eval_*.json")), reverse=True):
+        with open(path) as f:
+            d = json.load(f)
+        if "detectors" in d:
+            return d, path
+    return None, None
+
+
+def surrogate_accuracy(ev, src, out):
+    """Adds the report's accuracy on this workload's detector to `out` (the
+    rates of surrogate_rate) and withdraws the surrogate rate when the report
+    misses the 2% target: the metric counts converged images only."""
+    if ev is None:
+        out["meets_2pct_target"] = None
+        return out
+    det = ev["detectors"].get(C3_DETECTOR)
+    name = C3_DETECTOR if det else sorted(ev["detectors"])[0]
+    det = det or ev["detectors"][name]
+    met = det["surrogate_rel_rms_mean"] <= 0.02
+    out["meets_2pct_target"] = bool(met)
+    if not met:
+        out["surrogate_4config_unconverged"] = out.pop("surrogate_4config")
+        out["surrogate_4config"] = None
+    out["accuracy"] = {"source": os.path.relpath(src, ROOT), "detector": name,
+                       "held_out_specimens": ev["eval_specimens"],
+                       "surrogate_rel_rms_mean": round(det["surrogate_rel_rms_mean"], 4),
+                       "input_4config_rel_rms_mean": round(det["input_rel_rms_mean"], 4),
+                       "fraction_within_2pct": round(det["fraction_within_2pct_surrogate"], 3),
+                       "floor_64config_vs_256config_rel_rms_mean": round(det["floor_rel_rms_mean"], 4),
+                       "floor_fraction_within_2pct": round(det["fraction_floor_within_2pct"], 3),
+                       "setup": "SrTiO3/Si/Au-on-aC, 5-40 nm, 128^2 grid, 32x32 ADF scans, "
+                                f"{ev['network'].get('in_channels', 1)} input channels, measured against plain "
+                                "64-config multislice"}
+    return out
+
+
 def surrogate_rate(msx, torch, ctx, value):
     """Converged ADF images/hr of this workload (128x128 scan, 100 slices):
     plain 64-config multislice vs the surrogate (4 configs + the U-Net passes
@@ -159,15 +203,7 @@ def surrogate_rate(msx, torch, ctx, value):
     from the newest evaluation report committed under profiles/ (its own
     scaled-down specimen set, named there), for this workload's ADF detector
     (70-200 mrad, clipped by the 2/3 band limit) where the report has it."""
-    import glob
-    reps_ = sorted(glob.glob(os.path.join(ROOT, "profiles", "surrogate_eval_*.json")), key=os.path.getmtime)
-    ev = None
-    for path in reversed(reps_):
-        with open(path) as f:
-            d = json.load(f)
-        if "detectors" in d:
-            ev, src = d, path
-            break
+    ev, src = surrogate_report()
     cin = ev["network"].get("in_channels", 1) if ev else 1
     net = msx.UNet(ctx, 4, 32, seed=0, in_channels=cin)
     x = torch.rand((1, cin, SCAN, SCAN) if cin > 1 else (1, SCAN, SCAN), device="cuda") + 0.5
@@ -188,29 +224,7 @@ def surrogate_rate(msx, torch, ctx, value):
            "surrogate_4config": round(3600.0 / t_all, 3),
            "gpu_time_ratio": round(64 * per_config / t_all, 2),
            "unet_forward_ms_128px": round(1e3 * t_unet, 3), "unet_passes_per_image": passes}
-    if ev is None:
-        out["meets_2pct_target"] = None
-        return out
-    det = ev["detectors"].get(C3_DETECTOR)
-    name = C3_DETECTOR if det else sorted(ev["detectors"])[0]
-    det = det or ev["detectors"][name]
-    met = det["surrogate_rel_rms_mean"] <= 0.02
-    out["meets_2pct_target"] = bool(met)
-    if not met:
-        # the metric counts images converged to 2%: this rate is not one
-        out["surrogate_4config_unconverged"] = out.pop("surrogate_4config")
-        out["surrogate_4config"] = None
-    out["accuracy"] = {"source": os.path.relpath(src, ROOT), "detector": name,
-                       "held_out_specimens": ev["eval_specimens"],
-                       "surrogate_rel_rms_mean": round(det["surrogate_rel_rms_mean"], 4),
-                       "input_4config_rel_rms_mean": round(det["input_rel_rms_mean"], 4),
-                       "fraction_within_2pct": round(det["fraction_within_2pct_surrogate"], 3),
-                       "floor_64config_vs_256config_rel_rms_mean": round(det["floor_rel_rms_mean"], 4),
-                       "floor_fraction_within_2pct": round(det["fraction_floor_within_2pct"], 3),
-                       "setup": "SrTiO3/Si/Au-on-aC, 5-40 nm, 128^2 grid, 32x32 ADF scans, "
-                                f"{ev['network'].get('in_channels', 1)} input channels, measured against plain "
-                                "64-config multislice"}
-    return out
+    return surrogate_accuracy(ev, src, out)
 
 
 def main():
