@@ -1016,7 +1016,7 @@ int batch_class(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g, std
         r.ky2 = tb.ky2;
         r.k2max = tb.k2max;
         r.prune = (s == 0) ? 2 : 3;
-        if (tu) r.wpb = tu->row_wpb;
+        if (tu) r.wpb = tu->row_wpb, r.plan = tu->row_plan;
         if (s == 0) {
             LAUNCH_ROW(ny, kRowFwd, r);
         } else {
@@ -1029,6 +1029,7 @@ int batch_class(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g, std
         c.ky2 = tb.ky2;
         c.k2max = tb.k2max;
         c.tile = tu ? tu->col_tile : 0;
+        if (tu) c.plan = tu->col_plan;
         // one propagator for every running wave (frozen-phonon configurations
         // of one specimen): the plain column sweep with that factor array;
         // otherwise the per-wave tables and the work queue

@@ -238,7 +238,7 @@ template <> struct Plan<16> : PlanT<16, 4, 4, 4> {};
 template <> struct Plan<32> : PlanT<32, 4, 4, 8> {};
 template <> struct Plan<64> : PlanT<64, 8, 8, 8> {};
 template <> struct Plan<128> : PlanT<128, 8, 16, 8> {};
-template <> struct Plan<256> : PlanT<256, 16, 16, 16> {};
+template <> struct Plan<256> : PlanT<256, 32, 8, 8, 4> {};
 template <> struct Plan<384> : PlanT<384, 32, 6, 4, 4, 4> {};
 template <> struct Plan<512> : PlanT<512, 64, 8, 8, 8> {};
 template <> struct Plan<768> : PlanT<768, 64, 12, 4, 4, 4> {};
@@ -250,6 +250,7 @@ template <> struct Plan<2048> : PlanT<2048, 256, 8, 8, 8, 4> {};
 #define MSX_COL512_T 32
 #endif
 template <int N> struct ColPlan : Plan<N> {};
+template <> struct ColPlan<256> : PlanT<256, 16, 16, 16> {};
 template <> struct ColPlan<512> : PlanT<512, MSX_COL512_T, 8, 8, 8> {};
 template <> struct ColPlan<1024> : PlanT<1024, 64, 16, 8, 8> {};
 template <> struct ColPlan<2048> : PlanT<2048, 128, 16, 16, 8> {};
@@ -286,10 +287,12 @@ template <> struct RowPlanSel<768, 1> {
     using type = PlanT<768, 32, 12, 8, 8>;
     static constexpr bool alt = true;
 };
-// 256 on twice the threads per line (8 elements per thread, three passes):
-// the faster row sweep for a few waves per launch (more work-groups per wave)
+// 256 on half the threads per line (16 elements per thread, two passes):
+// the built-in plan's 8 lines per work-group (against 16) spread a few waves
+// over twice the work-groups (one wave: 7.7 against 9.5 us per row sweep),
+// at 3% cost for 48 waves (23.2 against 22.6 us)
 template <> struct RowPlanSel<256, 1> {
-    using type = PlanT<256, 32, 8, 8, 4>;
+    using type = PlanT<256, 16, 16, 16>;
     static constexpr bool alt = true;
 };
 // 1024 and 2048 on half the threads per line and one pass less (16
