@@ -137,7 +137,7 @@ def traffic_from_profile(batch, tuning):
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy
     probe) of the same launch shapes, scaled per wave to this batch (the
     sweep loads the same tiles for every wave); None when there is none."""
-    path = os.path.join(ROOT, "profiles", "col_sweep_stem512_b128_traffic.json")
+    path = os.path.join(ROOT, "profiles", "col_sweep_stem512_b160_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:

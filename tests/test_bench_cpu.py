@@ -32,7 +32,7 @@ def test_surrogate_report_and_accuracy():
 
 
 def test_traffic_scaled_from_the_committed_summary():
-    with open(os.path.join(ROOT, "profiles", "col_sweep_stem512_b128_traffic.json")) as f:
+    with open(os.path.join(ROOT, "profiles", "col_sweep_stem512_b160_traffic.json")) as f:
         d = json.load(f)
     shapes = dict(col_wpb=d["col_wpb"], col_tile=d["col_tile"], col_plan=0)
     assert bench.traffic_from_profile(d["batch"], shapes) == d["hbm_bytes_per_launch"]
