@@ -185,27 +185,18 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
             }
         } else {
             // buffer descriptor of wave b's image (empty when !valid: the
-            // loads then read zero and touch no memory); masked elements read
-            // element t (in band, fetched anyway) and are zeroed after the
-            // load: unconditional loads keep the prefetch free of
-            // branch-induced waits
+            // loads then read zero and touch no memory); out-of-band
+            // elements are loaded from past the descriptor's bound, which
+            // reads +0 and touches no memory, so they need no zeroing after
+            // the load (unconditional loads keep the prefetch free of
+            // branch-induced waits)
             const C* img = p.src ? p.src + (long long)__builtin_amdgcn_readfirstlane(b) * p.src_bstride
                                  : p.psi + (long long)__builtin_amdgcn_readfirstlane(b) * p.nx * N;
             const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
                 const_cast<C*>(img), 0, valid ? p.nx * N * (int)sizeof(C) : 0, kBufWord3);
 #pragma unroll
-            for (int e = 0; e < E; ++e) v[e] = buf_load<C>(rs, eo(((ld_mask >> e) & 1u) ? t + e * T : t) * (unsigned)sizeof(C));
-        }
-    };
-
-    // Out-of-band elements are loaded from a redirected in-band address and
-    // zeroed only when the registers are consumed: zeroing right after the
-    // load would make the prefetch wait for its data immediately.
-    auto mask_in = [&](C (&v)[E]) {
-        if (ld_mask != ~0u) {
-#pragma unroll
             for (int e = 0; e < E; ++e)
-                if (!((ld_mask >> e) & 1u)) v[e] = mk<C>((Re)0, (Re)0);
+                v[e] = buf_load<C>(rs, ((ld_mask >> e) & 1u) ? eo(t + e * T) * (unsigned)sizeof(C) : kBufDrop);
         }
     };
     C a[E], nxt[E];
@@ -215,7 +206,6 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
     constexpr bool PF = MSX_ROW_PREFETCH && !INTEN && !VREAL;
     if (PF && b0 < b1) {
         load(b0, a);
-        mask_in(a);
     }
     // jobs batched in one launch: each wave has its own transmission, whose
     // address is read one wave ahead (after the prefetch, so that using it
@@ -253,7 +243,6 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
             load(b + 1 < b1 ? b + 1 : b, nxt, b + 1 < b1);
         } else {
             load(b, a);
-            mask_in(a);
         }
         Re oi[INTEN ? E : 1];
         if constexpr (INTEN) {
@@ -302,7 +291,6 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
         if constexpr (PF) {
 #pragma unroll
             for (int e = 0; e < E; ++e) a[e] = nxt[e];
-            mask_in(a);
         }
     }
 }

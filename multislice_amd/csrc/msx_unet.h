@@ -23,9 +23,15 @@ struct ConvArgs {
     int relu;
     int out_c, out_c0;  // channel count and offset of the destination tensor
     int accumulate;
+    // kBwdWeight split over several slices of its reduction: per-slice
+    // partial weight gradients [splits][Cout][Cin * KS * KS], summed in a
+    // fixed order into out (conv_wgrad_scratch floats; unused otherwise)
+    float* partial;
 };
 
 hipError_t launch_conv(int mode, int ks, const ConvArgs& a, hipStream_t s);
+// floats of ConvArgs::partial that launch_conv(kBwdWeight, ks, a) needs (0: none)
+size_t conv_wgrad_scratch(int ks, const ConvArgs& a);
 hipError_t launch_avgpool_fwd(const float* x, float* y, long long nc, int H, int W, hipStream_t s);
 hipError_t launch_avgpool_bwd(const float* gy, float* gx, long long nc, int H, int W, hipStream_t s);
 hipError_t launch_upsample_fwd(const float* x, float* y, int N, int C, int H, int W, int Ct, int c0, hipStream_t s);

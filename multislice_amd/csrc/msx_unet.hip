@@ -64,6 +64,8 @@ struct msx_unet_s {
     float* ws = nullptr;
     size_t ws_bytes = 0;
     double* partial = nullptr;
+    float* wscr = nullptr;  // weight-gradient slices (conv_wgrad_scratch)
+    size_t wscr_n = 0;
     std::vector<float*> ea, eb, pl, cat, da, db, g_ea, g_eb, g_pl, g_cat, g_da, g_db;
     float *xin = nullptr, *tin = nullptr, *z = nullptr, *out = nullptr, *gz = nullptr;
     float* xbase = nullptr;  // channel mean of xin (xin itself with one channel)
@@ -86,6 +88,7 @@ struct msx_unet_s {
         if (pbuf) hipFree(pbuf);
         if (ws) hipFree(ws);
         if (partial) hipFree(partial);
+        if (wscr) hipFree(wscr);
     }
 
     // lay out every activation and gradient tensor for batch n of h x w images
@@ -193,6 +196,16 @@ struct msx_unet_s {
         a.y = y;
         a.g = g;
         a.out = grads() + c.w_off;
+        const size_t need = conv_wgrad_scratch(c.ks, a);
+        if (need > wscr_n) {
+            UHIP(hipStreamSynchronize(stream()));
+            if (wscr) hipFree(wscr);
+            wscr = nullptr;
+            wscr_n = 0;
+            UHIP(hipMalloc(&wscr, need * sizeof(float)));
+            wscr_n = need;
+        }
+        a.partial = wscr;
         UHIP(launch_conv(kBwdWeight, c.ks, a, stream()));
         UHIP(launch_bias_grad(g, y, c.relu, wn, c.cout, a.H * a.W, grads() + c.b_off, stream()));
         if (gx) {
@@ -395,8 +408,12 @@ MSX_API int msx_unet_conv(msx_ctx ctx, int mode, int ks, const float* x, const f
     a.relu = relu != 0;
     a.out_c = mode == 1 ? cin : cout;
     a.accumulate = 0;
-    UHIP(launch_conv(mode, ks, a, msx_ctx_stream(ctx)));
-    UHIP(hipStreamSynchronize(msx_ctx_stream(ctx)));
+    const size_t need = mode == kBwdWeight ? conv_wgrad_scratch(ks, a) : 0;
+    if (need) UHIP(hipMalloc(&a.partial, need * sizeof(float)));
+    hipError_t e = launch_conv(mode, ks, a, msx_ctx_stream(ctx));
+    if (e == hipSuccess) e = hipStreamSynchronize(msx_ctx_stream(ctx));
+    if (a.partial) hipFree(a.partial);
+    UHIP(e);
     return MSX_OK;
 }
 

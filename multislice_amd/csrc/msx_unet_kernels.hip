@@ -171,27 +171,57 @@ __global__ __launch_bounds__(256) void conv_gemm(ConvArgs p) {
             const int img = (int)(n / HW), pix = (int)(n % HW);
             float* dst = p.out + ((long long)img * p.out_c + p.out_c0 + m) * HW + pix;
             if (p.accumulate) *dst += c; else *dst = c;
+        } else if (gridDim.z > 1) {
+            // this reduction slice's share of dW[co][ci][tap]; wgrad_reduce
+            // adds the slices in order (no atomics: the same sums every run)
+            p.partial[(long long)blockIdx.z * M * N + m * N + n] = c;
         } else {
-            atomicAdd(p.out + m * N + n, c);  // dW[co][ci][tap], vector atomics
+            p.out[m * N + n] += c;  // dW[co][ci][tap], one writer per element
         }
     }
 }
 
-hipError_t launch_conv(int mode, int ks, const ConvArgs& a, hipStream_t s) {
+__global__ void wgrad_reduce(const float* __restrict__ part, float* out, long long mn, int splits) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mn) return;
+    float s = 0.f;
+    for (int z = 0; z < splits; ++z) s += part[(long long)z * mn + i];
+    out[i] += s;
+}
+
+namespace {
+void conv_gemm_shape(int mode, int ks, const ConvArgs& a, long long* M, long long* N, long long* K) {
     const int HW = a.H * a.W;
+    if (mode == kFwd) { *M = a.Cout; *N = (long long)a.N * HW; *K = (long long)a.Cin * ks * ks; }
+    else if (mode == kBwdData) { *M = a.Cin; *N = (long long)a.N * HW; *K = (long long)a.Cout * ks * ks; }
+    else { *M = a.Cout; *N = (long long)a.Cin * ks * ks; *K = (long long)a.N * HW; }
+}
+int conv_bm(long long M) { return M <= 32 ? 32 : 64; }
+// reduction slices of the weight gradient: enough work-groups to fill the
+// chip, each slice at least 4 K-steps long
+int wgrad_splits(long long M, long long N, long long K) {
+    const int bm = conv_bm(M), bn = 4096 / bm;
+    const long long tiles = ((M + bm - 1) / bm) * ((N + bn - 1) / bn);
+    long long want = 2048 / (tiles > 0 ? tiles : 1);
+    long long maxs = (K + 4 * BK - 1) / (4 * BK);
+    int splits = (int)(want < 1 ? 1 : (want > maxs ? maxs : want));
+    return splits < 1 ? 1 : splits;
+}
+}  // namespace
+
+size_t conv_wgrad_scratch(int ks, const ConvArgs& a) {
     long long M, N, K;
-    if (mode == kFwd) { M = a.Cout; N = (long long)a.N * HW; K = (long long)a.Cin * ks * ks; }
-    else if (mode == kBwdData) { M = a.Cin; N = (long long)a.N * HW; K = (long long)a.Cout * ks * ks; }
-    else { M = a.Cout; N = (long long)a.Cin * ks * ks; K = (long long)a.N * HW; }
-    const int bm = M <= 32 ? 32 : 64, bn = 4096 / bm;
-    int splits = 1;
-    if (mode == kBwdWeight) {
-        const long long tiles = ((M + bm - 1) / bm) * ((N + bn - 1) / bn);
-        long long want = 2048 / (tiles > 0 ? tiles : 1);
-        long long maxs = (K + 4 * BK - 1) / (4 * BK);
-        splits = (int)(want < 1 ? 1 : (want > maxs ? maxs : want));
-        if (splits < 1) splits = 1;
-    }
+    conv_gemm_shape(kBwdWeight, ks, a, &M, &N, &K);
+    const int splits = wgrad_splits(M, N, K);
+    return splits > 1 ? (size_t)splits * (size_t)(M * N) : 0;
+}
+
+hipError_t launch_conv(int mode, int ks, const ConvArgs& a, hipStream_t s) {
+    long long M, N, K;
+    conv_gemm_shape(mode, ks, a, &M, &N, &K);
+    const int bm = conv_bm(M), bn = 4096 / bm;
+    const int splits = mode == kBwdWeight ? wgrad_splits(M, N, K) : 1;
+    if (splits > 1 && !a.partial) return hipErrorInvalidValue;
     dim3 grid((unsigned)((N + bn - 1) / bn), (unsigned)((M + bm - 1) / bm), (unsigned)splits);
 #define MSX_CONV(MODE_, KS_, BM_) hipLaunchKernelGGL((conv_gemm<MODE_, KS_, BM_>), grid, dim3(256), 0, s, a)
 #define MSX_CONV_BM(MODE_, KS_)                    \
@@ -212,6 +242,11 @@ hipError_t launch_conv(int mode, int ks, const ConvArgs& a, hipStream_t s) {
     }
 #undef MSX_CONV_BM
 #undef MSX_CONV
+    if (splits > 1) {
+        const long long mn = M * N;
+        hipLaunchKernelGGL(wgrad_reduce, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, a.partial, a.out, mn,
+                           splits);
+    }
     return hipGetLastError();
 }
 
