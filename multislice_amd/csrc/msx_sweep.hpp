@@ -221,7 +221,9 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
                                                  p.nx * N * (int)sizeof(Re), kBufWord3);
     };
     const unsigned row0 = (unsigned)ix * (unsigned)N;
-    for (int b = b0; b < b1; ++b) {
+    // one wave: its row is in cur (with the prefetch) or loaded here; the
+    // next wave's row is prefetched into next
+    auto wave = [&](int b, C (&cur)[E], C (&next)[E]) {
         if (p.trans_tab) {
             // Loaded before the prefetch, through a buffer descriptor, and
             // scaled only where it is used: the wait counters count in issue
@@ -240,9 +242,9 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
         if constexpr (PF) {
             // unconditional: past the last wave of the chunk the descriptor
             // is empty
-            load(b + 1 < b1 ? b + 1 : b, nxt, b + 1 < b1);
+            load(b + 1 < b1 ? b + 1 : b, next, b + 1 < b1);
         } else {
-            load(b, a);
+            load(b, cur);
         }
         Re oi[INTEN ? E : 1];
         if constexpr (INTEN) {
@@ -254,29 +256,29 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
             }
         }
         if (p.trans_tab) tnext = p.trans_tab[b + 1 < b1 ? b + 1 : b];
-        if constexpr (IN != 0) fft_line<PL, IN, WAVE>(a, t, lds, tw);
+        if constexpr (IN != 0) fft_line<PL, IN, WAVE>(cur, t, lds, tw);
         if (p.scale != (Re)1 && !fold) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) a[e] = cscale(a[e], p.scale);
+            for (int e = 0; e < E; ++e) cur[e] = cscale(cur[e], p.scale);
         }
         if (p.trans || p.trans_tab) {
             // cfpix::operator*= (cfpix.cpp:296-318): (wr*tr - wi*ti, wr*ti + wi*tr)
             const bool scale_here = p.trans_tab && fold;
 #pragma unroll
-            for (int e = 0; e < E; ++e) a[e] = cmul(a[e], scale_here ? cscale(tr[e], p.scale) : tr[e]);
+            for (int e = 0; e < E; ++e) cur[e] = cmul(cur[e], scale_here ? cscale(tr[e], p.scale) : tr[e]);
         }
         if constexpr (INTEN) {
             // buffer stores (dead rows aim past the bound)
             const __amdgpu_buffer_rsrc_t ri = inten_rsrc(b);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                Re v = a[e].x * a[e].x + a[e].y * a[e].y;
+                Re v = cur[e].x * cur[e].x + cur[e].y * cur[e].y;
                 if (p.iacc != 0) v = p.iw * v;
                 if (p.iacc == 2) v = oi[e] + v;
                 buf_store(v, ri, live ? (row0 + (unsigned)(t + e * T)) * (unsigned)sizeof(Re) : kBufDrop);
             }
         }
-        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(a, t, lds, tw);
+        if constexpr (OUT != 0) fft_line<PL, OUT, WAVE>(cur, t, lds, tw);
         {
             // buffer stores into wave b's image; dead rows, pruned elements
             // and a sweep that stores nothing aim past the bound (dropped):
@@ -286,12 +288,18 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
                 img, 0, p.nostore ? 0 : p.nx * N * (int)sizeof(C), kBufWord3);
 #pragma unroll
             for (int e = 0; e < E; ++e)
-                buf_store(a[e], rs, (live && ((st_mask >> e) & 1u)) ? eo(t + e * T) * (unsigned)sizeof(C) : kBufDrop);
+                buf_store(cur[e], rs, (live && ((st_mask >> e) & 1u)) ? eo(t + e * T) * (unsigned)sizeof(C) : kBufDrop);
         }
-        if constexpr (PF) {
-#pragma unroll
-            for (int e = 0; e < E; ++e) a[e] = nxt[e];
+    };
+    if constexpr (PF) {
+        // two waves per trip with the row buffers swapping roles: no copy of
+        // the prefetched row into the working registers
+        for (int b = b0; b < b1; b += 2) {
+            wave(b, a, nxt);
+            if (b + 1 < b1) wave(b + 1, nxt, a);
         }
+    } else {
+        for (int b = b0; b < b1; ++b) wave(b, a, nxt);
     }
 }
 
