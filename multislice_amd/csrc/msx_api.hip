@@ -1050,6 +1050,7 @@ int batch_class(msx_ctx ctx, const msx_job* jobs, std::vector<BatchItem>& g, std
         const int done = na - active[s + 1];  // jobs whose last slice this was
         if (done > 0) {
             RowArgs<C> f = row_args<C>(w + (long long)active[s + 1] * img, nx, done, twy);
+            if (tu) f.plan = tu->row_plan;  // the single-job loop's final sweep
             f.scale = inv;
             f.ky2 = tb.ky2;
             f.k2max = tb.k2max;

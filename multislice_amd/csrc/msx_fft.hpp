@@ -37,8 +37,17 @@ __device__ __forceinline__ C mk(typename real_of<C>::type x, typename real_of<C>
 }
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return mk<C>(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float fused(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fused(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// (a.x b.x - a.y b.y, a.x b.y + a.y b.x) with the fused multiply-adds spelled
+// out: left to the compiler, which product it fuses can differ between
+// kernels that inline the same code (and so can the last bit of the result)
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
-    return mk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return mk<C>(fused(a.x, b.x, -(a.y * b.y)), fused(a.x, b.y, a.y * b.x));
+}
+// |a|^2 with the same pinned rounding
+template <typename C> __device__ __forceinline__ typename real_of<C>::type cnorm2(C a) {
+    return fused(a.x, a.x, a.y * a.y);
 }
 template <typename C> __device__ __forceinline__ C cscale(C a, typename real_of<C>::type s) {
     return mk<C>(a.x * s, a.y * s);

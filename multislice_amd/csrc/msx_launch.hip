@@ -212,7 +212,9 @@ template <typename C> hipError_t MSX_CAT(launch_col_, MSX_N)(int mode, const Col
             if (ops == (kColProp | kColObl)) return col_go<C, 1, -1, kColProp | kColObl>(a, s);
             if (ops == (kColProp | kColJobs)) {
                 if (!a.prop_tab || !a.queue) return hipErrorInvalidValue;
-                return col_go<C, 1, -1, kColProp | kColJobs>(a, s);
+                // the plan of the single-job loop, so that a job gives the
+                // same bits alone and batched on a tuned grid
+                return col_go_plan<C, 1, -1, kColProp | kColJobs>(a, s);
             }
             if (ops == kPD) return col_go_plan<C, 1, -1, kPD>(a, s);
             if (ops == kColMask) return col_go<C, 1, -1, kColMask>(a, s);

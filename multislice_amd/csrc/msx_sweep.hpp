@@ -272,7 +272,7 @@ __global__ __launch_bounds__((RowPlanSel<N, PV>::type::T * LPW)) void row_sweep(
             const __amdgpu_buffer_rsrc_t ri = inten_rsrc(b);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                Re v = cur[e].x * cur[e].x + cur[e].y * cur[e].y;
+                Re v = cnorm2(cur[e]);
                 if (p.iacc != 0) v = p.iw * v;
                 if (p.iacc == 2) v = oi[e] + v;
                 buf_store(v, ri, live ? (row0 + (unsigned)(t + e * T)) * (unsigned)sizeof(Re) : kBufDrop);
@@ -737,7 +737,7 @@ __global__ __launch_bounds__((ColPlanSel<N, PV>::type::T * TW), MSX_COL_MINW) vo
                     }
                 }
                 if constexpr ((OPS & kColDetect) != 0) {
-                    const Re v = a[e].x * a[e].x + a[e].y * a[e].y;
+                    const Re v = cnorm2(a[e]);
 #pragma unroll
                     for (int d = 0; d < kMaxDet; ++d) {
                         if (d < p.ndet && k2 >= p.dlo[d] && k2 <= p.dhi[d]) {

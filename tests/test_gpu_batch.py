@@ -42,6 +42,26 @@ def run_batch(ctx, jobs, cdtype):
     return [j["psi"].cpu().numpy() for j in out]
 
 
+@pytest.mark.parametrize("rp,cp", [(1, 0), (0, 1), (1, 1)])
+def test_batch_equals_single_jobs_on_tuned_plans(ctx, rp, cp):
+    """With a tuning that selects the alternative FFT plans of a grid, the
+    batched jobs (per-wave propagators: the work-queue column sweep) still
+    equal each job run alone bit for bit."""
+    import torch
+    rng = np.random.default_rng(9)
+    jobs = make_jobs(rng, 4, [512], "f32")
+    jobs[0]["kev"], jobs[1]["kev"] = 200.0, 300.0  # two propagators: the work-queue sweep
+    ctx.set_tuning(512, prec=msx.F32, tuning=dict(row_wpb=8, col_wpb=-1, batch=0, col_tile=8, row_plan=rp, col_plan=cp))
+    try:
+        got = run_batch(ctx, jobs, torch.complex64)
+        for j, g in zip(jobs, got):
+            psi = torch.ones((j["n"], j["n"]), dtype=torch.complex64, device="cuda")
+            ctx.multislice(j["trans"], j["cz"], j["layer"], psi, j["ax"], j["by"], j["kev"])
+            np.testing.assert_array_equal(g, psi.cpu().numpy())
+    finally:
+        ctx.set_tuning(512, prec=msx.F32, tuning=None)
+
+
 def test_batch_equals_single_jobs(ctx):
     import torch
     rng = np.random.default_rng(4)
