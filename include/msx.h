@@ -99,7 +99,8 @@ MSX_API int msx_set_graphs(msx_ctx ctx, int enable);
  * smaller within 1%);
  * then, at that batch, the FFT plan of each sweep (0 = the built-in radix
  * sequence, 1 = the alternative where the line length has one: both sweeps
- * at 512 (16 x 16 x 2 against the built-in 8 x 8 x 8), the row sweep at 256,
+ * at 512 (rows 16 x 16 x 2, columns 4 x 4 x 4 x 8, against the built-in
+ * 8 x 8 x 8), the row sweep at 256,
  * 384, 768, 1024 and 2048 (other thread counts per line and radix
  * sequences)).  ms_per_slice is the time of one wave
  * through one slice with the chosen shapes.  msx_set_tuning installs a saved

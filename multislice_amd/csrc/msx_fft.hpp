@@ -256,7 +256,7 @@ template <> struct Plan<2048> : PlanT<2048, 256, 8, 8, 8, 4> {};
 // Column-sweep plans: fewer threads per line (more elements per thread) so a
 // work-group of TW columns stays within 512 threads and its register budget.
 #ifndef MSX_COL512_T
-#define MSX_COL512_T 32
+#define MSX_COL512_T 64
 #endif
 template <int N> struct ColPlan : Plan<N> {};
 template <> struct ColPlan<256> : PlanT<256, 16, 16, 16> {};
@@ -275,14 +275,16 @@ template <int N, int V> struct ColPlanSel {
     using type = ColPlan<N>;
     static constexpr bool alt = false;
 };
-// 512 = 16 x 16 x 2 on 32 threads per line: radix-16 butterflies (one
+// rows: 512 = 16 x 16 x 2 on 32 threads per line: radix-16 butterflies (one
 // twiddle stage of 15 factors instead of two of 7) and 16 elements per thread
 template <> struct RowPlanSel<512, 1> {
     using type = PlanT<512, 32, 16, 16, 2>;
     static constexpr bool alt = true;
 };
+// columns (the same threads per line as the built-in plan, which the tile
+// widths assume): 512 = 4 x 4 x 4 x 8, one exchange more, radix-4 butterflies
 template <> struct ColPlanSel<512, 1> {
-    using type = PlanT<512, MSX_COL512_T, 16, 16, 2>;
+    using type = PlanT<512, MSX_COL512_T, 4, 4, 4, 8>;
     static constexpr bool alt = true;
 };
 // 384 and 768 with half the threads per line and one pass less (6 x 8 x 8

@@ -1,7 +1,8 @@
 """FFT plans the autotuner chooses between, beyond the work-group shapes:
 
-* the alternative FFT plan of each sweep (512 = 16 x 16 x 2 against the
-  built-in 8 x 8 x 8) is another factorisation of the same transform: the
+* the alternative FFT plan of each sweep (512: rows 16 x 16 x 2 on 32
+  threads per line, columns 4 x 4 x 4 x 8, against the built-in 8 x 8 x 8
+  on 64 threads) is another factorisation of the same transform: the
   slice loop and STEM agree with the oracle's FP64 loop to the FP32 bars of
   the built-in plan, in every combination of row and column plan;
 * the row sweep's second plan at 256, 384, 768, 1024 and 2048 (other thread
