@@ -441,7 +441,9 @@ MSX_API int msx_multislice_noalloc(msx_ctx ctx, const msx_stack* stack, const vo
  * kV, tilt and slice thicknesses: one row and one column sweep per slice for
  * all of their waves still running, each wave with its own transmission and
  * propagator, the column sweep's persistent work-groups taking (wave, column
- * group) units from a device-side work counter.  Device buffers only (where
+ * group) units from device-side work counters (one per XCD, so that
+ * neighbouring column groups stay on one L2).  A tuned grid's FFT plans
+ * apply to the batch as to msx_multislice.  Device buffers only (where
  * = MSX_DEVICE); psi0 may be NULL (psi holds the incident wave). */
 typedef struct msx_job {
     const msx_stack* stack;
