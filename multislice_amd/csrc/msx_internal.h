@@ -115,6 +115,9 @@ struct SplatArgs {
     const double* rtab;      // radial tables [nsp][rt_noct * (kRadPts + 3) + 1] (FP32 output), or nullptr: exact
     const double2* rcoef;    // their interval coefficients [nsp][rt_noct * kRadPts][2]
     int rt_noct;
+    // filled by launch_splat: pixel and bin widths (a / nx, b / ny, a / nbx,
+    // b / nby; IEEE division on the host gives the kernel's own quotients)
+    double dx, dy, bwx, bwy;
 };
 struct RadArgs {
     double* tab;    // [nsp][noct * (kRadPts + 3) + 1]

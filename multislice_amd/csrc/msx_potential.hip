@@ -86,26 +86,28 @@ __device__ __forceinline__ double d_vz(int Z, double r, int table, int formula) 
     return 300.73079394295 * suml + 150.36539697148 * sumg;
 }
 
-// V_Z(s = r^2) from the radial table of species `slot` (s in [2^kRadOmin,
-// 2^(kRadOmin + noct))): the octave from the exponent of s, the position in
-// it from the mantissa (uniform spacing within an octave), the cubic through
-// the four surrounding points (Lagrange interpolation, evaluated as
+// V_Z(s = r^2) from the radial table of one species (s in [2^kRadOmin,
+// 2^(kRadOmin + noct))), `cs` its interval coefficients and `clamped` its
+// value at the clamp: the octave from the exponent of s, the position in it
+// from the mantissa (uniform spacing within an octave), the cubic through the
+// four surrounding points (Lagrange interpolation, evaluated as
 // c0 + t (c1 + t (c2 + t c3)) from the interval's coefficients).  Relative
 // error below 3e-9 (99.9th percentile), up to 3e-8 next to the branch
 // switches of the reference's Bessel polynomials: below FP32 resolution.
 // Below s = 1e-20 (r < 1e-10, where the formula clamps r) the point table's
 // last entry holds the clamped value.
-__device__ __forceinline__ double radial_interp(const SplatArgs& p, int slot, double s) {
-    if (s < 1.0e-20) return p.rtab[(long long)slot * ((long long)p.rt_noct * (kRadPts + 3) + 1) +
-                                   (long long)p.rt_noct * (kRadPts + 3)];
-    int e;
-    const double m = frexp(s, &e);  // s = m 2^e, m in [0.5, 1)
-    const int o = e - 1 - kRadOmin;
-    const double u = (2.0 * m - 1.0) * kRadPts;
-    const int k = (int)u;
-    const double t = u - k;
-    const double2* c = p.rcoef + 2 * (((long long)slot * p.rt_noct + o) * kRadPts + k);
-    const double2 c01 = c[0], c23 = c[1];
+// With kRadPts = 2^8 intervals per octave the IEEE fields of s give both
+// parts: bits 63..44 (sign 0, biased exponent, the mantissa's top 8 bits)
+// are octave * kRadPts + interval once offset by (1023 + kRadOmin) * kRadPts,
+// and the 44 mantissa bits below are t, exactly (1.f - 1 with f those bits
+// moved to the top).
+static_assert(kRadPts == 256, "radial_interp takes the interval from 8 mantissa bits");
+__device__ __forceinline__ double radial_interp(const double2* __restrict__ cs, const double* clamped, double s) {
+    if (s < 1.0e-20) return *clamped;
+    const unsigned long long u = (unsigned long long)__double_as_longlong(s);
+    const unsigned idx = (unsigned)(u >> 44) - (unsigned)((1023 + kRadOmin) * kRadPts);
+    const double t = __longlong_as_double((long long)(0x3FF0000000000000ull | ((u << 8) & 0x000FFFFFFFFFFFFFull))) - 1.0;
+    const double2 c01 = cs[2 * idx], c23 = cs[2 * idx + 1];
     return c01.x + t * (c01.y + t * (c23.x + t * c23.y));
 }
 
@@ -159,13 +161,26 @@ hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
 // whose minimum-image distance to the tile centre allows a pixel of the tile
 // within the cutoff (a conservative bound: every pixel-atom pair it drops
 // fails the per-pixel test), and compacts them in LDS in that order
-// (wavefront ballots).  Every thread then sums over the list: the same pairs
+// (wavefront ballots).  Every thread then sums over that list: the same pairs
 // in the same order as testing every atom of the bins, so the potential is
 // bit-identical, with the atom loop reading LDS instead of per-wavefront
 // scalar loads and branches.
+// A kept atom is tagged with its species slot and with whether any pixel of
+// the tile can take its periodic image (|x_atom - x_pixel| >= a/2 on an axis,
+// tested against the tile's extreme pixels with a margin far above
+// rounding).  Untagged atoms, nearly all of them, skip the minimum-image
+// selects: the signed differences square to the same bits.
+struct SplatCand {
+    double x, y;
+    float occ;
+    int16_t Z;
+    uint16_t tag;  // slot << 1 | periodic image possible
+};
+static_assert(sizeof(SplatCand) == 24, "candidate record");
+
 template <bool TAB>
 __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
-    __shared__ AtomRec cand[256];
+    __shared__ SplatCand cand[256];
     __shared__ int wcnt[4];
     // each wavefront covers an 8 x 8 quarter of the 16 x 16 tile: a compact
     // footprint leaves fewer lanes idle outside the radius of an atom the
@@ -174,11 +189,11 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     const int iy = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
     const int ix = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
     const int kz = blockIdx.z;
-    const double dx = p.a / p.nx, dy = p.b / p.ny;
+    const double dx = p.dx, dy = p.dy;
     const double xp = ix * dx, yp = iy * dy;
     const double r2max = p.radius * p.radius;
     // bins reached by this tile
-    const double bwx = p.a / p.nbx, bwy = p.b / p.nby;
+    const double bwx = p.bwx, bwy = p.bwy;
     const double xlo = (double)(blockIdx.y * 16) * dx - p.radius, xhi = (double)(blockIdx.y * 16 + 15) * dx + p.radius;
     const double ylo = (double)(blockIdx.x * 16) * dy - p.radius, yhi = (double)(blockIdx.x * 16 + 15) * dy + p.radius;
     int bxa = (int)floor(xlo / bwx), bxb = (int)floor(xhi / bwx);
@@ -191,6 +206,13 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     const double cx = (double)(blockIdx.y * 16) * dx + 7.5 * dx, cy = (double)(blockIdx.x * 16) * dy + 7.5 * dy;
     const double hx = 7.5 * dx + p.radius + 1e-6, hy = 7.5 * dy + p.radius + 1e-6;
     const bool fx = hx < 0.5 * p.a, fy = hy < 0.5 * p.b;
+    // the tile's extreme pixel positions (every xp, yp of the tile lies
+    // between them) and the image test's bounds
+    const double x0 = (double)(blockIdx.y * 16) * dx, x1 = (double)(blockIdx.y * 16 + 15) * dx;
+    const double y0 = (double)(blockIdx.x * 16) * dy, y1 = (double)(blockIdx.x * 16 + 15) * dy;
+    const double wxm = 0.5 * p.a * (1.0 - 1e-9), wym = 0.5 * p.b * (1.0 - 1e-9);
+    const long long cstride = 2ll * p.rt_noct * kRadPts;
+    const long long tstride = (long long)p.rt_noct * (kRadPts + 3) + 1;
     double v = 0.0;
     for (int bx = bxa; bx <= bxb; ++bx) {
         const int bxw = ((bx % p.nbx) + p.nbx) % p.nbx;
@@ -218,20 +240,33 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
                 __syncthreads();
                 int off = 0;
                 for (int w = 0; w < wv; ++w) off += wcnt[w];
-                if (keep) cand[off + __popcll(bal & ((1ull << lane) - 1ull))] = at;
+                if (keep) {
+                    const bool img = !(fmax(fabs(at.x - x0), fabs(at.x - x1)) < wxm) ||
+                                     !(fmax(fabs(at.y - y0), fabs(at.y - y1)) < wym);
+                    cand[off + __popcll(bal & ((1ull << lane) - 1ull))] =
+                        SplatCand{at.x, at.y, at.occ, at.Z, (uint16_t)((at.slot << 1) | (img ? 1 : 0))};
+                }
                 const int n = __builtin_amdgcn_readfirstlane(wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3]);
                 __syncthreads();
                 for (int k = 0; k < n; ++k) {
-                    const AtomRec c = cand[k];
-                    double dX = fabs(c.x - xp);
-                    if (dX >= p.a / 2.0) dX -= p.a;
-                    double dY = fabs(c.y - yp);
-                    if (dY >= p.b / 2.0) dY -= p.b;
-                    const double r2 = dX * dX + dY * dY;
+                    const SplatCand c = cand[k];
+                    const int tag = __builtin_amdgcn_readfirstlane((int)c.tag);
+                    double dX = c.x - xp, dY = c.y - yp;
+                    if (tag & 1) {
+                        dX = fabs(dX);
+                        if (dX >= p.a / 2.0) dX -= p.a;
+                        dY = fabs(dY);
+                        if (dY >= p.b / 2.0) dY -= p.b;
+                    }
+                    const double r2 = fma(dX, dX, dY * dY);
                     if (r2 < r2max) {
                         double vz;
-                        if constexpr (TAB) vz = radial_interp(p, c.slot, r2);
-                        else vz = d_vz(c.Z, sqrt(r2), p.table, p.formula);
+                        if constexpr (TAB) {
+                            const int slot = tag >> 1;
+                            vz = radial_interp(p.rcoef + slot * cstride, p.rtab + slot * tstride + tstride - 1, r2);
+                        } else {
+                            vz = d_vz(c.Z, sqrt(r2), p.table, p.formula);
+                        }
                         v += (double)c.occ * vz;
                     }
                 }
@@ -245,7 +280,12 @@ __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     }
 }
 
-hipError_t launch_splat(const SplatArgs& a, hipStream_t s) {
+hipError_t launch_splat(const SplatArgs& args, hipStream_t s) {
+    SplatArgs a = args;
+    a.dx = a.a / a.nx;
+    a.dy = a.b / a.ny;
+    a.bwx = a.a / a.nbx;
+    a.bwy = a.b / a.nby;
     dim3 grid((a.ny + 15) / 16, (a.nx + 15) / 16, a.nslices);
     if (a.rtab) hipLaunchKernelGGL(splat_kernel<true>, grid, dim3(256), 0, s, a);
     else hipLaunchKernelGGL(splat_kernel<false>, grid, dim3(256), 0, s, a);
