@@ -87,27 +87,27 @@ __device__ __forceinline__ double d_vz(int Z, double r, int table, int formula) 
 }
 
 // V_Z(s = r^2) from the radial table of one species (s in [2^kRadOmin,
-// 2^(kRadOmin + noct))), `cs` its interval coefficients and `clamped` its
-// value at the clamp: the octave from the exponent of s, the position in it
-// from the mantissa (uniform spacing within an octave), the cubic through the
-// four surrounding points (Lagrange interpolation, evaluated as
+// 2^(kRadOmin + noct))): the octave from the exponent of s, the position in
+// it from the mantissa (uniform spacing within an octave), the cubic through
+// the four surrounding points (Lagrange interpolation, evaluated as
 // c0 + t (c1 + t (c2 + t c3)) from the interval's coefficients).  Relative
 // error below 3e-9 (99.9th percentile), up to 3e-8 next to the branch
 // switches of the reference's Bessel polynomials: below FP32 resolution.
 // Below s = 1e-20 (r < 1e-10, where the formula clamps r) the point table's
 // last entry holds the clamped value.
-// With kRadPts = 2^8 intervals per octave the IEEE fields of s give both
-// parts: bits 63..44 (sign 0, biased exponent, the mantissa's top 8 bits)
-// are octave * kRadPts + interval once offset by (1023 + kRadOmin) * kRadPts,
-// and the 44 mantissa bits below are t, exactly (1.f - 1 with f those bits
-// moved to the top).
-static_assert(kRadPts == 256, "radial_interp takes the interval from 8 mantissa bits");
-__device__ __forceinline__ double radial_interp(const double2* __restrict__ cs, const double* clamped, double s) {
-    if (s < 1.0e-20) return *clamped;
-    const unsigned long long u = (unsigned long long)__double_as_longlong(s);
-    const unsigned idx = (unsigned)(u >> 44) - (unsigned)((1023 + kRadOmin) * kRadPts);
-    const double t = __longlong_as_double((long long)(0x3FF0000000000000ull | ((u << 8) & 0x000FFFFFFFFFFFFFull))) - 1.0;
-    const double2 c01 = cs[2 * idx], c23 = cs[2 * idx + 1];
+// With kRadPts = 2^8 intervals per octave the IEEE fields of s (bits u) give
+// both parts: bits 63..44 (sign 0, biased exponent, the mantissa's top 8
+// bits) are octave * kRadPts + interval once offset by
+// (1023 + kRadOmin) * kRadPts, and the 44 mantissa bits below are t, exactly
+// (1.f - 1 with f those bits moved to the top).
+static_assert(kRadPts == 256, "the radial interval is read from 8 mantissa bits");
+__device__ __forceinline__ unsigned rad_interval(unsigned long long u) {
+    return (unsigned)(u >> 44) - (unsigned)((1023 + kRadOmin) * kRadPts);
+}
+__device__ __forceinline__ double rad_position(unsigned long long u) {
+    return __longlong_as_double((long long)(0x3FF0000000000000ull | ((u << 8) & 0x000FFFFFFFFFFFFFull))) - 1.0;
+}
+__device__ __forceinline__ double rad_cubic(double2 c01, double2 c23, double t) {
     return c01.x + t * (c01.y + t * (c23.x + t * c23.y));
 }
 
@@ -154,22 +154,26 @@ hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
 // TAB: V_Z from the radial tables (FP32 output); otherwise the exact FP64
 // formula per pair (separate instantiations: the table kernel carries none of
 // the Bessel code).
-// A work-group is a 16 x 16 pixel tile of one slice.  It walks the atom bins
-// the tile can reach in the same order as a per-pixel loop would (bin column
-// bx, then bin rows by, atoms in cell order), and for each run of contiguous
-// cells loads up to 256 atom records at once, one per thread, keeps those
-// whose minimum-image distance to the tile centre allows a pixel of the tile
-// within the cutoff (a conservative bound: every pixel-atom pair it drops
-// fails the per-pixel test), and compacts them in LDS in that order
+// A work-group is a 16 x 16 pixel tile of one slice.  The atoms of the bins
+// the tile can reach are taken in the order a per-pixel loop would test them:
+// bin column bx, then bin rows by, atoms in cell order.  The bin rows of a
+// column form at most two runs of contiguous cells (before and after the
+// periodic wrap), so up to kSplatCols columns give a table of 2 kSplatCols
+// runs, one per lane, read with one vector load per lane.  The work-group
+// then loads the concatenated runs 256 atoms at a time, one per thread,
+// keeps those whose minimum-image distance to the tile centre allows a pixel
+// of the tile within the cutoff (a conservative bound: every pixel-atom pair
+// it drops fails the per-pixel test), and compacts them in LDS in that order
 // (wavefront ballots).  Every thread then sums over that list: the same pairs
 // in the same order as testing every atom of the bins, so the potential is
-// bit-identical, with the atom loop reading LDS instead of per-wavefront
-// scalar loads and branches.
+// bit-identical to a per-pixel loop over the bins.
 // A kept atom is tagged with its species slot and with whether any pixel of
 // the tile can take its periodic image (|x_atom - x_pixel| >= a/2 on an axis,
-// tested against the tile's extreme pixels with a margin far above
-// rounding).  Untagged atoms, nearly all of them, skip the minimum-image
-// selects: the signed differences square to the same bits.
+// bounded by the distance to the tile centre plus the half extent, with a
+// margin far above rounding).  Untagged atoms, nearly all of them, skip the
+// minimum-image selects: the signed differences square to the same bits.
+// The per-work-group set-up (bin bounds, wrapped first bins) runs one value
+// per lane, the four quotients and two remainders in one pass each.
 struct SplatCand {
     double x, y;
     float occ;
@@ -177,106 +181,171 @@ struct SplatCand {
     uint16_t tag;  // slot << 1 | periodic image possible
 };
 static_assert(sizeof(SplatCand) == 24, "candidate record");
+constexpr int kSplatCols = 32;
+// pixels per thread, 8 rows apart: a tile is 16 columns (x) by kSplatRows
+// rows (y)
+#ifndef MSX_SPLAT_PPT
+#define MSX_SPLAT_PPT 2
+#endif
+constexpr int kSplatPpt = MSX_SPLAT_PPT;
+constexpr int kSplatRows = 16 * kSplatPpt;
 
 template <bool TAB>
 __global__ __launch_bounds__(256) void splat_kernel(SplatArgs p) {
     __shared__ SplatCand cand[256];
     __shared__ int wcnt[4];
-    // each wavefront covers an 8 x 8 quarter of the 16 x 16 tile: a compact
-    // footprint leaves fewer lanes idle outside the radius of an atom the
-    // wavefront evaluates
+    // per run of the current column group: its first position in the
+    // concatenated list and (atom index - position)
+    __shared__ int2 runs[2 * kSplatCols];
+    // each wavefront covers 8 columns by 8 kSplatPpt rows, a quarter of the
+    // tile (a compact footprint leaves fewer lanes idle outside the radius of
+    // an atom the wavefront evaluates); a thread takes rows iy0 + 8 m
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int iy = blockIdx.x * 16 + (wv & 1) * 8 + (lane & 7);
+    const int iy0 = blockIdx.x * kSplatRows + (wv & 1) * (kSplatRows / 2) + (lane & 7);
     const int ix = blockIdx.y * 16 + (wv >> 1) * 8 + (lane >> 3);
     const int kz = blockIdx.z;
     const double dx = p.dx, dy = p.dy;
-    const double xp = ix * dx, yp = iy * dy;
+    const double xp = ix * dx;
+    double yp[kSplatPpt], v[kSplatPpt];
+#pragma unroll
+    for (int m = 0; m < kSplatPpt; ++m) {
+        yp[m] = (iy0 + 8 * m) * dy;
+        v[m] = 0.0;
+    }
     const double r2max = p.radius * p.radius;
-    // bins reached by this tile
-    const double bwx = p.bwx, bwy = p.bwy;
-    const double xlo = (double)(blockIdx.y * 16) * dx - p.radius, xhi = (double)(blockIdx.y * 16 + 15) * dx + p.radius;
-    const double ylo = (double)(blockIdx.x * 16) * dy - p.radius, yhi = (double)(blockIdx.x * 16 + 15) * dy + p.radius;
-    int bxa = (int)floor(xlo / bwx), bxb = (int)floor(xhi / bwx);
-    int bya = (int)floor(ylo / bwy), byb = (int)floor(yhi / bwy);
+    // bins reached by this tile: floor((t d -/+ radius) / w) for its first and
+    // last pixel t on each axis, one bound per lane (lanes 0-3: x low, x high,
+    // y low, y high)
+    const bool ly = (lane & 2) != 0, lhi = (lane & 1) != 0;
+    const int tl = ly ? (int)blockIdx.x * kSplatRows + (lhi ? kSplatRows - 1 : 0) : (int)blockIdx.y * 16 + (lhi ? 15 : 0);
+    const double el = (double)tl * (ly ? dy : dx) + (lhi ? p.radius : -p.radius);
+    const int ql = (int)floor(el / (ly ? p.bwy : p.bwx));
+    int bxa = __builtin_amdgcn_readlane(ql, 0), bxb = __builtin_amdgcn_readlane(ql, 1);
+    int bya = __builtin_amdgcn_readlane(ql, 2), byb = __builtin_amdgcn_readlane(ql, 3);
     if (bxb - bxa + 1 >= p.nbx) { bxa = 0; bxb = p.nbx - 1; }
     if (byb - bya + 1 >= p.nby) { bya = 0; byb = p.nby - 1; }
+    // the first bin column and row wrapped into the cell (lanes 0 and 1)
+    const int vl = (lane & 1) ? bya : bxa, ml = (lane & 1) ? p.nby : p.nbx;
+    int wl = vl % ml;
+    if (wl < 0) wl += ml;
+    const int bxw_a = __builtin_amdgcn_readlane(wl, 0), byw_a = __builtin_amdgcn_readlane(wl, 1);
+    // bin rows of the two runs of every column: bya up to the wrap, then
+    // from row 0
+    const int len1 = min(byb - bya + 1, p.nby - byw_a), len2 = byb - bya + 1 - len1;
     // the tile filter: centre and reach (half extent + cutoff, plus a margin
     // far above rounding) per axis; no filtering along an axis where the
     // reach covers half the period
-    const double cx = (double)(blockIdx.y * 16) * dx + 7.5 * dx, cy = (double)(blockIdx.x * 16) * dy + 7.5 * dy;
-    const double hx = 7.5 * dx + p.radius + 1e-6, hy = 7.5 * dy + p.radius + 1e-6;
+    constexpr double ry = 0.5 * (kSplatRows - 1);  // half the tile's extent in rows
+    const double cx = (double)(blockIdx.y * 16) * dx + 7.5 * dx, cy = (double)(blockIdx.x * kSplatRows) * dy + ry * dy;
+    const double hx = 7.5 * dx + p.radius + 1e-6, hy = ry * dy + p.radius + 1e-6;
     const bool fx = hx < 0.5 * p.a, fy = hy < 0.5 * p.b;
-    // the tile's extreme pixel positions (every xp, yp of the tile lies
-    // between them) and the image test's bounds
-    const double x0 = (double)(blockIdx.y * 16) * dx, x1 = (double)(blockIdx.y * 16 + 15) * dx;
-    const double y0 = (double)(blockIdx.x * 16) * dy, y1 = (double)(blockIdx.x * 16 + 15) * dy;
-    const double wxm = 0.5 * p.a * (1.0 - 1e-9), wym = 0.5 * p.b * (1.0 - 1e-9);
+    // the image test: no pixel of the tile takes an atom's periodic image on
+    // an axis where the atom's distance to the tile centre plus the tile's
+    // half extent stays below half the period (margin far above rounding)
+    const double gx = 0.5 * p.a * (1.0 - 1e-9) - 7.5 * dx, gy = 0.5 * p.b * (1.0 - 1e-9) - ry * dy;
     const long long cstride = 2ll * p.rt_noct * kRadPts;
     const long long tstride = (long long)p.rt_noct * (kRadPts + 3) + 1;
-    double v = 0.0;
-    for (int bx = bxa; bx <= bxb; ++bx) {
-        const int bxw = ((bx % p.nbx) + p.nbx) % p.nbx;
-        for (int by = bya; by <= byb;) {
-            // the run of bins by..byr whose wrapped rows are contiguous cells
-            const int byw = ((by % p.nby) + p.nby) % p.nby;
-            const int byr = min(byb, by + (p.nby - 1 - byw));
-            const int cell = (kz * p.nbx + bxw) * p.nby + byw;
-            const int i0 = p.cell_start[cell], i1 = p.cell_start[cell + (byr - by) + 1];
-            by = byr + 1;
-            for (int base = i0; base < i1; base += 256) {
-                __syncthreads();  // the previous list has been read
-                const int i = base + tid;
-                AtomRec at{};
-                bool keep = false;
-                if (i < i1) {
-                    at = p.atoms[i];
-                    double ex = fabs(at.x - cx), ey = fabs(at.y - cy);
-                    if (ex > 0.5 * p.a) ex = p.a - ex;
-                    if (ey > 0.5 * p.b) ey = p.b - ey;
-                    keep = (!fx || ex <= hx) && (!fy || ey <= hy);
+    for (int g0 = bxa; g0 <= bxb; g0 += kSplatCols) {
+        // run table: lane j holds run j (column g0 + j / 2, part j % 2): its
+        // first atom and its atom count
+        const int nrun = 2 * min(kSplatCols, bxb - g0 + 1);
+        int rst = 0, rcnt = 0;
+        if (lane < nrun) {
+            int bxw = bxw_a + (g0 - bxa) + (lane >> 1);  // < 2 nbx: the range spans at most nbx columns
+            if (bxw >= p.nbx) bxw -= p.nbx;
+            const int nb = (lane & 1) ? len2 : len1;
+            if (nb > 0) {
+                const int cell = (kz * p.nbx + bxw) * p.nby + ((lane & 1) ? 0 : byw_a);
+                rst = p.cell_start[cell];
+                rcnt = p.cell_start[cell + nb] - rst;
+            }
+        }
+        int incl = rcnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        const int total = __builtin_amdgcn_readlane(incl, 63);
+        // every wavefront holds the same table; the first publishes it (the
+        // previous table was last read before the previous list's barrier)
+        if (wv == 0 && lane < nrun) runs[lane] = make_int2(incl - rcnt, rst - (incl - rcnt));
+        for (int base = 0; base < total; base += 256) {
+            __syncthreads();  // the run table is written, the previous list has been read
+            const int g = base + tid;
+            AtomRec at{};
+            bool keep = false, img = false;
+            if (g < total) {
+                // the run holding position g: the last one starting at or
+                // before g (an empty run starts where the next one does)
+                int delta = 0;
+                for (int j = 0; j < nrun; ++j) {
+                    const int2 e = runs[j];
+                    if (g >= e.x) delta = e.y;
                 }
-                const unsigned long long bal = __ballot(keep);
-                if (lane == 0) wcnt[wv] = __popcll(bal);
-                __syncthreads();
-                int off = 0;
-                for (int w = 0; w < wv; ++w) off += wcnt[w];
-                if (keep) {
-                    const bool img = !(fmax(fabs(at.x - x0), fabs(at.x - x1)) < wxm) ||
-                                     !(fmax(fabs(at.y - y0), fabs(at.y - y1)) < wym);
-                    cand[off + __popcll(bal & ((1ull << lane) - 1ull))] =
-                        SplatCand{at.x, at.y, at.occ, at.Z, (uint16_t)((at.slot << 1) | (img ? 1 : 0))};
-                }
-                const int n = __builtin_amdgcn_readfirstlane(wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3]);
-                __syncthreads();
-                for (int k = 0; k < n; ++k) {
-                    const SplatCand c = cand[k];
-                    const int tag = __builtin_amdgcn_readfirstlane((int)c.tag);
-                    double dX = c.x - xp, dY = c.y - yp;
-                    if (tag & 1) {
-                        dX = fabs(dX);
-                        if (dX >= p.a / 2.0) dX -= p.a;
-                        dY = fabs(dY);
-                        if (dY >= p.b / 2.0) dY -= p.b;
+                at = p.atoms[g + delta];
+                const double rx = fabs(at.x - cx), ry = fabs(at.y - cy);
+                const double ex = rx > 0.5 * p.a ? p.a - rx : rx, ey = ry > 0.5 * p.b ? p.b - ry : ry;
+                keep = (!fx || ex <= hx) && (!fy || ey <= hy);
+                img = !(rx < gx) || !(ry < gy);
+            }
+            const unsigned long long bal = __ballot(keep);
+            if (lane == 0) wcnt[wv] = __popcll(bal);
+            __syncthreads();
+            int off = 0;
+            for (int w = 0; w < wv; ++w) off += wcnt[w];
+            if (keep) {
+                cand[off + __popcll(bal & ((1ull << lane) - 1ull))] =
+                    SplatCand{at.x, at.y, at.occ, at.Z, (uint16_t)((at.slot << 1) | (img ? 1 : 0))};
+            }
+            const int n = __builtin_amdgcn_readfirstlane(wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3]);
+            __syncthreads();
+            for (int k = 0; k < n; ++k) {
+                const SplatCand c = cand[k];
+                const int tag = __builtin_amdgcn_readfirstlane((int)c.tag);
+                double dX = c.x - xp, dY[kSplatPpt];
+#pragma unroll
+                for (int m = 0; m < kSplatPpt; ++m) dY[m] = c.y - yp[m];
+                if (tag & 1) {
+                    dX = fabs(dX);
+                    if (dX >= p.a / 2.0) dX -= p.a;
+#pragma unroll
+                    for (int m = 0; m < kSplatPpt; ++m) {
+                        dY[m] = fabs(dY[m]);
+                        if (dY[m] >= p.b / 2.0) dY[m] -= p.b;
                     }
-                    const double r2 = fma(dX, dX, dY * dY);
-                    if (r2 < r2max) {
-                        double vz;
-                        if constexpr (TAB) {
-                            const int slot = tag >> 1;
-                            vz = radial_interp(p.rcoef + slot * cstride, p.rtab + slot * tstride + tstride - 1, r2);
-                        } else {
-                            vz = d_vz(c.Z, sqrt(r2), p.table, p.formula);
+                }
+#pragma unroll
+                for (int m = 0; m < kSplatPpt; ++m) {
+                    const double r2 = fma(dX, dX, dY[m] * dY[m]);
+                    if constexpr (TAB) {
+                        const bool in = r2 < r2max, cl = r2 < 1.0e-20;
+                        // V_Z(r2) from the species' table: interval and
+                        // position from the bits of r2 (every lane reads a
+                        // valid interval)
+                        const unsigned long long u = (unsigned long long)__double_as_longlong(r2);
+                        const double2* cs = p.rcoef + (tag >> 1) * cstride;
+                        const unsigned idx = (in && !cl) ? rad_interval(u) : 0u;
+                        const double2 c01 = cs[2 * idx], c23 = cs[2 * idx + 1];
+                        if (in) {
+                            double vz = rad_cubic(c01, c23, rad_position(u));
+                            if (cl) vz = p.rtab[(tag >> 1) * tstride + tstride - 1];
+                            v[m] += (double)c.occ * vz;
                         }
-                        v += (double)c.occ * vz;
+                    } else {
+                        if (r2 < r2max) v[m] += (double)c.occ * d_vz(c.Z, sqrt(r2), p.table, p.formula);
                     }
                 }
             }
         }
     }
-    if (ix < p.nx && iy < p.ny) {
-        const long long o = ((long long)kz * p.nx + ix) * p.ny + iy;
-        if (p.v64) p.v64[o] = v;
-        if (p.v32) p.v32[o] = (float)v;
+#pragma unroll
+    for (int m = 0; m < kSplatPpt; ++m) {
+        const int iy = iy0 + 8 * m;
+        if (ix < p.nx && iy < p.ny) {
+            const long long o = ((long long)kz * p.nx + ix) * p.ny + iy;
+            if (p.v64) p.v64[o] = v[m];
+            if (p.v32) p.v32[o] = (float)v[m];
+        }
     }
 }
 
@@ -286,7 +355,7 @@ hipError_t launch_splat(const SplatArgs& args, hipStream_t s) {
     a.dy = a.b / a.ny;
     a.bwx = a.a / a.nbx;
     a.bwy = a.b / a.nby;
-    dim3 grid((a.ny + 15) / 16, (a.nx + 15) / 16, a.nslices);
+    dim3 grid((a.ny + kSplatRows - 1) / kSplatRows, (a.nx + 15) / 16, a.nslices);
     if (a.rtab) hipLaunchKernelGGL(splat_kernel<true>, grid, dim3(256), 0, s, a);
     else hipLaunchKernelGGL(splat_kernel<false>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
