@@ -51,6 +51,27 @@ def test_fp32_table_splat_within_fp32_resolution(radius, table, formula):
     ctx.close()
 
 
+@pytest.mark.parametrize("cell,nx,ny,radius", [((120.0, 90.0, 4.0), 24, 20, 1.5), ((47.0, 150.0, 4.0), 10, 40, 1.2)])
+def test_coarse_grid_tiles_reach_many_bins(cell, nx, ny, radius):
+    """Coarse pixels and a small cutoff: a 16 x 32 pixel tile then reaches
+    more than 32 bin columns (the bins are at least the cutoff wide), so the
+    splat walks its bins in several column groups, and along the other axis
+    the tile covers the whole period (every bin row, no tile filter).  Both
+    precisions against the oracle's per-pixel sum over every atom."""
+    ctx = msx.Context(0)
+    rng = np.random.default_rng(nx + ny)
+    at = specimen(rng, 300, cell, [6, 8, 38, 79])
+    V64 = ctx.potential(at, cell, nx, ny, 2, radius, prec=msx.F64)
+    V32 = ctx.potential(at, cell, nx, ny, 2, radius, prec=msx.F32)
+    xyz = np.stack([at["x"], at["y"], at["z"]], 1)
+    Vo = O.potential(at["Z"], xyz, cell, nx, ny, 2, radius, occ=at["occ"])
+    assert np.abs(Vo).max() > 0
+    e64, e32 = rel_l2(V64, Vo), rel_l2(V32, Vo)
+    print(f"cell {cell[:2]} grid {nx}x{ny} radius {radius}: FP64 {e64:.1e}, FP32 {e32:.1e} rel-L2 vs oracle")
+    assert e64 < 1e-13 and e32 < 1e-7
+    ctx.close()
+
+
 def test_potential_edge_cases():
     """No atoms, or only atoms outside the slab: a zero potential in both
     precisions (no species, so no radial tables); an element outside 1..103
