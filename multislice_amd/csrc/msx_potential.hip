@@ -154,13 +154,15 @@ hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
 // TAB: V_Z from the radial tables (FP32 output); otherwise the exact FP64
 // formula per pair (separate instantiations: the table kernel carries none of
 // the Bessel code).
-// A work-group is a 16 x 16 pixel tile of one slice.  The atoms of the bins
-// the tile can reach are taken in the order a per-pixel loop would test them:
-// bin column bx, then bin rows by, atoms in cell order.  The bin rows of a
-// column form at most two runs of contiguous cells (before and after the
-// periodic wrap), so up to kSplatCols columns give a table of 2 kSplatCols
-// runs, one per lane, read with one vector load per lane.  The work-group
-// then loads the concatenated runs 256 atoms at a time, one per thread,
+// A work-group is a tile of 16 columns (x) by kSplatRows rows (y) of one
+// slice, two pixels per thread.  The atoms of the bins the tile can reach are
+// taken in the order a per-pixel loop would test them: bin column bx, then
+// bin rows by, atoms in cell order.  The bin rows of a column form at most
+// two runs of contiguous cells (before and after the periodic wrap), so up to
+// kSplatCols columns give a table of 2 kSplatCols runs, one per lane, read
+// with one vector load per lane and published in LDS as (start in the
+// concatenated list, atom offset).  The work-group then loads the
+// concatenated runs 256 atoms at a time, one per thread,
 // keeps those whose minimum-image distance to the tile centre allows a pixel
 // of the tile within the cutoff (a conservative bound: every pixel-atom pair
 // it drops fails the per-pixel test), and compacts them in LDS in that order
