@@ -155,7 +155,7 @@ hipError_t launch_radial_tables(const RadArgs& a, hipStream_t s) {
 // formula per pair (separate instantiations: the table kernel carries none of
 // the Bessel code).
 // A work-group is a tile of 16 columns (x) by kSplatRows rows (y) of one
-// slice, two pixels per thread.  The atoms of the bins the tile can reach are
+// slice, kSplatPpt pixels per thread (two).  The atoms of the bins the tile can reach are
 // taken in the order a per-pixel loop would test them: bin column bx, then
 // bin rows by, atoms in cell order.  The bin rows of a column form at most
 // two runs of contiguous cells (before and after the periodic wrap), so up to
