@@ -3,8 +3,8 @@
 // detector finishing reduction.
 //
 // Differences from the reference kernel, by design:
-//  * atoms are binned per (slice, lateral bin) on the host, bins at least
-//    `radius` wide, so a 16x16 pixel tile only visits the bins within reach
+//  * atoms are binned per (slice, lateral bin) on the host or the GPU, bins
+//    at least `radius` wide, so a pixel tile only visits the bins within reach
 //    (the reference loops every pixel over every atom of the slice);
 //  * the whole slab is one launch over (tile, slice) instead of one launch per
 //    4608-atom constant-memory batch per slice;
